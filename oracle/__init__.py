@@ -1,0 +1,323 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so built from nuto_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as the checker and by
+bench.py's cpu_baseline leg. The product package (nuto_amd) never imports it.
+
+The oracle restates NuTo's assembly path (SimpleAssembler.cpp:38-156 and the files listed at the
+top of nuto_oracle.cpp). Element types: 0 quad4, 1 hex8, 2 tet4, 3 tet10, 4 hex20.
+Integrand kinds: 0 MomentumBalance + LinearElastic, 1 GradientDamage<3>,
+2 MomentumBalance<3> + LocalIsotropicDamage<3>.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+
+QUAD4, HEX8, TET4, TET10, HEX20 = 0, 1, 2, 3, 4
+NUM_NODES = {QUAD4: 4, HEX8: 8, TET4: 4, TET10: 10, HEX20: 20}
+DIM = {QUAD4: 2, HEX8: 3, TET4: 3, TET10: 3, HEX20: 3}
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int),
+        ("dim", ctypes.c_int),
+        ("E", ctypes.c_double),
+        ("nu", ctypes.c_double),
+        ("plane_state", ctypes.c_int),
+        ("k", ctypes.c_double),
+        ("kappa0", ctypes.c_double),
+        ("beta", ctypes.c_double),
+        ("alpha", ctypes.c_double),
+        ("c", ctypes.c_double),
+        ("damage_application", ctypes.c_int),
+    ]
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    import subprocess
+
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ctypes.CDLL(_LIB)
+        P = ctypes.c_void_p
+        dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+        ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+        L.orc_mesh_quads.restype = P
+        L.orc_mesh_bricks.restype = P
+        L.orc_mesh_from_arrays.restype = P
+        L.orc_mesh_from_arrays.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, ip]
+        L.orc_mesh_free.argtypes = [P]
+        L.orc_mesh_scale.argtypes = [P, ctypes.c_double]
+        L.orc_mesh_info.argtypes = [P, ip]
+        L.orc_mesh_coords.argtypes = [P, dp]
+        L.orc_mesh_conn.argtypes = [P, ip]
+        L.orc_mesh_add_dof.argtypes = [P, ctypes.c_int, ctypes.c_int]
+        L.orc_dof_info.argtypes = [P, ctypes.c_int, ip]
+        L.orc_dof_conn.argtypes = [P, ctypes.c_int, ip]
+        L.orc_dof_numbers.argtypes = [P, ctypes.c_int, ip]
+        L.orc_problem.restype = P
+        L.orc_problem.argtypes = [P, ctypes.POINTER(Params), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_problem_free.argtypes = [P]
+        L.orc_problem_set_theta.argtypes = [P, dp, dp, dp]
+        L.orc_problem_ndof.argtypes = [P, ctypes.c_int]
+        L.orc_problem_nip.argtypes = [P]
+        L.orc_problem_assemble.restype = ctypes.c_longlong
+        L.orc_problem_assemble.argtypes = [P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_problem_R.argtypes = [P, dp, P]
+        L.orc_problem_K.argtypes = [P, ip, ip, P, P]
+        L.orc_problem_cell.argtypes = [P, ctypes.c_int, P, P, P, P, P]
+        L.orc_problem_cell_dofs.argtypes = [P, ctypes.c_int, ip]
+        L.orc_mises.restype = ctypes.c_double
+        L.orc_damage_exp.restype = ctypes.c_double
+        L.orc_mock_assemble.restype = ctypes.c_longlong
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def legendre_roots(n):
+    out = np.zeros(n)
+    lib().orc_legendre_roots(n, out.ctypes.data_as(ctypes.c_void_p))
+    return out
+
+
+def gauss_1d(n):
+    w = np.zeros(n)
+    x = np.zeros(n)
+    lib().orc_gauss_1d(n, _ptr(w), _ptr(x))
+    return w, x
+
+
+def rule(kind, dim, n):
+    """kind 0: tensor Gauss with n points per direction; kind 1: tetrahedron rule of order n."""
+    num = lib().orc_rule(kind, dim, n, None, None)
+    xi = np.zeros((num, 3))
+    w = np.zeros(num)
+    lib().orc_rule(kind, dim, n, _ptr(xi), _ptr(w))
+    return xi, w
+
+
+def shape(etype, xi):
+    n = NUM_NODES[etype]
+    N = np.zeros(n)
+    dN = np.zeros((n, DIM[etype]))
+    x = np.zeros(3)
+    x[: len(xi)] = xi
+    lib().orc_shape(etype, _ptr(x), _ptr(N), _ptr(dN))
+    return N, dN
+
+
+def local_coords(etype):
+    out = np.zeros((NUM_NODES[etype], 3))
+    lib().orc_local_coords(etype, _ptr(out))
+    return out
+
+
+def jacobian(X, dN):
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    dN = np.ascontiguousarray(dN, dtype=np.float64)
+    n, dim = dN.shape
+    J = np.zeros((dim, dim))
+    inv = np.zeros((dim, dim))
+    det = ctypes.c_double()
+    lib().orc_jacobian(dim, n, _ptr(X), _ptr(dN), _ptr(J), _ptr(inv), ctypes.byref(det))
+    return J, inv, det.value
+
+
+def led(E, nu, app, strain, omega):
+    s = np.zeros(6)
+    D = np.zeros((6, 6))
+    w = np.zeros(6)
+    strain = np.ascontiguousarray(strain, dtype=np.float64)
+    lib().orc_led_stress(ctypes.c_double(E), ctypes.c_double(nu), app, _ptr(strain), ctypes.c_double(omega),
+                         _ptr(s), _ptr(D), _ptr(w))
+    return s, D, w
+
+
+def mises(nu, k, strain):
+    d = np.zeros(6)
+    strain = np.ascontiguousarray(strain, dtype=np.float64)
+    v = lib().orc_mises(ctypes.c_double(nu), ctypes.c_double(k), _ptr(strain), _ptr(d))
+    return v, d
+
+
+def damage_exp(k0, beta, alpha, kappa):
+    d = ctypes.c_double()
+    v = lib().orc_damage_exp(ctypes.c_double(k0), ctypes.c_double(beta), ctypes.c_double(alpha),
+                             ctypes.c_double(kappa), ctypes.byref(d))
+    return v, d.value
+
+
+def elastic_c(dim, E, nu, plane_state=0):
+    n = 3 if dim == 2 else 6
+    C = np.zeros((n, n))
+    lib().orc_elastic_c(dim, ctypes.c_double(E), ctypes.c_double(nu), plane_state, _ptr(C))
+    return C
+
+
+class Mesh:
+    """MeshFem restatement: coordinate nodes/elements plus dof layers (AddDofInterpolation)."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    @classmethod
+    def quads(cls, nx, ny):
+        return cls(lib().orc_mesh_quads(nx, ny))
+
+    @classmethod
+    def bricks(cls, nx, ny, nz):
+        return cls(lib().orc_mesh_bricks(nx, ny, nz))
+
+    @classmethod
+    def from_arrays(cls, coord_type, X, conn):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        conn = np.ascontiguousarray(conn, dtype=np.int32)
+        return cls(lib().orc_mesh_from_arrays(coord_type, X.shape[0], X, conn.shape[0], conn))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_mesh_free(self.h)
+            self.h = None
+
+    def scale(self, s):
+        lib().orc_mesh_scale(self.h, ctypes.c_double(s))
+
+    def info(self):
+        o = np.zeros(5, dtype=np.int32)
+        lib().orc_mesh_info(self.h, o)
+        return dict(dim=int(o[0]), coord_type=int(o[1]), n_cells=int(o[2]), n_nodes=int(o[3]), n_layers=int(o[4]))
+
+    def coords(self):
+        i = self.info()
+        X = np.zeros((i["n_nodes"], i["dim"]))
+        lib().orc_mesh_coords(self.h, X)
+        return X
+
+    def conn(self):
+        i = self.info()
+        c = np.zeros((i["n_cells"], NUM_NODES[i["coord_type"]]), dtype=np.int32)
+        lib().orc_mesh_conn(self.h, c)
+        return c
+
+    def add_dof(self, etype, ncomp):
+        r = lib().orc_mesh_add_dof(self.h, etype, ncomp)
+        if r < 0:
+            raise RuntimeError("AddDofInterpolation failed (shape mismatch)")
+        return r
+
+    def dof_layer(self, layer):
+        o = np.zeros(3, dtype=np.int32)
+        lib().orc_dof_info(self.h, layer, o)
+        etype, ncomp, nnodes = int(o[0]), int(o[1]), int(o[2])
+        conn = np.zeros((self.info()["n_cells"], NUM_NODES[etype]), dtype=np.int32)
+        lib().orc_dof_conn(self.h, layer, conn)
+        num = np.zeros((nnodes, ncomp), dtype=np.int32)
+        lib().orc_dof_numbers(self.h, layer, num)
+        return dict(type=etype, ncomp=ncomp, n_nodes=nnodes, conn=conn, numbers=num)
+
+
+class Problem:
+    """A cell group + integrand + integration rule, assembled by the one-thread restatement."""
+
+    def __init__(self, mesh, params, rule_kind, rule_n, layer0, layer1=-1):
+        self.mesh = mesh  # keep alive
+        self.p = params
+        self.h = lib().orc_problem(mesh.h, ctypes.byref(params), rule_kind, rule_n, layer0, layer1)
+        self.ntypes = 2 if layer1 >= 0 else 1
+        self.ndof = [lib().orc_problem_ndof(self.h, t) for t in range(self.ntypes)]
+        self.nip = lib().orc_problem_nip(self.h)
+        self.ncells = mesh.info()["n_cells"]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_problem_free(self.h)
+            self.h = None
+
+    def set_theta(self, E, ft, gf):
+        lib().orc_problem_set_theta(self.h, np.ascontiguousarray(E, np.float64), np.ascontiguousarray(ft, np.float64),
+                                    np.ascontiguousarray(gf, np.float64))
+
+    def assemble(self, u, e=None, kappa=None, want_r=True, want_k=True, nthreads=1):
+        u = np.ascontiguousarray(u, np.float64)
+        e = None if e is None else np.ascontiguousarray(e, np.float64)
+        kappa = None if kappa is None else np.ascontiguousarray(kappa, np.float64)
+        nnz = lib().orc_problem_assemble(self.h, _ptr(u), _ptr(e), _ptr(kappa), int(want_r), int(want_k), nthreads)
+        if nnz < 0:
+            raise RuntimeError("oracle assembly failed")
+        out = {}
+        ntot = sum(self.ndof)
+        if want_r:
+            R = np.zeros(ntot)
+            Rb = np.zeros(ntot)
+            lib().orc_problem_R(self.h, R, _ptr(Rb))
+            out["R"] = R
+            out["R_bound"] = Rb
+        if want_k:
+            rp = np.zeros(ntot + 1, dtype=np.int32)
+            ci = np.zeros(nnz, dtype=np.int32)
+            v = np.zeros(nnz)
+            b = np.zeros(nnz)
+            lib().orc_problem_K(self.h, rp, ci, _ptr(v), _ptr(b))
+            out.update(row_ptr=rp, col=ci, val=v, bound=b)
+        return out
+
+    def cell(self, c, u, e=None, kappa=None, want_r=True, want_k=True):
+        ntot_loc = lib().orc_problem_cell(self.h, c, _ptr(np.ascontiguousarray(u, np.float64)),
+                                          _ptr(None if e is None else np.ascontiguousarray(e, np.float64)),
+                                          _ptr(None if kappa is None else np.ascontiguousarray(kappa, np.float64)),
+                                          None, None)
+        Re = np.zeros(ntot_loc) if want_r else None
+        Ke = np.zeros((ntot_loc, ntot_loc)) if want_k else None
+        lib().orc_problem_cell(self.h, c, _ptr(np.ascontiguousarray(u, np.float64)),
+                               _ptr(None if e is None else np.ascontiguousarray(e, np.float64)),
+                               _ptr(None if kappa is None else np.ascontiguousarray(kappa, np.float64)), _ptr(Re),
+                               _ptr(Ke))
+        return Re, Ke
+
+    def cell_dofs(self, c):
+        out = np.zeros(64, dtype=np.int32)
+        n = lib().orc_problem_cell_dofs(self.h, c, out)
+        return out[:n].copy()
+
+
+def mock_assemble(numbering, Re, Ke, ndof):
+    """SimpleAssembler over mock cells with fixed local vectors/matrices (no integrand)."""
+    numbering = np.ascontiguousarray(numbering, np.int32)
+    Re = np.ascontiguousarray(Re, np.float64)
+    Ke = np.ascontiguousarray(Ke, np.float64)
+    ncells, nloc = numbering.shape
+    R = np.zeros(ndof)
+    nnz = lib().orc_mock_assemble(ncells, nloc, _ptr(numbering), _ptr(Re), _ptr(Ke), ndof, _ptr(R), None, None,
+                                  None)
+    rp = np.zeros(ndof + 1, np.int32)
+    ci = np.zeros(nnz, np.int32)
+    v = np.zeros(nnz)
+    lib().orc_mock_assemble(ncells, nloc, _ptr(numbering), _ptr(Re), _ptr(Ke), ndof, _ptr(R), _ptr(rp), _ptr(ci),
+                            _ptr(v))
+    return R, rp, ci, v
+
+
+def csr_to_dense(rp, ci, v, n, m=None):
+    m = n if m is None else m
+    A = np.zeros((n, m))
+    for r in range(n):
+        A[r, ci[rp[r]:rp[r + 1]]] = v[rp[r]:rp[r + 1]]
+    return A
