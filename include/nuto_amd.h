@@ -1,0 +1,148 @@
+/* nuto_amd: MI355X (gfx950) element assembly for NuTo's SimpleAssembler path.
+ *
+ * Plain C ABI: no exceptions cross it, every call returns an int status (NUTO_OK == 0) and the
+ * message of the last failure on a context is available from nuto_last_error().
+ *
+ * Units of data (all host arrays are row-major, all indices int32):
+ *   cell group     n_cells coordinate elements (quad4 / hex8 / tet4) + per dof type the cell dof
+ *                  numbering exactly as Cell::DofNumbering returns it (node-major, component-minor)
+ *   dof vectors    one contiguous FP64 array per dof type, indexed by global dof number
+ *   history        kappa(cellId, ip) at kappa[cellId + ip * history_ld] (Eigen column-major MatrixXd)
+ *   matrix         monolithic CSR, rows [type0 dofs ; type1 dofs], cols [type0 , type1]
+ *                  (ToEigen(K, {type0, type1}), DofMatrixSparseConvertEigen.h:40-67)
+ *
+ * Device pointers: dof vectors, history and outputs passed to the nuto_build_* calls live in
+ * device memory (e.g. torch tensors on cuda:0); kernels run on the context's stream.
+ */
+#ifndef NUTO_AMD_H
+#define NUTO_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NUTO_OK 0
+#define NUTO_ERR_ARGUMENT 1    /* bad argument (NuTo::Exception in the reference) */
+#define NUTO_ERR_STATE 2       /* call out of order, e.g. no dof numbering (SimpleAssembler.cpp:16-23) */
+#define NUTO_ERR_DEVICE 3      /* HIP runtime error */
+#define NUTO_ERR_MEMORY 4      /* device or host allocation failed */
+#define NUTO_ERR_UNSUPPORTED 5 /* combination not provided by these kernels */
+
+/* element types (interpolations): InterpolationQuadLinear, InterpolationBrickLinear,
+ * InterpolationTetrahedronLinear, InterpolationTetrahedronQuadratic, InterpolationBrickQuadratic */
+#define NUTO_QUAD4 0
+#define NUTO_HEX8 1
+#define NUTO_TET4 2
+#define NUTO_TET10 3
+#define NUTO_HEX20 4
+
+/* integrands */
+#define NUTO_MOMENTUM_LINEAR_ELASTIC 0 /* Integrands::MomentumBalance<D> + Laws::LinearElastic<D> */
+#define NUTO_GRADIENT_DAMAGE 1         /* Integrands::GradientDamage<3> (Constant interaction, exp. damage) */
+#define NUTO_MOMENTUM_LOCAL_DAMAGE 2   /* MomentumBalance<3> + Laws::LocalIsotropicDamage<3> */
+
+typedef struct nuto_ctx nuto_ctx;
+
+/* Integrand parameters. Mirrors the constructor arguments of
+ * Laws::LinearElastic(E, nu, planeState)                       LinearElastic.h:16
+ * Laws::LinearElasticDamage(E, nu, damageApplication)          LinearElasticDamage.h:32
+ * Constitutive::ModifiedMisesStrainNorm(nu, k)                 ModifiedMisesStrainNorm.h:38
+ * Constitutive::DamageLawExponential(kappa0, beta, alpha)      DamageLawExponential.h:26
+ * Integrands::GradientDamage(disp, eeq, c, ...)                GradientDamage.h:36 */
+typedef struct nuto_params
+{
+    int kind;            /* NUTO_MOMENTUM_LINEAR_ELASTIC | NUTO_GRADIENT_DAMAGE | NUTO_MOMENTUM_LOCAL_DAMAGE */
+    int dim;             /* 2 or 3 (2 only for linear elasticity) */
+    double E, nu;
+    int plane_state;     /* 0 PLANE_STRESS (reference default for LinearElastic<2>), 1 PLANE_STRAIN */
+    double k, kappa0, beta, alpha, c;
+    int damage_application; /* 0 FULL, 1 UNILATERAL (LinearElasticDamage.h:14-18) */
+} nuto_params;
+
+/* ---- set-up helpers (host only, no device needed) ------------------------------------------- */
+
+/* Structured mesh of [0,lx]x[0,ly](x[0,lz]): UnitMeshFem::CreateQuads / CreateBricks followed by
+ * UnitMeshFem::Transform(x * L) (UnitMeshFem.cpp:67-125); coord_type NUTO_TET4 splits every brick
+ * into six tetrahedra. jitter > 0 moves each node by a seeded uniform offset of up to
+ * jitter * spacing per direction. Pass coords == conn == NULL to query the sizes. */
+int nuto_mesh_structured(int coord_type, int nx, int ny, int nz, double lx, double ly, double lz, double jitter,
+                         unsigned long long seed, int* n_nodes, int* n_cells, double* coords, int* conn);
+
+/* First-touch dof layer: AddDofInterpolation (MeshFemDofConvert.cpp:164-210) numbered by
+ * DofNumbering::Build (DofNumbering.cpp:32-58). Writes per-cell dof node ranks
+ * (n_cells x nodes(dof_type)); the dof numbers are rank * ncomp + component.
+ * Pass cell_ranks == NULL to query n_dof_nodes only. */
+int nuto_dof_layer(int coord_type, int n_cells, const int* coord_conn, int dof_type, int* cell_ranks,
+                   int* n_dof_nodes);
+
+/* ---- context ------------------------------------------------------------------------------------ */
+
+/* Replaces constructing SimpleAssembler (SimpleAssembler.h:14-15); one context per device. */
+int nuto_create(nuto_ctx** ctx, int device);
+int nuto_destroy(nuto_ctx* ctx);
+const char* nuto_last_error(const nuto_ctx* ctx);
+/* Run subsequent kernels on this HIP stream (hipStream_t); NULL is the device's default (null)
+ * stream. A new context uses a stream of its own until this is called. */
+int nuto_set_stream(nuto_ctx* ctx, void* stream);
+int nuto_synchronize(nuto_ctx* ctx);
+
+/* The cell group: Group<CellInterface> of Cells built by CellStorage::AddCells over a MeshFem
+ * (CellStorage.cpp:6-16, Cell.h:14-30). coords: n_coord_nodes x dim (node-major),
+ * coord_conn: n_cells x nodes(coord_type), cell_ids: Cell::Id per cell (NULL: 0..n_cells-1). */
+int nuto_set_cells(nuto_ctx* ctx, int coord_type, int n_cells, int n_coord_nodes, const double* coords,
+                   const int* coord_conn, const int* cell_ids);
+
+/* Dof type slot 0 (displacements) or 1 (nonlocal equivalent strains): its interpolation, number
+ * of components, total dofs (numIndependent + numDependent, SimpleAssembler.cpp:158-166) and the
+ * cell dof numbering n_cells x (nodes(dof_type) * ncomp) as ElementFem::GetDofNumbering returns it
+ * (ElementFem.h:62-75). Replaces SimpleAssembler::SetDofInfo (SimpleAssembler.cpp:11-14). */
+int nuto_set_dof_type(nuto_ctx* ctx, int slot, int dof_type, int ncomp, int n_dofs, const int* cell_dofs);
+
+/* The integrand bound into the VectorFunction / MatrixFunction (CellInterface.h:19-20, Bind.h:11-22). */
+int nuto_set_integrand(nuto_ctx* ctx, const nuto_params* params);
+
+/* IntegrationTypeBase of the cells: kind 0 = IntegrationTypeTensorProduct<dim>(n, GAUSS),
+ * kind 1 = IntegrationTypeTetrahedron(order n) (IntegrationCompanion.cpp:18-52). */
+int nuto_set_rule(nuto_ctx* ctx, int kind, int n);
+
+/* Leading dimension of the history matrix (rows of GradientDamage::mKappas, GradientDamage.h:137). */
+int nuto_set_history_ld(nuto_ctx* ctx, int history_ld);
+
+/* Per-cell material fields (E, f_t, G_f) for gradient damage: E replaces E, kappa0 = f_t / E and
+ * beta = f_t / G_f (the DamageLawExponential(Material::Softening) mapping, DamageLawExponential.h:32-37).
+ * Host arrays of n_cells, or NULL to return to the scalar parameters. */
+int nuto_set_material_fields(nuto_ctx* ctx, const double* E, const double* ft, const double* gf);
+
+/* Sparsity of BuildMatrix's result (setFromTriplets pattern: every dof pair of every cell, explicit
+ * zeros kept; SimpleAssembler.cpp:100-156), built once and kept on the device. */
+int nuto_build_pattern(nuto_ctx* ctx, long long* nnz, int* n_rows);
+int nuto_copy_pattern(nuto_ctx* ctx, int* row_ptr, int* col);
+int nuto_pattern_device(nuto_ctx* ctx, const int** row_ptr, const int** col);
+
+/* SimpleAssembler::BuildVector (SimpleAssembler.cpp:38-65): R (length n_rows) from the dof vectors
+ * u (slot 0) and e (slot 1, NULL for single-field integrands) and history kappa (NULL if unused). */
+int nuto_build_vector(nuto_ctx* ctx, const double* u, const double* e, const double* kappa, double* R);
+
+/* SimpleAssembler::BuildMatrix (SimpleAssembler.cpp:100-156): K values (length nnz) in the pattern. */
+int nuto_build_matrix(nuto_ctx* ctx, const double* u, const double* e, const double* kappa, double* K);
+
+/* BuildVector and BuildMatrix of the same state in one pass over the cells. */
+int nuto_build_vector_matrix(nuto_ctx* ctx, const double* u, const double* e, const double* kappa, double* R,
+                             double* K);
+
+/* TimeDependentProblem::UpdateHistory -> GradientDamage::Update (TimeDependentProblem.cpp:78-85,
+ * GradientDamage.h:122-125) or EvolutionImplicit::Update (LocalIsotropicDamage.h:130-133):
+ * kappa(cell, ip) = max(kappa(cell, ip), eeq(ip)), in place. */
+int nuto_update_history(nuto_ctx* ctx, const double* u, const double* e, double* kappa);
+
+/* ---- device memory helpers for hosts without a HIP toolchain -------------------------------- */
+int nuto_device_alloc(nuto_ctx* ctx, void** ptr, unsigned long long bytes);
+int nuto_device_free(nuto_ctx* ctx, void* ptr);
+int nuto_copy_to_device(nuto_ctx* ctx, void* dst, const void* src, unsigned long long bytes);
+int nuto_copy_to_host(nuto_ctx* ctx, void* dst, const void* src, unsigned long long bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NUTO_AMD_H */

@@ -1,0 +1,269 @@
+"""ctypes binding of libnuto_amd.so (the C ABI in include/nuto_amd.h).
+
+Device arrays are torch tensors on cuda:0 (float64 / int32); host arrays are numpy. There is no
+CPU fallback: if the HIP library is missing or no GPU is present the calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnuto_amd.so")
+
+QUAD4, HEX8, TET4, TET10, HEX20 = 0, 1, 2, 3, 4
+NUM_NODES = {QUAD4: 4, HEX8: 8, TET4: 4, TET10: 10, HEX20: 20}
+DIM = {QUAD4: 2, HEX8: 3, TET4: 3, TET10: 3, HEX20: 3}
+
+MOMENTUM_LINEAR_ELASTIC, GRADIENT_DAMAGE, MOMENTUM_LOCAL_DAMAGE = 0, 1, 2
+PLANE_STRESS, PLANE_STRAIN = 0, 1
+FULL, UNILATERAL = 0, 1
+
+_ERRORS = {1: "argument", 2: "state", 3: "device", 4: "memory", 5: "unsupported"}
+
+
+class NutoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"nuto_amd error {code} ({_ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int),
+        ("dim", ctypes.c_int),
+        ("E", ctypes.c_double),
+        ("nu", ctypes.c_double),
+        ("plane_state", ctypes.c_int),
+        ("k", ctypes.c_double),
+        ("kappa0", ctypes.c_double),
+        ("beta", ctypes.c_double),
+        ("alpha", ctypes.c_double),
+        ("c", ctypes.c_double),
+        ("damage_application", ctypes.c_int),
+    ]
+
+
+def gradient_damage_params(E=30000.0, nu=0.2, k=10.0, kappa0=1e-4, beta=100.0, alpha=0.99, c=4.0,
+                           damage_application=FULL):
+    return Params(kind=GRADIENT_DAMAGE, dim=3, E=E, nu=nu, plane_state=0, k=k, kappa0=kappa0, beta=beta,
+                  alpha=alpha, c=c, damage_application=damage_application)
+
+
+def linear_elastic_params(dim, E, nu, plane_state=PLANE_STRESS):
+    return Params(kind=MOMENTUM_LINEAR_ELASTIC, dim=dim, E=E, nu=nu, plane_state=plane_state)
+
+
+def local_damage_params(E=30000.0, nu=0.2, k=10.0, kappa0=1e-4, beta=100.0, alpha=0.99,
+                        damage_application=FULL):
+    return Params(kind=MOMENTUM_LOCAL_DAMAGE, dim=3, E=E, nu=nu, plane_state=0, k=k, kappa0=kappa0, beta=beta,
+                  alpha=alpha, c=0.0, damage_application=damage_application)
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        # torch brings its own libamdhip64 (SONAME libamdhip64.so.7). Loading it first makes the
+        # dynamic linker bind libnuto_amd.so to that same runtime instead of a second copy from
+        # /opt/rocm, so both share one HIP runtime, device context and stream namespace.
+        import torch  # noqa: F401
+
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+                              " (make -C nuto_amd/csrc)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ip, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+        L.nuto_last_error.restype = ctypes.c_char_p
+        L.nuto_last_error.argtypes = [vp]
+        L.nuto_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int]
+        L.nuto_destroy.argtypes = [vp]
+        L.nuto_set_stream.argtypes = [vp, vp]
+        L.nuto_synchronize.argtypes = [vp]
+        L.nuto_mesh_structured.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong, ip,
+                                           ip, vp, vp]
+        L.nuto_dof_layer.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp, ip]
+        L.nuto_set_cells.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp]
+        L.nuto_set_dof_type.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
+        L.nuto_set_integrand.argtypes = [vp, ctypes.POINTER(Params)]
+        L.nuto_set_rule.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.nuto_set_history_ld.argtypes = [vp, ctypes.c_int]
+        L.nuto_set_material_fields.argtypes = [vp, vp, vp, vp]
+        L.nuto_build_pattern.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ip]
+        L.nuto_copy_pattern.argtypes = [vp, vp, vp]
+        L.nuto_pattern_device.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp)]
+        for f in ("nuto_build_vector", "nuto_build_matrix"):
+            getattr(L, f).argtypes = [vp, vp, vp, vp, vp]
+        L.nuto_update_history.argtypes = [vp, vp, vp, vp]
+        L.nuto_build_vector_matrix.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.nuto_device_alloc.argtypes = [vp, ctypes.POINTER(vp), ctypes.c_ulonglong]
+        L.nuto_device_free.argtypes = [vp, vp]
+        L.nuto_copy_to_device.argtypes = [vp, vp, vp, ctypes.c_ulonglong]
+        L.nuto_copy_to_host.argtypes = [vp, vp, vp, ctypes.c_ulonglong]
+        _lib = L
+    return _lib
+
+
+def _np_ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _dev_ptr(t):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+# ---- set-up helpers (host) --------------------------------------------------------------------
+def structured_mesh(coord_type, nx, ny, nz=0, lengths=(1.0, 1.0, 1.0), jitter=0.0, seed=0):
+    """Structured mesh: returns (coords n_nodes x dim, conn n_cells x nodes)."""
+    L = lib()
+    nn, nc = ctypes.c_int(), ctypes.c_int()
+    args = (coord_type, nx, ny, nz, float(lengths[0]), float(lengths[1]), float(lengths[2]), float(jitter), seed)
+    r = L.nuto_mesh_structured(*args, ctypes.byref(nn), ctypes.byref(nc), None, None)
+    if r:
+        raise NutoError(r, "invalid structured mesh arguments")
+    dim = DIM[coord_type]
+    X = np.zeros((nn.value, dim))
+    C = np.zeros((nc.value, NUM_NODES[coord_type]), np.int32)
+    r = L.nuto_mesh_structured(*args, None, None, _np_ptr(X), _np_ptr(C))
+    if r:
+        raise NutoError(r, "structured mesh failed")
+    return X, C
+
+
+def dof_layer(coord_type, conn, dof_type):
+    """First-touch dof node ranks per cell (n_cells x nodes(dof_type)) and the dof node count."""
+    L = lib()
+    conn = np.ascontiguousarray(conn, np.int32)
+    n = ctypes.c_int()
+    r = L.nuto_dof_layer(coord_type, conn.shape[0], _np_ptr(conn), dof_type, None, ctypes.byref(n))
+    if r:
+        raise NutoError(r, "dof layer failed")
+    ranks = np.zeros((conn.shape[0], NUM_NODES[dof_type]), np.int32)
+    r = L.nuto_dof_layer(coord_type, conn.shape[0], _np_ptr(conn), dof_type, _np_ptr(ranks), ctypes.byref(n))
+    if r:
+        raise NutoError(r, "dof layer failed")
+    return ranks, n.value
+
+
+def cell_dofs(ranks, ncomp):
+    """Cell dof numbering (node-major, component-minor) from per-cell node ranks."""
+    ranks = np.asarray(ranks, np.int32)
+    return (ranks[:, :, None] * ncomp + np.arange(ncomp, dtype=np.int32)[None, None, :]).reshape(
+        ranks.shape[0], -1).astype(np.int32)
+
+
+class Context:
+    """One assembly context on one device (the C ABI nuto_ctx)."""
+
+    def __init__(self, device=0, use_torch_stream=True):
+        import torch
+
+        if not torch.cuda.is_available():
+            raise RuntimeError("nuto_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+        self._torch = torch
+        L = lib()
+        h = ctypes.c_void_p()
+        r = L.nuto_create(ctypes.byref(h), device)
+        if r:
+            raise NutoError(r, "nuto_create failed")
+        self.h = h
+        self.device = device
+        if use_torch_stream:
+            self.set_stream(torch.cuda.current_stream(device).cuda_stream)
+        self.n_rows = 0
+        self.nnz = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().nuto_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _chk(self, r):
+        if r:
+            raise NutoError(r, lib().nuto_last_error(self.h).decode())
+
+    def set_stream(self, stream):
+        # 0 / None selects the null stream (torch's default stream)
+        self._chk(lib().nuto_set_stream(self.h, ctypes.c_void_p(stream) if stream else None))
+
+    def synchronize(self):
+        self._chk(lib().nuto_synchronize(self.h))
+
+    def set_cells(self, coord_type, coords, conn, cell_ids=None):
+        coords = np.ascontiguousarray(coords, np.float64)
+        conn = np.ascontiguousarray(conn, np.int32)
+        cid = None if cell_ids is None else np.ascontiguousarray(cell_ids, np.int32)
+        self.n_cells = conn.shape[0]
+        self._chk(lib().nuto_set_cells(self.h, coord_type, conn.shape[0], coords.shape[0], _np_ptr(coords),
+                                       _np_ptr(conn), _np_ptr(cid)))
+
+    def set_dof_type(self, slot, dof_type, ncomp, n_dofs, cell_dofs_):
+        cd = np.ascontiguousarray(cell_dofs_, np.int32)
+        self._chk(lib().nuto_set_dof_type(self.h, slot, dof_type, ncomp, n_dofs, _np_ptr(cd)))
+
+    def set_integrand(self, params):
+        self._chk(lib().nuto_set_integrand(self.h, ctypes.byref(params)))
+
+    def set_rule(self, kind, n):
+        self._chk(lib().nuto_set_rule(self.h, kind, n))
+
+    def set_history_ld(self, ld):
+        self._chk(lib().nuto_set_history_ld(self.h, ld))
+
+    def set_material_fields(self, E=None, ft=None, gf=None):
+        a = [None if v is None else np.ascontiguousarray(v, np.float64) for v in (E, ft, gf)]
+        self._chk(lib().nuto_set_material_fields(self.h, *[_np_ptr(v) for v in a]))
+
+    def build_pattern(self):
+        nnz, nr = ctypes.c_longlong(), ctypes.c_int()
+        self._chk(lib().nuto_build_pattern(self.h, ctypes.byref(nnz), ctypes.byref(nr)))
+        self.nnz, self.n_rows = nnz.value, nr.value
+        return self.nnz, self.n_rows
+
+    def pattern(self):
+        rp = np.zeros(self.n_rows + 1, np.int32)
+        ci = np.zeros(self.nnz, np.int32)
+        self._chk(lib().nuto_copy_pattern(self.h, _np_ptr(rp), _np_ptr(ci)))
+        return rp, ci
+
+    def pattern_device(self):
+        """(row_ptr, col) as torch int32 tensors on the device (copied through the host)."""
+        torch = self._torch
+        rp, ci = self.pattern()
+        return torch.from_numpy(rp).cuda(), torch.from_numpy(ci).cuda()
+
+    def _new(self, n):
+        return self._torch.empty(n, dtype=self._torch.float64, device="cuda")
+
+    def build_vector(self, u, e=None, kappa=None, out=None):
+        R = self._new(self.n_rows) if out is None else out
+        self._chk(lib().nuto_build_vector(self.h, _dev_ptr(u), _dev_ptr(e), _dev_ptr(kappa), _dev_ptr(R)))
+        return R
+
+    def build_matrix(self, u, e=None, kappa=None, out=None):
+        K = self._new(self.nnz) if out is None else out
+        self._chk(lib().nuto_build_matrix(self.h, _dev_ptr(u), _dev_ptr(e), _dev_ptr(kappa), _dev_ptr(K)))
+        return K
+
+    def build_vector_matrix(self, u, e=None, kappa=None, R=None, K=None):
+        R = self._new(self.n_rows) if R is None else R
+        K = self._new(self.nnz) if K is None else K
+        self._chk(lib().nuto_build_vector_matrix(self.h, _dev_ptr(u), _dev_ptr(e), _dev_ptr(kappa), _dev_ptr(R),
+                                                 _dev_ptr(K)))
+        return R, K
+
+    def update_history(self, u, e, kappa):
+        self._chk(lib().nuto_update_history(self.h, _dev_ptr(u), _dev_ptr(e), _dev_ptr(kappa)))
+        return kappa

@@ -1,0 +1,290 @@
+#include "mesh.hpp"
+
+#include <algorithm>
+#include <climits>
+#include <unordered_map>
+
+#include "elements.hpp"
+
+namespace nuto_amd
+{
+
+uint64_t SplitMix64(uint64_t seed, uint64_t counter)
+{
+    uint64_t z = seed + (counter + 1) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+double UniformAt(uint64_t seed, uint64_t counter)
+{
+    return (double)(SplitMix64(seed, counter) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+bool StructuredMesh(int coordType, int nx, int ny, int nz, double lx, double ly, double lz, double jitter,
+                    uint64_t seed, std::vector<double>& X, std::vector<int>& conn, std::string& err)
+{
+    if (coordType != kQuad4 && coordType != kHex8 && coordType != kTet4)
+    {
+        err = "structured meshes are built from quad4, hex8 or tet4 coordinate elements";
+        return false;
+    }
+    const bool is2d = coordType == kQuad4;
+    if (nx < 1 || ny < 1 || (!is2d && nz < 1))
+    {
+        err = "structured mesh needs at least one cell per direction";
+        return false;
+    }
+    if (is2d)
+        nz = 0;
+    const int ex = nx + 1, ey = ny + 1, ez = is2d ? 1 : nz + 1;
+    const int dim = is2d ? 2 : 3;
+    const long long nNodes = (long long)ex * ey * ez;
+    if (nNodes > INT_MAX / 3)
+    {
+        err = "structured mesh too large for int32 node ids";
+        return false;
+    }
+    X.assign((size_t)nNodes * dim, 0.0);
+    const double L[3] = {lx, ly, lz};
+    const int n[3] = {nx, ny, nz};
+    for (int iz = 0; iz < ez; ++iz)
+        for (int iy = 0; iy < ey; ++iy)
+            for (int ix = 0; ix < ex; ++ix)
+            {
+                const long long id = ix + (long long)iy * ex + (long long)iz * ex * ey;
+                const int idx[3] = {ix, iy, iz};
+                for (int d = 0; d < dim; d++)
+                {
+                    // unit-cube coordinate first, then the Transform scaling, as the reference does
+                    double v = static_cast<double>(idx[d]) / n[d];
+                    v = v * L[d];
+                    if (jitter > 0.0)
+                        v += (2.0 * UniformAt(seed, (uint64_t)id * 3 + d) - 1.0) * jitter * (L[d] / n[d]);
+                    X[(size_t)id * dim + d] = v;
+                }
+            }
+    conn.clear();
+    auto id = [&](int a, int b, int c) { return a + b * ex + c * ex * ey; };
+    if (is2d)
+    {
+        conn.reserve((size_t)nx * ny * 4);
+        for (int iy = 0; iy < ny; ++iy)
+            for (int ix = 0; ix < nx; ++ix)
+                conn.insert(conn.end(), {id(ix, iy, 0), id(ix + 1, iy, 0), id(ix + 1, iy + 1, 0), id(ix, iy + 1, 0)});
+        return true;
+    }
+    const bool tets = coordType == kTet4;
+    conn.reserve((size_t)nx * ny * nz * (tets ? 24 : 8));
+    // six tetrahedra around the 0-6 diagonal; the split is conforming between neighbouring bricks
+    static const int kuhn[6][4] = {{0, 1, 2, 6}, {0, 2, 3, 6}, {0, 3, 7, 6}, {0, 7, 4, 6}, {0, 4, 5, 6}, {0, 5, 1, 6}};
+    for (int iz = 0; iz < nz; ++iz)
+        for (int iy = 0; iy < ny; ++iy)
+            for (int ix = 0; ix < nx; ++ix)
+            {
+                const int h[8] = {id(ix, iy, iz),         id(ix + 1, iy, iz),         id(ix + 1, iy + 1, iz),
+                                  id(ix, iy + 1, iz),     id(ix, iy, iz + 1),         id(ix + 1, iy, iz + 1),
+                                  id(ix + 1, iy + 1, iz + 1), id(ix, iy + 1, iz + 1)};
+                if (!tets)
+                    conn.insert(conn.end(), h, h + 8);
+                else
+                    for (int t = 0; t < 6; t++)
+                        for (int k = 0; k < 4; k++)
+                            conn.push_back(h[kuhn[t][k]]);
+            }
+    return true;
+}
+
+bool DofLayer(int coordType, int nCells, const int* coordConn, int dofType, std::vector<int>& ranks, int& nNodes,
+              std::string& err)
+{
+    if (FamilyOf(coordType) != FamilyOf(dofType))
+    {
+        err = "dof interpolation and coordinate element have different shapes";
+        return false;
+    }
+    const int nnC = NodesOf(coordType), nnD = NodesOf(dofType);
+    const int nVert = FamilyOf(coordType) == 0 ? 4 : (FamilyOf(coordType) == 1 ? 8 : 4);
+    if (nnC != nVert)
+    {
+        err = "coordinate elements must be linear (quad4, hex8, tet4)";
+        return false;
+    }
+    // for each dof node: either a vertex (index < nVert) or the midpoint of a vertex pair
+    std::vector<std::pair<int, int>> pairOf(nnD, {-1, -1});
+    for (int k = nVert; k < nnD; k++)
+    {
+        double xk[3], xi[3], xj[3];
+        NodeCoords(dofType, k, xk);
+        for (int i = 0; i < nVert && pairOf[k].first < 0; i++)
+            for (int j = i + 1; j < nVert; j++)
+            {
+                NodeCoords(coordType, i, xi);
+                NodeCoords(coordType, j, xj);
+                if (xi[0] + xj[0] == 2 * xk[0] && xi[1] + xj[1] == 2 * xk[1] && xi[2] + xj[2] == 2 * xk[2])
+                {
+                    pairOf[k] = {i, j};
+                    break;
+                }
+            }
+        if (pairOf[k].first < 0)
+        {
+            err = "dof node is neither a vertex nor an edge midpoint";
+            return false;
+        }
+    }
+    ranks.assign((size_t)nCells * nnD, -1);
+    int maxNode = -1;
+    for (size_t i = 0; i < (size_t)nCells * nnC; i++)
+        maxNode = std::max(maxNode, coordConn[i]);
+    std::vector<int> vertRank(maxNode + 1, -1);
+    std::unordered_map<unsigned long long, int> edgeRank;
+    int next = 0;
+    for (int c = 0; c < nCells; c++)
+    {
+        const int* cn = coordConn + (size_t)c * nnC;
+        for (int a = 0; a < nnD; a++)
+        {
+            int r;
+            if (a < nVert)
+            {
+                int& vr = vertRank[cn[a]];
+                if (vr < 0)
+                    vr = next++;
+                r = vr;
+            }
+            else
+            {
+                unsigned long long v0 = (unsigned)cn[pairOf[a].first], v1 = (unsigned)cn[pairOf[a].second];
+                if (v0 > v1)
+                    std::swap(v0, v1);
+                auto ins = edgeRank.emplace((v0 << 32) | v1, next);
+                if (ins.second)
+                    next++;
+                r = ins.first->second;
+            }
+            ranks[(size_t)c * nnD + a] = r;
+        }
+    }
+    nNodes = next;
+    return true;
+}
+
+bool BuildNodeGraph(int nCells, int nnD, const int* ranks, int nNodes, NodeGraph& g, std::string& err)
+{
+    g = NodeGraph();
+    g.nNodes = nNodes;
+    g.nnD = nnD;
+    g.incPtr.assign((size_t)nNodes + 1, 0);
+    for (int c = 0; c < nCells; c++)
+        for (int a = 0; a < nnD; a++)
+        {
+            const int r = ranks[(size_t)c * nnD + a];
+            if (r < 0 || r >= nNodes)
+            {
+                err = "cell dof node out of range";
+                return false;
+            }
+            for (int b = 0; b < a; b++)
+                if (ranks[(size_t)c * nnD + b] == r)
+                {
+                    err = "a cell references the same dof node twice";
+                    return false;
+                }
+            g.incPtr[r + 1]++;
+        }
+    for (int p = 0; p < nNodes; p++)
+        g.incPtr[p + 1] += g.incPtr[p];
+    const size_t nInc = (size_t)g.incPtr[nNodes];
+    g.incCell.resize(nInc);
+    g.incLocal.resize(nInc);
+    {
+        std::vector<int> cur(g.incPtr.begin(), g.incPtr.end() - 1);
+        for (int c = 0; c < nCells; c++)
+            for (int a = 0; a < nnD; a++)
+            {
+                const int r = ranks[(size_t)c * nnD + a];
+                g.incCell[cur[r]] = c;
+                g.incLocal[cur[r]] = (unsigned char)a;
+                cur[r]++;
+            }
+    }
+    g.adjPtr.assign((size_t)nNodes + 1, 0);
+    g.adj.clear();
+    g.adj.reserve(nInc * 4);
+    g.slot.resize(nInc * nnD);
+    std::vector<int> buf;
+    std::vector<char> seen;
+    for (int p = 0; p < nNodes; p++)
+    {
+        buf.clear();
+        for (int i = g.incPtr[p]; i < g.incPtr[p + 1]; i++)
+            for (int b = 0; b < nnD; b++)
+                buf.push_back(ranks[(size_t)g.incCell[i] * nnD + b]);
+        std::sort(buf.begin(), buf.end());
+        buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+        if (buf.size() >= 32768)
+        {
+            err = "node neighbourhood too large for 15-bit slots";
+            return false;
+        }
+        const size_t base = g.adj.size();
+        g.adj.insert(g.adj.end(), buf.begin(), buf.end());
+        g.adjPtr[p + 1] = (int)g.adj.size();
+        seen.assign(buf.size(), 0);
+        for (int i = g.incPtr[p]; i < g.incPtr[p + 1]; i++)
+            for (int b = 0; b < nnD; b++)
+            {
+                const int q = ranks[(size_t)g.incCell[i] * nnD + b];
+                const int s = (int)(std::lower_bound(g.adj.begin() + base, g.adj.end(), q) - (g.adj.begin() + base));
+                unsigned short v = (unsigned short)s;
+                if (!seen[s])
+                {
+                    seen[s] = 1;
+                    v |= 0x8000;
+                }
+                g.slot[(size_t)i * nnD + b] = v;
+            }
+        if (g.adj.size() > (size_t)INT_MAX)
+        {
+            err = "node graph too large";
+            return false;
+        }
+    }
+    return true;
+}
+
+bool BuildRowPtr(const NodeGraph& g, int nFields, const int* ncomp, std::vector<int>& rowPtr, long long& nnz,
+                 std::string& err)
+{
+    int W = 0;
+    long long nRows = 0;
+    for (int f = 0; f < nFields; f++)
+    {
+        W += ncomp[f];
+        nRows += (long long)g.nNodes * ncomp[f];
+    }
+    rowPtr.assign((size_t)nRows + 1, 0);
+    long long acc = 0;
+    size_t row = 0;
+    for (int f = 0; f < nFields; f++)
+        for (int p = 0; p < g.nNodes; p++)
+        {
+            const long long len = (long long)W * (g.adjPtr[p + 1] - g.adjPtr[p]);
+            for (int c = 0; c < ncomp[f]; c++)
+            {
+                acc += len;
+                if (acc > INT_MAX)
+                {
+                    err = "matrix has more than 2^31-1 nonzeros (int32 CSR indices)";
+                    return false;
+                }
+                rowPtr[++row] = (int)acc;
+            }
+        }
+    nnz = acc;
+    return true;
+}
+
+}  // namespace nuto_amd

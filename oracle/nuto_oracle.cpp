@@ -463,6 +463,20 @@ static Mat Col(const std::vector<double>& x)
     return o;
 }
 
+// Magnitude shadows: |A| and |A| * |B|. The parity scale of an output entry is the same expression
+// evaluated on absolute values (the componentwise scale of the rounding-error bound of a sum of
+// products); an entry that is a cancellation result is compared against the size of its terms.
+static Mat Abs(Mat a)
+{
+    for (double& x : a.v)
+        x = std::abs(x);
+    return a;
+}
+static Mat MulAbs(const Mat& a, const Mat& b)
+{
+    return Mul(Abs(a), Abs(b));
+}
+
 // ---------------------------------------------------------------------------------------------
 // Jacobian (Jacobian.h:16-48,100-103,131-146): J = X(dim x n) * dN(n x dim), inverse by the
 // adjugate, determinant by cofactor expansion along the first row.
@@ -976,7 +990,63 @@ struct CellResult
 {
     std::vector<double> R[2];
     Mat K[2][2];
+    std::vector<double> Rm[2];  // magnitude scales (see Abs / MulAbs)
+    Mat Km[2][2];
 };
+
+// magnitudes of the 3D law outputs for a strain of magnitude em (em = |B| |u|)
+struct LawMag
+{
+    double sig[6], dsw[6], eeq, m[6];
+};
+
+static LawMag LawMagnitudes(const double* em, double omega, double E, double nu, double k, int app, double A)
+{
+    LawMag o;
+    const double m3K = 3 * E / (3 - 6 * nu), m2G = 2 * E / (2 + 2 * nu);
+    const double eV = (em[0] + em[1] + em[2]) / 3.0;
+    const double H = 1.0;  // bounds use the larger (damaged) volumetric factor for UNILATERAL as well
+    for (int i = 0; i < 3; i++)
+    {
+        o.sig[i] = std::abs(1. - omega * H) * m3K * eV + std::abs(1. - omega) * m2G * (em[i] + eV);
+        o.dsw[i] = m3K * eV * H + m2G * (em[i] + eV);
+    }
+    for (int i = 3; i < 6; i++)
+    {
+        o.sig[i] = std::abs(1. - omega) * m2G * 0.5 * em[i];
+        o.dsw[i] = m2G * 0.5 * em[i];
+    }
+    (void)app;
+    const double K1 = (k - 1.) / (2. * k * (1 - 2 * nu)), K2 = 3. / (k * (1. + nu) * (1. + nu));
+    const double I1 = em[0] + em[1] + em[2];
+    const double J2 = ((em[0] + em[1]) * (em[0] + em[1]) + (em[1] + em[2]) * (em[1] + em[2]) +
+                       (em[2] + em[0]) * (em[2] + em[0])) / 6. +
+                      0.25 * (em[3] * em[3] + em[4] * em[4] + em[5] * em[5]);
+    o.eeq = K1 * I1 + std::sqrt(K1 * K1 * I1 * I1 + K2 * J2);
+    if (A == 0)
+    {
+        for (int i = 0; i < 3; i++)
+            o.m[i] = K1;
+        for (int i = 3; i < 6; i++)
+            o.m[i] = 0;
+    }
+    else
+    {
+        const double h = 1. / (2 * A);
+        for (int i = 0; i < 3; i++)
+            o.m[i] = K1 + h * (2 * K1 * K1 * I1 + K2 * (2 * em[i] + em[(i + 1) % 3] + em[(i + 2) % 3]) / 3.);
+        for (int i = 3; i < 6; i++)
+            o.m[i] = h * K2 * 0.5 * em[i];
+    }
+    return o;
+}
+
+static double MisesA(const double* s, double nu, double k)
+{
+    const double K1 = (k - 1.) / (2. * k * (1 - 2 * nu)), K2 = 3. / (k * (1. + nu) * (1. + nu));
+    double i1 = ModMises3::I1(s), j2 = ModMises3::J2(s);
+    return std::sqrt(K1 * K1 * i1 * i1 + K2 * j2);
+}
 
 // element-local material for cell c
 struct CellLaw
@@ -1047,22 +1117,29 @@ static void IntegrateCell(const Problem& P, int c, const double* u, const double
             DShapeDim(ty, xi, dN);
             dNdX[t] = GlobalDerivatives(jac, dN, nnD[t]);
         }
-        std::vector<double> Rip[2];
-        Mat Kip[2][2];
+        std::vector<double> Rip[2], Rmag[2];
+        Mat Kip[2][2], Kmag[2][2];
 
         if (P.p.kind == 0 || P.p.kind == 2)
         {
             Mat B = StrainB(dNdX[0]);
             Mat strainM = Mul(B, Col(nodeVal[0]));
             const double* strain = strainM.v.data();
+            Mat strainMag = MulAbs(B, Col(nodeVal[0]));
             if (P.p.kind == 0)
             {
                 // MomentumBalance::Gradient/Hessian0 (MomentumBalance.h:26-46) with LinearElastic
                 Mat C = ElasticC(dim, law.E, P.p.nu, P.p.plane_state);
                 if (wantR)
+                {
                     Rip[0] = Mul(T(B), Mul(C, strainM)).v;
+                    Rmag[0] = MulAbs(T(B), MulAbs(C, strainMag)).v;
+                }
                 if (wantK)
+                {
                     Kip[0][0] = Mul(Mul(T(B), C), B);
+                    Kmag[0][0] = MulAbs(MulAbs(T(B), C), B);
+                }
             }
             else
             {
@@ -1074,8 +1151,13 @@ static void IntegrateCell(const Problem& P, int c, const double* u, const double
                 double eeq = norm.Value(strain);
                 double kap = std::max(eeq, kh);
                 double omega = dl.Damage(kap);
+                LawMag lm = LawMagnitudes(strainMag.v.data(), omega, law.E, P.p.nu, P.p.k, P.p.damage_application,
+                                          MisesA(strain, P.p.nu, P.p.k));
                 if (wantR)
+                {
                     Rip[0] = Mul(T(B), Col(led.Stress(strain, omega))).v;
+                    Rmag[0] = MulAbs(T(B), Col(std::vector<double>(lm.sig, lm.sig + 6))).v;
+                }
                 if (wantK)
                 {
                     Mat D = led.DstressDstrain(strain, omega);
@@ -1089,6 +1171,12 @@ static void IntegrateCell(const Problem& P, int c, const double* u, const double
                         for (int j = 0; j < 6; j++)
                             tang(i, j) = D(i, j) + (dsw[i] * dw) * dk[j];
                     Kip[0][0] = Mul(Mul(T(B), tang), B);
+                    Mat tm(6, 6);
+                    const bool active = norm.Value(strain) >= kh;
+                    for (int i = 0; i < 6; i++)
+                        for (int j = 0; j < 6; j++)
+                            tm(i, j) = std::abs(D(i, j)) + (active ? lm.dsw[i] * std::abs(dw) * lm.m[j] : 0.);
+                    Kmag[0][0] = MulAbs(MulAbs(T(B), tm), B);
                 }
             }
         }
@@ -1110,12 +1198,19 @@ static void IntegrateCell(const Problem& P, int c, const double* u, const double
             Mat strainM = Mul(Bd, Col(nodeVal[0]));
             const double* strain = strainM.v.data();
             const double g = 1., dgdw = 0.;
+            Mat strainMag = MulAbs(Bd, Col(nodeVal[0]));
+            LawMag lm = LawMagnitudes(strainMag.v.data(), omega, law.E, P.p.nu, P.p.k, P.p.damage_application,
+                                      MisesA(strain, P.p.nu, P.p.k));
             if (wantR)
             {
                 Rip[0] = Mul(T(Bd), Col(led.Stress(strain, omega))).v;
                 Mat r1 = Scale(T(Ne), eeq - norm.Value(strain));
                 Mat r2 = Mul(T(Be), Scale(eeqGrad, P.p.c * g));
                 Rip[1] = Add(r1, r2).v;
+                Rmag[0] = MulAbs(T(Bd), Col(std::vector<double>(lm.sig, lm.sig + 6))).v;
+                Mat m1 = Scale(Abs(T(Ne)), std::abs(eeq) + lm.eeq);
+                Mat m2 = MulAbs(T(Be), Scale(MulAbs(Be, Col(nodeVal[1])), P.p.c));
+                Rmag[1] = Add(m1, m2).v;
             }
             if (wantK)
             {
@@ -1129,31 +1224,54 @@ static void IntegrateCell(const Problem& P, int c, const double* u, const double
                 Mat ee3 = Mul(Scale(Scale(Mul(T(Be), Scale(eeqGrad, P.p.c)), dgdw), dwdk * dkde), Ne);
                 Kip[1][1] = Add(ee, ee3);
                 Mat dsw = Col(led.DstressDomega(strain));
-                Kip[0][1] = Mul(Mul(T(Bd), Scale(Scale(Scale(dsw, dwdk), dkde), 1.)), Ne);
+                Kip[0][1] = Mul(Mul(T(Bd), Scale(Scale(dsw, dwdk), dkde)), Ne);
+                Kmag[0][0] = MulAbs(MulAbs(T(Bd), led.DstressDstrain(strain, omega)), Bd);
+                Mat mm(1, 6);
+                mm.v.assign(lm.m, lm.m + 6);
+                Kmag[1][0] = MulAbs(MulAbs(T(Ne), mm), Bd);
+                Kmag[1][1] = Add(MulAbs(T(Ne), Ne), Scale(MulAbs(T(Be), Be), P.p.c));
+                Mat dm = Col(std::vector<double>(lm.dsw, lm.dsw + 6));
+                Kmag[0][1] = MulAbs(MulAbs(T(Bd), Scale(dm, std::abs(dwdk) * dkde)), Ne);
             }
         }
         // result += (f * detJ) * w ; first IP assigns ("uninitialized addition")
         for (int t = 0; t < P.ntypes; t++)
         {
+            const double sw = std::abs(detJ) * w;
             if (wantR)
             {
                 for (double& x : Rip[t])
                     x = (x * detJ) * w;
+                for (double& x : Rmag[t])
+                    x = x * sw;
                 if (first)
+                {
                     out.R[t] = Rip[t];
+                    out.Rm[t] = Rmag[t];
+                }
                 else
                     for (size_t i = 0; i < Rip[t].size(); i++)
+                    {
                         out.R[t][i] += Rip[t][i];
+                        out.Rm[t][i] += Rmag[t][i];
+                    }
             }
             if (wantK)
                 for (int s = 0; s < P.ntypes; s++)
                 {
                     for (double& x : Kip[t][s].v)
                         x = (x * detJ) * w;
+                    Kmag[t][s] = Scale(Kmag[t][s], sw);
                     if (first)
+                    {
                         out.K[t][s] = Kip[t][s];
+                        out.Km[t][s] = Kmag[t][s];
+                    }
                     else
+                    {
                         out.K[t][s] = Add(out.K[t][s], Kip[t][s]);
+                        out.Km[t][s] = Add(out.Km[t][s], Kmag[t][s]);
+                    }
                 }
         }
         first = false;
@@ -1176,6 +1294,7 @@ struct Triplet
 {
     int i, j;
     double v;
+    double m = 0.;  // magnitude scale of the contribution
 };
 
 // setFromTriplets-equivalent: bucket by outer index in list order, collapse duplicates
@@ -1185,7 +1304,7 @@ static void TripletsToCSR(const std::list<Triplet>& trip, int nrows, std::vector
 {
     std::vector<std::vector<std::pair<int, std::pair<double, double>>>> rows(nrows);
     for (const Triplet& t : trip)
-        rows[t.i].push_back({t.j, {t.v, std::abs(t.v)}});
+        rows[t.i].push_back({t.j, {t.v, t.m != 0. ? t.m : std::abs(t.v)}});
     rp.assign(nrows + 1, 0);
     ci.clear();
     v.clear();
@@ -1261,16 +1380,17 @@ static void Assemble(Problem& P, const double* u, const double* e, const double*
                     for (size_t i = 0; i < num[t].size(); i++)
                     {
                         localR[off[t] + num[t][i]] += cr.R[t][i];
-                        localB[off[t] + num[t][i]] += std::abs(cr.R[t][i]);
+                        localB[off[t] + num[t][i]] += cr.Rm[t][i];
                     }
             if (wantK)
                 for (int t = 0; t < nt; t++)
                     for (int s = 0; s < nt; s++)
                     {
                         const Mat& K = cr.K[t][s];
+                        const Mat& Km = cr.Km[t][s];
                         for (size_t i = 0; i < num[t].size(); i++)
                             for (size_t j = 0; j < num[s].size(); j++)
-                                local[t][s].push_back({num[t][i], num[s][j], K((int)i, (int)j)});
+                                local[t][s].push_back({num[t][i], num[s][j], K((int)i, (int)j), Km((int)i, (int)j)});
                     }
         }
 #ifdef _OPENMP
