@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X assembly path on the headline workload.
+
+Workload (one "step"): SimpleAssembler::BuildVector + BuildMatrix of the hex8 gradient-damage
+integrand (GradientDamage<3>, 3 displacement + 1 nonlocal-strain dof per node, 2x2x2 Gauss) on
+the structured 100 x 100 x 100 mesh of a 100 mm cube: 1,000,000 cells, 4,121,204 dofs,
+436,334,416 nonzeros. E = 30000, nu = 0.2, k = 10, kappa0 = 1e-4, beta = 100, alpha = 0.99,
+c = 4 mm^2; u ~ U[-5e-3, 5e-3] mm, eeq ~ U[0, 5e-4], kappa ~ U[0, 3e-4] per IP (seeded).
+Inputs and outputs are resident in HBM; the timed region is K fused R+K launches.
+
+Prints one JSON line (rank 0). `value` is cells/s; `roofline` prices the assembly kernel's
+algorithmic bytes per launch against 8 TB/s HBM; `cpu_baseline` times the CPU restatement of
+NuTo's algorithm (the oracle, OpenMP, triplets + setFromTriplets) on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(nnz, n_rows, n_nodes, n_cells, nip, nn_cell):
+    """Compulsory bytes of one R+K assembly: K and R written once; coordinates, u, eeq, history and
+    connectivity read once (index arrays of the pattern are not counted)."""
+    return (8 * nnz + 8 * n_rows + 8 * (3 + 3 + 1) * n_nodes + 8 * n_cells * nip + 4 * n_cells * nn_cell)
+
+
+def cpu_baseline(n, threads, budget_s=30.0):
+    """Times the oracle (NuTo's algorithm restated: per-thread triplet lists, critical splice,
+    setFromTriplets) on an n^3 sample of the same workload."""
+    import numpy as np
+
+    import oracle as O
+    from nuto_amd import synthetic as S
+
+    m = O.Mesh.bricks(n, n, n)
+    m.scale(float(n))  # 1 mm spacing, as the 100 mm / 100 cells workload
+    Ld = m.add_dof(O.HEX8, 3)
+    Le = m.add_dof(O.HEX8, 1)
+    p = O.Params(kind=1, dim=3, E=30000.0, nu=0.2, k=10.0, kappa0=1e-4, beta=100.0, alpha=0.99, c=4.0)
+    P = O.Problem(m, p, 0, 2, Ld, Le)
+    u = S.uniform(S.SEED_U, P.ndof[0], -5e-3, 5e-3)
+    e = S.uniform(S.SEED_E, P.ndof[1], 0.0, 5e-4)
+    k = S.uniform(S.SEED_KAPPA, P.ncells * 8, 0.0, 3e-4)
+    times = []
+    t_start = time.time()
+    while len(times) < 5 and (time.time() - t_start) < budget_s:
+        t0 = time.perf_counter()
+        P.assemble(u, e, k, want_r=True, want_k=True, nthreads=threads)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    return {"value": P.ncells / t, "unit": "cells/s", "cores": threads, "kind": "port",
+            "sample": f"hex8 gradient damage {n}x{n}x{n} cells (1 mm spacing), R+K, median of {len(times)} runs "
+                      f"of the oracle with {threads} OpenMP threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=100, help="cells per direction (100 = the headline workload)")
+    ap.add_argument("--cpu-sample", type=int, default=24, help="cells per direction of the CPU baseline sample")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="OpenMP threads for the CPU baseline (0: auto)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build measures the single-GPU assembly kernel (asked for "
+              f"--gpus {args.gpus}). Multi-GPU sharding is described in DESIGN.md and not launched.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import nuto_amd as N
+    from nuto_amd import synthetic as S
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        sys.exit(1)
+
+    n = args.n
+    t0 = time.time()
+    X, C = N.structured_mesh(N.HEX8, n, n, n, (100.0 * n / 100, 100.0 * n / 100, 100.0 * n / 100))
+    ranks, nn = N.dof_layer(N.HEX8, C, N.HEX8)
+    ctx = N.Context()
+    ctx.set_cells(N.HEX8, X, C)
+    ctx.set_dof_type(0, N.HEX8, 3, 3 * nn, N.cell_dofs(ranks, 3))
+    ctx.set_dof_type(1, N.HEX8, 1, nn, N.cell_dofs(ranks, 1))
+    ctx.set_integrand(N.gradient_damage_params())
+    ctx.set_rule(0, 2)
+    nnz, n_rows = ctx.build_pattern()
+    n_cells, nip = C.shape[0], 8
+    u = torch.from_numpy(S.uniform(S.SEED_U, 3 * nn, -5e-3, 5e-3)).cuda()
+    e = torch.from_numpy(S.uniform(S.SEED_E, nn, 0.0, 5e-4)).cuda()
+    kap = torch.from_numpy(S.uniform(S.SEED_KAPPA, n_cells * nip, 0.0, 3e-4)).cuda()
+    R = torch.empty(n_rows, dtype=torch.float64, device="cuda")
+    K = torch.empty(nnz, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    setup_s = time.time() - t0
+    log(f"setup {setup_s:.1f}s: {n_cells} cells, {n_rows} rows, {nnz} nonzeros")
+
+    for _ in range(args.warmup):
+        ctx.build_vector_matrix(u, e, kap, R, K)
+    torch.cuda.synchronize()
+
+    # kernel time with HIP events on the stream the kernels run on (torch's current stream)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for a, b in evs:
+        a.record()
+        ctx.build_vector_matrix(u, e, kap, R, K)
+        b.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    ms_per_step = 1000.0 * wall / args.steps
+    value = n_cells * args.steps / wall
+
+    bytes_alg = algorithmic_bytes(nnz, n_rows, nn, n_cells, nip, 8)
+    achieved = bytes_alg / (kernel_ms * 1e-3) / 1e9
+    rec = {
+        "metric": "hex8 GDM cells/s assembled (R+K FP64), % HBM roofline; adjoint/forward ratio",
+        "value": value,
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64 state on a structured mesh)",
+        "config": {"workload": f"hex8 gradient damage R+K, {n}x{n}x{n} cells of a {100 * n // 100} mm cube, "
+                               f"2x2x2 Gauss", "cells": n_cells, "dofs": n_rows, "nnz": nnz,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel_ms": kernel_ms, "algorithmic_bytes": bytes_alg},
+        "setup_s": setup_s,
+    }
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        try:
+            rec["cpu_baseline"] = cpu_baseline(args.cpu_sample, threads)
+        except Exception as ex:  # the baseline is reported, never the measured value
+            rec["cpu_baseline"] = {"error": str(ex)}
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

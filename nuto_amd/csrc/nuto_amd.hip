@@ -99,8 +99,9 @@ struct nuto_ctx
     DevBuf incPtr, incCell, incLocal, slot, adjPtr, adj, rowPtr, col;
     long long nnz = 0;
     int nRows = 0;
-    // tiled hex8 gradient-damage plan
+    // hex8 gradient-damage node kernel (prepared lazily from the node graph)
     GdmHex8Plan gdm;
+    bool gdmTried = false;
 };
 
 namespace
@@ -163,6 +164,14 @@ void SyncCheck(nuto_ctx* ctx, const char* what)
     }
 }
 
+// NUTO_AMD_GENERIC=1 routes every call through the generic row-owner kernel (used by the tests
+// to check the specialised kernels against it).
+bool ForceGeneric()
+{
+    const char* v = std::getenv("NUTO_AMD_GENERIC");
+    return v && v[0] == '1';
+}
+
 template <class T>
 void Upload(nuto_ctx* ctx, DevBuf& b, const T* src, size_t n, const char* what)
 {
@@ -176,6 +185,7 @@ void InvalidatePattern(nuto_ctx* ctx)
 {
     ctx->patternBuilt = false;
     ctx->gdm.Release();
+    ctx->gdmTried = false;
 }
 
 // IP tables: per IP w, dNc[nnC][dim], N[nnD], dNd[nnD][dim] (shape tables of the coordinate
@@ -370,11 +380,19 @@ void Assemble(nuto_ctx* ctx, const double* u, const double* e, const double* kap
     const bool wr = R != nullptr, wk = K != nullptr;
     if (!wr && !wk)
         return;
-    if (ctx->gdm.Usable(ctx->params.kind, ctx->coordType, ctx->f[0].type, ctx->nip, ctx->hasCellIds))
+    if (!ctx->gdmTried)
+    {
+        ctx->gdm.Prepare(ctx->params.kind, ctx->coordType, ctx->f[0].type, ctx->nip, ctx->hasCellIds, ctx->graph,
+                         ctx->nCells, ctx->stream);
+        ctx->gdmTried = true;
+    }
+    if (!ForceGeneric() &&
+        ctx->gdm.Usable(ctx->params.kind, ctx->coordType, ctx->f[0].type, ctx->nip, ctx->hasCellIds))
     {
         RowArgs a = MakeRowArgs(ctx, u, e, kappa, R, K);
         ctx->gdm.Launch(a, wr, wk, ctx->stream);
         Check(hipGetLastError(), "gdm hex8 launch");
+        SyncCheck(ctx, "gdm hex8 kernel");
         return;
     }
     RowArgs a = MakeRowArgs(ctx, u, e, kappa, R, K);
@@ -591,6 +609,8 @@ int nuto_set_integrand(nuto_ctx* ctx, const nuto_params* params)
                 "damage application must be FULL (0) or UNILATERAL (1)");
         ctx->params = *params;
         ctx->integrandSet = true;
+        ctx->gdm.Release();
+        ctx->gdmTried = false;
     });
 }
 
@@ -605,6 +625,7 @@ int nuto_set_rule(nuto_ctx* ctx, int kind, int n)
         ctx->ruleSet = true;
         ctx->tabValid = false;
         ctx->gdm.Release();
+        ctx->gdmTried = false;
     });
 }
 
@@ -670,15 +691,6 @@ int nuto_build_pattern(nuto_ctx* ctx, long long* nnz, int* n_rows)
         ctx->nnz = nz;
         ctx->nRows = (int)(rp.size() - 1);
         ctx->patternBuilt = true;
-        // the tiled gradient-damage plan is derived lazily from the same graph
-        ctx->gdm.Release();
-        if (ctx->ruleSet)
-        {
-            Rule r;
-            if (MakeRule(ctx->ruleKind, ctx->dim, ctx->ruleN, r))
-                ctx->gdm.Prepare(ctx->integrandSet ? ctx->params.kind : -1, ctx->coordType, ctx->f[0].type, r.Num(),
-                                 ctx->hasCellIds, g, ctx->nCells, ctx->stream);
-        }
         if (nnz)
             *nnz = nz;
         if (n_rows)

@@ -211,3 +211,20 @@ def test_error_paths():
     with pytest.raises(N.NutoError) as ei:
         ctx.set_dof_type(0, N.HEX8, 3, 24, bad)
     assert ei.value.code == 5  # not node-blocked
+
+
+def test_hex8_node_kernel_matches_generic_kernel(monkeypatch):
+    # the specialised hex8 gradient-damage kernel against the generic row-owner kernel and the oracle
+    X, C = make_mesh(N.HEX8, (6, 5, 4), (6.0, 5.0, 4.0))
+    dev, orc = gdm_params()
+    p = Pair(N.HEX8, N.HEX8, X, C, dev, orc, True)
+    u, e, kap = p.inputs()
+    ctx = p.context()
+    Rf, Kf = ctx.build_vector_matrix(to_dev(u), to_dev(e), to_dev(kap))
+    monkeypatch.setenv("NUTO_AMD_GENERIC", "1")
+    Rg, Kg = ctx.build_vector_matrix(to_dev(u), to_dev(e), to_dev(kap))
+    monkeypatch.delenv("NUTO_AMD_GENERIC")
+    ref = p.orc.assemble(u, e, kap)
+    for R, K in ((Rf, Kf), (Rg, Kg)):
+        check_values(R.cpu().numpy(), ref["R"], ref["R_bound"], "R")
+        check_values(K.cpu().numpy(), ref["val"], ref["bound"], "K")
