@@ -21,7 +21,18 @@ struct LawParams
     double k, kappa0, beta, alpha, c;
     int damage_application;  // 0 FULL, 1 UNILATERAL
     int plane_state;         // 0 PLANE_STRESS, 1 PLANE_STRAIN (2D linear elasticity)
+    // derived once per launch (see MakeDerived): 3K = cK E, 2G = cG E, modified Mises K1, K2
+    double cK, cG, K1, K2;
 };
+
+inline LawParams MakeDerived(LawParams p)
+{
+    p.cK = 3.0 / (3.0 - 6.0 * p.nu);
+    p.cG = 2.0 / (2.0 + 2.0 * p.nu);
+    p.K1 = (p.k - 1.0) / (2.0 * p.k * (1.0 - 2.0 * p.nu));
+    p.K2 = 3.0 / (p.k * (1.0 + p.nu) * (1.0 + p.nu));
+    return p;
+}
 
 // B_a^T v for a 3D strain-type 6-vector v: component i collects the rows of B_a that act on u_i
 __device__ __host__ inline void BTv3(const double g[3], const double v[6], double o[3])

@@ -278,7 +278,8 @@ RowArgs MakeRowArgs(nuto_ctx* ctx, const double* u, const double* e, const doubl
     }
     a.tab = ctx->tab.As<double>();
     const nuto_params& p = ctx->params;
-    a.P = LawParams{p.E, p.nu, p.k, p.kappa0, p.beta, p.alpha, p.c, p.damage_application, p.plane_state};
+    a.P = MakeDerived(LawParams{p.E, p.nu, p.k, p.kappa0, p.beta, p.alpha, p.c, p.damage_application,
+                                p.plane_state, 0.0, 0.0, 0.0, 0.0});
     a.R = R;
     a.K = K;
     return a;
