@@ -20,6 +20,10 @@ import time
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+HERE = os.path.dirname(os.path.abspath(__file__))
+# HBM traffic of the assembly kernel per launch, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes
+# scaled by the same counters on a copy kernel of known size (written by --copy-calibration runs)
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "kGdmHex8Node_hex8_gdm_100cube_traffic.txt")
 
 
 def log(*a):
@@ -30,6 +34,30 @@ def algorithmic_bytes(nnz, n_rows, n_nodes, n_cells, nip, nn_cell):
     """Compulsory bytes of one R+K assembly: K and R written once; coordinates, u, eeq, history and
     connectivity read once (index arrays of the pattern are not counted)."""
     return (8 * nnz + 8 * n_rows + 8 * (3 + 3 + 1) * n_nodes + 8 * n_cells * nip + 4 * n_cells * nn_cell)
+
+
+def measured_traffic(n):
+    """Bytes per launch from the committed counter summary (None when absent or for another size)."""
+    if n != 100 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def copy_calibration(mib):
+    """A plain device copy of known size (reads and writes `mib` MiB each), the reference point for
+    scaling FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": 8 * n, "copy_calibration_bytes_written": 8 * n}), flush=True)
 
 
 def cpu_baseline(n, threads, budget_s=30.0):
@@ -70,6 +98,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=24, help="cells per direction of the CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0, help="OpenMP threads for the CPU baseline (0: auto)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -87,6 +117,10 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         sys.exit(1)
+
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     n = args.n
     t0 = time.time()
@@ -146,7 +180,7 @@ def main():
                                f"2x2x2 Gauss", "cells": n_cells, "dofs": n_rows, "nnz": nnz,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(n),
                      "kernel_ms": kernel_ms, "algorithmic_bytes": bytes_alg},
         "setup_s": setup_s,
     }

@@ -172,6 +172,7 @@ class Context:
             raise RuntimeError("nuto_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
         self._torch = torch
         L = lib()
+        self._L = L  # kept for close() during interpreter shutdown
         h = ctypes.c_void_p()
         r = L.nuto_create(ctypes.byref(h), device)
         if r:
@@ -184,9 +185,10 @@ class Context:
         self.nnz = 0
 
     def close(self):
-        if getattr(self, "h", None):
-            lib().nuto_destroy(self.h)
-            self.h = None
+        h, L = getattr(self, "h", None), getattr(self, "_L", None)
+        if h and L is not None:
+            L.nuto_destroy(h)
+        self.h = None
 
     __del__ = close
 

@@ -21,15 +21,23 @@
 // general form c1 g_a g_b^T + c2 g_b g_a^T + c2 (g_a . g_b) I.
 //
 // Usable when every node has at most 8 incident cells and at most 32 neighbours (structured hex
-// meshes: 8 and 27); other meshes use the generic row-owner kernel.
+// meshes: 8 and 27) and the coordinate and dof layers share their nodes; other meshes use the
+// generic row-owner kernel.
+//
+// Gathers are three dependent loads deep: per-node incidence tables padded to 8 cells (cell,
+// local index, history column, neighbour slots), then the cell's dof node ranks and its history,
+// then the node values (coordinates are stored by dof node for this).
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstdlib>
+#include <string>
+#include <vector>
 
 #include "assemble_rows.hpp"
+#include "devbuf.hpp"
 #include "laws.hpp"
 #include "mesh.hpp"
 
@@ -83,32 +91,45 @@ __device__ inline void GradOf(const double d[3], const double inv[3][3], double 
 }
 }  // namespace gdm8
 
-// Inputs of one node gathered from global memory (issued one node ahead of the computation).
-struct NodeFetch
+// Per-node tables of the node kernel (built once per pattern, see GdmHex8Plan::Prepare).
+struct GdmHex8Tables
 {
-    int beg, cnt, nA, cell, la, slot;
-    double v[7];  // x y z u0 u1 u2 e of node j of cell r
-    double th[3];
+    const int* nodeCell;            // [node][8]: r-th incident cell (group order), -1 past the last
+    const int* nodeHist;            // [node][8]: history column Cell::Id of that cell (nullptr: the cell index)
+    const unsigned char* nodeLa;    // [node][8]: local index of the node in that cell
+    const unsigned char* nodeSlot;  // [node][8][8]: neighbour slot (column block) of local node j
+    const double* xd;               // coordinates by dof node
+    const double* yd;
+    const double* zd;
+    int xcdMap;                     // 1: contiguous node ranges per XCD
 };
 
-__device__ inline void FetchNode(const RowArgs& A, int p, int r, int j, NodeFetch& f)
+// Inputs of lane (r, j) of node p.
+struct NodeFetch
 {
-    f.beg = A.incPtr[p];
-    f.cnt = A.incPtr[p + 1] - f.beg;
-    f.nA = A.adjPtr[p + 1] - A.adjPtr[p];
-    f.cell = 0;
-    f.la = 0;
-    f.slot = 0;
-    if (r < f.cnt)
+    int cell, la, slot;
+    double v[7];  // x y z u0 u1 u2 e of node j of cell r
+    double th[3];
+    double kh;  // history kappa of cell r at IP j
+    double w;   // weight of IP j
+};
+
+__device__ inline void FetchNode(const RowArgs& A, const GdmHex8Tables& T, int p, int r, int j, NodeFetch& f)
+{
+    const size_t pr = (size_t)p * 8 + r;
+    f.cell = T.nodeCell[pr];
+    f.la = T.nodeLa[pr];
+    f.slot = T.nodeSlot[pr * 8 + j];
+    f.w = A.tab[j * gdm8::kTab];
+    f.kh = 0.0;
+    if (f.cell >= 0)
     {
-        f.cell = A.incCell[f.beg + r];
-        f.la = A.incLocal[f.beg + r];
-        f.slot = A.slot[(size_t)(f.beg + r) * 8 + j] & 0x7fff;
-        const int cn = A.cconn[(size_t)f.cell * 8 + j];
+        const int hist = T.nodeHist ? T.nodeHist[pr] : f.cell;
         const int q = A.dconn[(size_t)f.cell * 8 + j];
-        f.v[0] = A.x[cn];
-        f.v[1] = A.y[cn];
-        f.v[2] = A.z[cn];
+        f.kh = A.kappa[(size_t)hist + (size_t)j * A.kappaLd];
+        f.v[0] = T.xd[q];
+        f.v[1] = T.yd[q];
+        f.v[2] = T.zd[q];
         f.v[3] = A.u[(size_t)q * 3];
         f.v[4] = A.u[(size_t)q * 3 + 1];
         f.v[5] = A.u[(size_t)q * 3 + 2];
@@ -126,38 +147,40 @@ __device__ inline void FetchNode(const RowArgs& A, int p, int r, int j, NodeFetc
 // xcdMap the node of block b is chosen so that each XCD processes a contiguous range of nodes
 // (neighbouring nodes share cells and node data, which then stay in that XCD's L2).
 template <bool WR, bool WK, bool FULLDMG>
-__global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, int xcdMap)
+__global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, const GdmHex8Tables T)
 {
     using namespace gdm8;
     __shared__ __align__(16) double state[8 * 8 * kSt];  // [ip][r][kSt]; later partial blocks + R terms
     __shared__ __align__(16) double rowbuf[8 * kNd];      // node data of the incident cells, later p's rows
-    __shared__ unsigned char slotOf[kMaxAdj][kMaxInc];
+    __shared__ __align__(8) unsigned char slotOf[kMaxAdj][kMaxInc];
 
     const int lane = threadIdx.x;
     const int r = lane >> 3, j = lane & 7;
     const LawParams& P = A.P;
     int p = blockIdx.x;
-    if (xcdMap)
+    if (T.xcdMap)
     {
         const int per = (int)((gridDim.x + 7) >> 3);
         p = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
         if (p >= A.nNodes)
             return;
     }
+    // p's rows: 3 displacement rows (consecutive in the CSR) and the eeq row, each 4 nA long
+    const int ru0 = A.rowPtr[(size_t)p * 3];
+    const int L = A.rowPtr[(size_t)p * 3 + 1] - ru0;
+    const int nA = L >> 2;
+    const int re0 = A.rowPtr[(size_t)A.nU + p];
     NodeFetch f;
-    FetchNode(A, p, r, j, f);
-    const int beg = f.beg;
-    const int cnt = f.cnt;
-    const int nA = f.nA;
-    const int L = 4 * nA;
-    const int cell = f.cell, la = f.la;
+    FetchNode(A, T, p, r, j, f);
+    const bool valid = f.cell >= 0;  // r < number of cells incident to p
+    const int la = f.la;
 
     // ---- node data of p's cells to LDS; map slots -> (r, b) -----------------------------------------
     double* nodeData = rowbuf;  // [r][n][8] (+2 per cell): x y z u0 u1 u2 e -
     for (int i = lane; i < kMaxAdj * kMaxInc; i += 64)
         (&slotOf[0][0])[i] = 0xFF;
     __syncthreads();
-    if (r < cnt)
+    if (valid)
     {
         double* nd = nodeData + r * kNd + j * 8;
 #pragma unroll
@@ -168,7 +191,7 @@ __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, int xcdMap)
     }
     // material of the cell (per-cell fields: E, kappa0 = f_t / E, beta = f_t / G_f)
     double E = P.E, k0 = P.kappa0, beta = P.beta;
-    if (A.thE && r < cnt)
+    if (A.thE && valid)
     {
         E = f.th[0];
         k0 = f.th[1] / E;
@@ -179,10 +202,10 @@ __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, int xcdMap)
 
     // ---- phase A: lane (r, ip) ---------------------------------------------------------------------
     double Ru0 = 0.0, Ru1 = 0.0, Ru2 = 0.0, Re = 0.0;
-    if (r < cnt)
+    if (valid)
     {
         const int ip = j;
-        const double w = A.tab[ip * kTab];
+        const double w = f.w;
         const double* nd = nodeData + r * kNd;
         // J = sum_n X_n dN_n^T, Hu = sum_n u_n dN_n^T, he = sum_n e_n dN_n, eN = sum_n N_n e_n
         double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
@@ -228,8 +251,7 @@ __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, int xcdMap)
         GradOf(he, inv, ge);
 
         // kappa = max(history, eeq), d kappa / d eeq (GradientDamage.h:127-135)
-        const int cid = A.cellId ? A.cellId[cell] : cell;
-        const double kh = A.kappa[(size_t)cid + (size_t)ip * A.kappaLd];
+        const double kh = f.kh;
         const double kap = kh < eN ? eN : kh;
         const double dk = eN >= kh ? 1.0 : 0.0;
         // DamageLawExponential (DamageLawExponential.h:39-52)
@@ -340,7 +362,7 @@ __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, int xcdMap)
 #pragma unroll
     for (int i = 0; i < 16; i++)
         acc[i] = 0.0;
-    if (WK && r < cnt)
+    if (WK && valid)
     {
         const int b = j;
 #pragma unroll 1
@@ -405,13 +427,17 @@ __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, int xcdMap)
     __syncthreads();
 
     // ---- partial blocks and R terms to LDS -----------------------------------------------------------
-    double* part = state;                 // [r][b][kPs]
-    double* rterm = state + 8 * 8 * kPs;  // [4][64]
+    double* part = state;                   // [r][b][kPs]
+    double* rterm = state + 8 * 8 * kPs;    // [4][r][ip]
+    double* rcell = rterm + 4 * 64;         // [4][r]: R terms summed over the IPs of cell r
+    double* zero = rcell + 32;              // 8 zeros: the block of an absent (cell, slot) pair
     if (WK)
     {
 #pragma unroll
         for (int i = 0; i < 16; i++)
             part[lane * kPs + i] = acc[i];
+        if (lane < 8)
+            zero[lane] = 0.0;
     }
     if (WR)
     {
@@ -420,27 +446,25 @@ __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, int xcdMap)
         rterm[2 * 64 + lane] = Ru2;
         rterm[3 * 64 + lane] = Re;
     }
-    for (int i = lane; i < 16 * kMaxAdj; i += 64)
-        rowbuf[i] = 0.0;  // p's four rows, 16 * kMaxAdj <= 8 * kNd
     __syncthreads();
 
     // ---- reduce: lane (q, half) sums its 8 block entries over the cells in order ---------------------
+    // (absent cells add the zero block, so every lane runs the same 8 steps)
     if (WK)
     {
         const int q = lane >> 1, h = lane & 1;
         if (q < nA)
         {
+            const unsigned long long row = *reinterpret_cast<const unsigned long long*>(&slotOf[q][0]);
             double sum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-            for (int rr = 0; rr < cnt; rr++)
-            {
-                const int b = slotOf[q][rr];
-                if (b != 0xFF)
-                {
-                    const double* src = part + (rr * 8 + b) * kPs + h * 8;
 #pragma unroll
-                    for (int k = 0; k < 8; k++)
-                        sum[k] += src[k];
-                }
+            for (int rr = 0; rr < 8; rr++)
+            {
+                const unsigned b = (unsigned)(row >> (8 * rr)) & 0xFFu;
+                const double* src = b == 0xFFu ? zero : part + (rr * 8 + (int)b) * kPs + h * 8;
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    sum[k] += src[k];
             }
             // block entry (i, c): i = row component (0..2 displacement rows, 3 eeq row),
             // c = column component (0..2 displacement columns, 3 eeq column)
@@ -454,23 +478,56 @@ __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, int xcdMap)
             }
         }
     }
+    // R: per cell over its IPs in order (Cell.h:108-117), then over the cells in order
+    if (WR && lane < 32)
+    {
+        const double* t = rterm + (lane >> 3) * 64 + (lane & 7) * 8;
+        double s = t[0];
+#pragma unroll
+        for (int ip = 1; ip < 8; ip++)
+            s += t[ip];
+        rcell[lane] = s;
+    }
     __syncthreads();
 
     // ---- write p's rows (displacement rows are consecutive in the CSR) -------------------------------
     if (WK)
     {
-        const int ru0 = A.rowPtr[p * 3];
-        const int re0 = A.rowPtr[A.nU + p];
-        for (int i = lane; i < 3 * L; i += 64)
-            A.K[(size_t)ru0 + i] = rowbuf[i];
-        for (int i = lane; i < L; i += 64)
-            A.K[(size_t)re0 + i] = rowbuf[3 * L + i];
+        double v[6], ve[2];
+#pragma unroll
+        for (int it = 0; it < 6; it++)
+        {
+            const int i = lane + 64 * it;
+            v[it] = i < 3 * L ? rowbuf[i] : 0.0;
+        }
+#pragma unroll
+        for (int it = 0; it < 2; it++)
+        {
+            const int i = lane + 64 * it;
+            ve[it] = i < L ? rowbuf[3 * L + i] : 0.0;
+        }
+#pragma unroll
+        for (int it = 0; it < 6; it++)
+        {
+            const int i = lane + 64 * it;
+            if (i < 3 * L)
+                A.K[(size_t)ru0 + i] = v[it];
+        }
+#pragma unroll
+        for (int it = 0; it < 2; it++)
+        {
+            const int i = lane + 64 * it;
+            if (i < L)
+                A.K[(size_t)re0 + i] = ve[it];
+        }
     }
     if (WR && lane < 4)
     {
+        const double* t = rcell + lane * 8;
         double s = 0.0;
-        for (int l = 0; l < 8 * cnt; l++)
-            s += rterm[lane * 64 + l];
+#pragma unroll
+        for (int rr = 0; rr < 8; rr++)
+            s += t[rr];
         if (lane < 3)
             A.R[(size_t)p * 3 + lane] = s;
         else
@@ -482,46 +539,107 @@ struct GdmHex8Plan
 {
     bool ready = false;
     int nNodes = 0;
+    int grid = 0;     // workgroups: one per node (rounded up to a multiple of 8 with the XCD mapping)
+    int xcdMap = 1;   // contiguous node ranges per XCD (env NUTO_AMD_GDM_XCD_MAP=0 switches it off)
+    DevBuf nodeCell, nodeHist, nodeLa, nodeSlot, xd, yd, zd;
 
     void Release()
     {
         ready = false;
+        nNodes = 0;
+        for (DevBuf* b : {&nodeCell, &nodeHist, &nodeLa, &nodeSlot, &xd, &yd, &zd})
+            b->Release();
     }
 
-    bool Usable(int kind, int coordType, int dofType, int nip, bool) const
+    bool Usable(int kind, int coordType, int dofType, int nip) const
     {
         return ready && kind == kGradientDamage && coordType == kHex8 && dofType == kHex8 && nip == 8;
     }
 
-    void Prepare(int kind, int coordType, int dofType, int nip, bool, const NodeGraph& g, int, hipStream_t)
+    // Builds the per-node tables from the node graph. Returns false (plan not used) when the mesh
+    // exceeds the kernel's bounds or a dof node sits on coordinate nodes with different
+    // coordinates; returns a HIP error through `err` when an upload fails.
+    bool Prepare(int kind, int coordType, int dofType, int nip, const NodeGraph& g, int nCells, const int* dconn,
+                 const int* cconn, const double* hx, const double* hy, const double* hz, const int* cellIds,
+                 hipStream_t stream, hipError_t& err)
     {
-        ready = false;
+        Release();
+        err = hipSuccess;
         if (kind != kGradientDamage || coordType != kHex8 || dofType != kHex8 || nip != 8)
-            return;
-        for (int p = 0; p < g.nNodes; p++)
+            return false;
+        const int n = g.nNodes;
+        for (int p = 0; p < n; p++)
         {
             if (g.incPtr[p + 1] - g.incPtr[p] > gdm8::kMaxInc || g.adjPtr[p + 1] - g.adjPtr[p] > gdm8::kMaxAdj)
-                return;
+                return false;
         }
-        nNodes = g.nNodes;
+        // coordinates by dof node: every coordinate node behind one dof node must agree exactly
+        std::vector<double> X(n), Y(n), Z(n);
+        std::vector<unsigned char> seen(n, 0);
+        for (size_t i = 0; i < (size_t)nCells * 8; i++)
+        {
+            const int q = dconn[i], c = cconn[i];
+            if (!seen[q])
+            {
+                seen[q] = 1;
+                X[q] = hx[c];
+                Y[q] = hy[c];
+                Z[q] = hz[c];
+            }
+            else if (X[q] != hx[c] || Y[q] != hy[c] || Z[q] != hz[c])
+                return false;
+        }
+        std::vector<int> cell((size_t)n * 8, -1), hist(cellIds ? (size_t)n * 8 : 0, 0);
+        std::vector<unsigned char> la((size_t)n * 8, 0), slot((size_t)n * 64, 0);
+        for (int p = 0; p < n; p++)
+        {
+            for (int k = g.incPtr[p], r = 0; k < g.incPtr[p + 1]; k++, r++)
+            {
+                const size_t pr = (size_t)p * 8 + r;
+                cell[pr] = g.incCell[k];
+                la[pr] = g.incLocal[k];
+                if (cellIds)
+                    hist[pr] = cellIds[g.incCell[k]];
+                for (int j = 0; j < 8; j++)
+                    slot[pr * 8 + j] = (unsigned char)(g.slot[(size_t)k * 8 + j] & 0x7fff);
+            }
+        }
+        if ((err = nodeCell.Upload(cell.data(), cell.size() * sizeof(int), stream)) != hipSuccess ||
+            (err = nodeLa.Upload(la.data(), la.size(), stream)) != hipSuccess ||
+            (err = nodeSlot.Upload(slot.data(), slot.size(), stream)) != hipSuccess ||
+            (err = xd.Upload(X.data(), X.size() * sizeof(double), stream)) != hipSuccess ||
+            (err = yd.Upload(Y.data(), Y.size() * sizeof(double), stream)) != hipSuccess ||
+            (err = zd.Upload(Z.data(), Z.size() * sizeof(double), stream)) != hipSuccess ||
+            (cellIds && (err = nodeHist.Upload(hist.data(), hist.size() * sizeof(int), stream)) != hipSuccess))
+        {
+            Release();
+            return false;
+        }
+        nNodes = n;
         const char* env = std::getenv("NUTO_AMD_GDM_XCD_MAP");
         xcdMap = env ? std::atoi(env) : 1;
         grid = xcdMap ? (nNodes + 7) / 8 * 8 : nNodes;
         ready = true;
+        return true;
     }
 
-    int grid = 0;     // workgroups: one per node, rounded up to a multiple of 8
-    int xcdMap = 1;   // contiguous node ranges per XCD
+    GdmHex8Tables Tables() const
+    {
+        return GdmHex8Tables{nodeCell.As<int>(), nodeHist.p ? nodeHist.As<int>() : nullptr,
+                             nodeLa.As<unsigned char>(), nodeSlot.As<unsigned char>(), xd.As<double>(),
+                             yd.As<double>(), zd.As<double>(), xcdMap};
+    }
 
     template <bool FULLDMG>
     void LaunchT(const RowArgs& a, bool wr, bool wk, hipStream_t stream)
     {
+        const GdmHex8Tables t = Tables();
         if (wr && wk)
-            hipLaunchKernelGGL((kGdmHex8Node<true, true, FULLDMG>), dim3(grid), dim3(64), 0, stream, a, xcdMap);
+            hipLaunchKernelGGL((kGdmHex8Node<true, true, FULLDMG>), dim3(grid), dim3(64), 0, stream, a, t);
         else if (wr)
-            hipLaunchKernelGGL((kGdmHex8Node<true, false, FULLDMG>), dim3(grid), dim3(64), 0, stream, a, xcdMap);
+            hipLaunchKernelGGL((kGdmHex8Node<true, false, FULLDMG>), dim3(grid), dim3(64), 0, stream, a, t);
         else
-            hipLaunchKernelGGL((kGdmHex8Node<false, true, FULLDMG>), dim3(grid), dim3(64), 0, stream, a, xcdMap);
+            hipLaunchKernelGGL((kGdmHex8Node<false, true, FULLDMG>), dim3(grid), dim3(64), 0, stream, a, t);
     }
 
     void Launch(const RowArgs& a, bool wr, bool wk, hipStream_t stream)

@@ -19,41 +19,6 @@ using namespace nuto_amd;
 namespace
 {
 
-struct DevBuf
-{
-    void* p = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf()
-    {
-        Release();
-    }
-    void Release()
-    {
-        if (p)
-            (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    hipError_t Alloc(size_t n)
-    {
-        Release();
-        hipError_t e = hipMalloc(&p, n ? n : 16);
-        if (e != hipSuccess)
-            p = nullptr;
-        else
-            bytes = n;
-        return e;
-    }
-    template <class T>
-    T* As() const
-    {
-        return static_cast<T*>(p);
-    }
-};
-
 struct Field
 {
     bool set = false;
@@ -79,6 +44,8 @@ struct nuto_ctx
     bool cellsSet = false;
     int coordType = 0, dim = 0, nCells = 0, nCoordNodes = 0;
     DevBuf x, y, z, cconn, cellId;
+    std::vector<double> hx, hy, hz;  // host copies of the cell group (pattern-time tables)
+    std::vector<int> hcconn, hcellId;
     bool hasCellIds = false;
     int maxCellId = 0;
     Field f[2];
@@ -383,12 +350,14 @@ void Assemble(nuto_ctx* ctx, const double* u, const double* e, const double* kap
         return;
     if (!ctx->gdmTried)
     {
-        ctx->gdm.Prepare(ctx->params.kind, ctx->coordType, ctx->f[0].type, ctx->nip, ctx->hasCellIds, ctx->graph,
-                         ctx->nCells, ctx->stream);
+        hipError_t err = hipSuccess;
+        ctx->gdm.Prepare(ctx->params.kind, ctx->coordType, ctx->f[0].type, ctx->nip, ctx->graph, ctx->nCells,
+                         ctx->f[0].ranks.data(), ctx->hcconn.data(), ctx->hx.data(), ctx->hy.data(), ctx->hz.data(),
+                         ctx->hasCellIds ? ctx->hcellId.data() : nullptr, ctx->stream, err);
+        Check(err, "hex8 node tables");
         ctx->gdmTried = true;
     }
-    if (!ForceGeneric() &&
-        ctx->gdm.Usable(ctx->params.kind, ctx->coordType, ctx->f[0].type, ctx->nip, ctx->hasCellIds))
+    if (!ForceGeneric() && ctx->gdm.Usable(ctx->params.kind, ctx->coordType, ctx->f[0].type, ctx->nip))
     {
         RowArgs a = MakeRowArgs(ctx, u, e, kappa, R, K);
         ctx->gdm.Launch(a, wr, wk, ctx->stream);
@@ -539,6 +508,13 @@ int nuto_set_cells(nuto_ctx* ctx, int coord_type, int n_cells, int n_coord_nodes
         if (dim == 3)
             Upload(ctx, ctx->z, sz.data(), sz.size(), "coords");
         Upload(ctx, ctx->cconn, coord_conn, (size_t)n_cells * nn, "connectivity");
+        ctx->hx = std::move(sx);
+        ctx->hy = std::move(sy);
+        ctx->hz = std::move(sz);
+        ctx->hcconn.assign(coord_conn, coord_conn + (size_t)n_cells * nn);
+        ctx->hcellId.clear();
+        if (cell_ids)
+            ctx->hcellId.assign(cell_ids, cell_ids + n_cells);
         ctx->hasCellIds = cell_ids != nullptr;
         ctx->maxCellId = n_cells - 1;
         if (cell_ids)
