@@ -25,8 +25,10 @@
 // generic row-owner kernel.
 //
 // Gathers are three dependent loads deep: per-node incidence tables padded to 8 cells (cell,
-// local index, history column, neighbour slots), then the cell's dof node ranks and its history,
-// then the node values (coordinates are stored by dof node for this).
+// local index, history column, neighbour slots), then the cell's dof node ranks, its history and
+// its geometry, then the node values. The geometry of every (cell, IP) - J^-1 and detJ * w
+// (Jacobian.h:35-47, Cell.h:108-117) - depends on the coordinates only; it is computed once per
+// pattern by kGdmHex8Geometry with the same arithmetic and stored (80 bytes per IP).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -48,9 +50,11 @@ namespace gdm8
 {
 constexpr int kMaxInc = 8;   // incident cells per node
 constexpr int kMaxAdj = 32;  // neighbours per node
-constexpr int kSt = 30;      // doubles per (ip, cell) record; the 8 records of one IP start 60 dwords
-                             // apart (8 disjoint 4-bank groups) and are 16-byte aligned
-constexpr int kNd = 66;      // doubles per cell of node data (8 nodes x 8 + 2)
+// doubles per (ip, cell) record: 26 for FULL damage application, 30 with the UNILATERAL terms.
+// The 8 records of one IP start 52 (60) dwords apart: 8 disjoint 4-bank groups, 16-byte aligned.
+template <bool FULLDMG>
+constexpr int kStOf = FULLDMG ? 26 : 30;
+constexpr int kGeo = 10;     // doubles per (cell, IP) geometry record: J^-1 (row-major), detJ * w
 constexpr int kPs = 17;      // doubles per partial block (odd: 64 lanes spread over the banks)
 constexpr int kTab = 57;     // per-IP table row: w, dNc[8][3], N[8], dNd[8][3]
 
@@ -62,6 +66,18 @@ constexpr int sMn = 15;   // -N_a s d eps_eq / d eps (6)
 constexpr int sCg = 21;   // c s g_a (3)
 constexpr int sSn = 24;   // s N_a (1)
 constexpr int sKa = 25;   // UNILATERAL: c2 g_a (3)
+constexpr int kRed = 1384;  // doubles of partial blocks, R terms and the zero block after phase B
+
+// value of lane (lane & ~7) + n within each group of 8 lanes (ds_swizzle bit mode:
+// and-mask 0x18 keeps the group, or-mask n selects the lane)
+template <int n>
+__device__ inline double GroupLane(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_ds_swizzle((int)(b & 0xffffffffLL), 0x18 | (n << 5));
+    const int hi = __builtin_amdgcn_ds_swizzle((int)(b >> 32), 0x18 | (n << 5));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 
 constexpr double kGauss = 0.57735026918962573;  // sqrt(1/3), the 2-point Gauss abscissa
 constexpr double kThird = 1.0 / 3.0;
@@ -98,9 +114,7 @@ struct GdmHex8Tables
     const int* nodeHist;            // [node][8]: history column Cell::Id of that cell (nullptr: the cell index)
     const unsigned char* nodeLa;    // [node][8]: local index of the node in that cell
     const unsigned char* nodeSlot;  // [node][8][8]: neighbour slot (column block) of local node j
-    const double* xd;               // coordinates by dof node
-    const double* yd;
-    const double* zd;
+    const double* geo;              // [cell][ip][kGeo] geometry records
     int xcdMap;                     // 1: contiguous node ranges per XCD
 };
 
@@ -108,10 +122,10 @@ struct GdmHex8Tables
 struct NodeFetch
 {
     int cell, la, slot;
-    double v[7];  // x y z u0 u1 u2 e of node j of cell r
+    double v[4];  // u0 u1 u2 e of node j of cell r
+    double g[gdm8::kGeo];  // geometry of cell r at IP j
     double th[3];
     double kh;  // history kappa of cell r at IP j
-    double w;   // weight of IP j
 };
 
 __device__ inline void FetchNode(const RowArgs& A, const GdmHex8Tables& T, int p, int r, int j, NodeFetch& f)
@@ -120,20 +134,24 @@ __device__ inline void FetchNode(const RowArgs& A, const GdmHex8Tables& T, int p
     f.cell = T.nodeCell[pr];
     f.la = T.nodeLa[pr];
     f.slot = T.nodeSlot[pr * 8 + j];
-    f.w = A.tab[j * gdm8::kTab];
     f.kh = 0.0;
     if (f.cell >= 0)
     {
         const int hist = T.nodeHist ? T.nodeHist[pr] : f.cell;
         const int q = A.dconn[(size_t)f.cell * 8 + j];
         f.kh = A.kappa[(size_t)hist + (size_t)j * A.kappaLd];
-        f.v[0] = T.xd[q];
-        f.v[1] = T.yd[q];
-        f.v[2] = T.zd[q];
-        f.v[3] = A.u[(size_t)q * 3];
-        f.v[4] = A.u[(size_t)q * 3 + 1];
-        f.v[5] = A.u[(size_t)q * 3 + 2];
-        f.v[6] = A.e[q];
+        const double2* gr = reinterpret_cast<const double2*>(T.geo + ((size_t)f.cell * 8 + j) * gdm8::kGeo);
+#pragma unroll
+        for (int k = 0; k < gdm8::kGeo / 2; k++)
+        {
+            const double2 v = gr[k];
+            f.g[2 * k] = v.x;
+            f.g[2 * k + 1] = v.y;
+        }
+        f.v[0] = A.u[(size_t)q * 3];
+        f.v[1] = A.u[(size_t)q * 3 + 1];
+        f.v[2] = A.u[(size_t)q * 3 + 2];
+        f.v[3] = A.e[q];
         if (A.thE)
         {
             f.th[0] = A.thE[f.cell];
@@ -150,9 +168,10 @@ template <bool WR, bool WK, bool FULLDMG>
 __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, const GdmHex8Tables T)
 {
     using namespace gdm8;
-    __shared__ __align__(16) double state[8 * 8 * kSt];  // [ip][r][kSt]; later partial blocks + R terms
-    __shared__ __align__(16) double rowbuf[8 * kNd];      // node data of the incident cells, later p's rows
-    __shared__ __align__(8) unsigned char slotOf[kMaxAdj][kMaxInc];
+    constexpr int kSt = kStOf<FULLDMG>;
+    static_assert(8 * 8 * kSt * 8 >= kRed * 8 + kMaxAdj * kMaxInc, "LDS layout");
+    // [ip][r][kSt] records; after phase B: partial blocks, R terms, zero block and the slot map
+    __shared__ __align__(16) double state[8 * 8 * kSt];
 
     const int lane = threadIdx.x;
     const int r = lane >> 3, j = lane & 7;
@@ -175,20 +194,6 @@ __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, const GdmHex
     const bool valid = f.cell >= 0;  // r < number of cells incident to p
     const int la = f.la;
 
-    // ---- node data of p's cells to LDS; map slots -> (r, b) -----------------------------------------
-    double* nodeData = rowbuf;  // [r][n][8] (+2 per cell): x y z u0 u1 u2 e -
-    for (int i = lane; i < kMaxAdj * kMaxInc; i += 64)
-        (&slotOf[0][0])[i] = 0xFF;
-    __syncthreads();
-    if (valid)
-    {
-        double* nd = nodeData + r * kNd + j * 8;
-#pragma unroll
-        for (int k = 0; k < 7; k++)
-            nd[k] = f.v[k];
-        if (WK)
-            slotOf[f.slot][r] = (unsigned char)j;
-    }
     // material of the cell (per-cell fields: E, kappa0 = f_t / E, beta = f_t / G_f)
     double E = P.E, k0 = P.kappa0, beta = P.beta;
     if (A.thE && valid)
@@ -198,17 +203,13 @@ __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, const GdmHex
         beta = f.th[1] / f.th[2];
     }
     const double m3K = P.cK * E, m2G = P.cG * E;
-    __syncthreads();
 
     // ---- phase A: lane (r, ip) ---------------------------------------------------------------------
     double Ru0 = 0.0, Ru1 = 0.0, Ru2 = 0.0, Re = 0.0;
     if (valid)
     {
         const int ip = j;
-        const double w = f.w;
-        const double* nd = nodeData + r * kNd;
-        // J = sum_n X_n dN_n^T, Hu = sum_n u_n dN_n^T, he = sum_n e_n dN_n, eN = sum_n N_n e_n
-        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        // Hu = sum_n u_n dN_n^T, he = sum_n e_n dN_n, eN = sum_n N_n e_n (dN/dxi at the IP)
         double Hu[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
         double he[3] = {0, 0, 0};
         double eN = 0.0;
@@ -217,27 +218,39 @@ __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, const GdmHex
         {
             double d[3], Nv;
             ShapeAt(n, ip, d, Nv);
-            const double2 xy = *reinterpret_cast<const double2*>(nd + n * 8);
-            const double2 zu = *reinterpret_cast<const double2*>(nd + n * 8 + 2);
-            const double2 uv = *reinterpret_cast<const double2*>(nd + n * 8 + 4);
-            const double en = nd[n * 8 + 6];
-            const double X[3] = {xy.x, xy.y, zu.x}, U[3] = {zu.y, uv.x, uv.y};
+            // node n of cell r is held by lane (r, n)
+            double U[3], en;
+            switch (n)
+            {
+#define NUTO_GROUP_LANE(N)                                                                                 \
+    case N:                                                                                                \
+        U[0] = GroupLane<N>(f.v[0]);                                                                       \
+        U[1] = GroupLane<N>(f.v[1]);                                                                       \
+        U[2] = GroupLane<N>(f.v[2]);                                                                       \
+        en = GroupLane<N>(f.v[3]);                                                                         \
+        break;
+                NUTO_GROUP_LANE(0)
+                NUTO_GROUP_LANE(1)
+                NUTO_GROUP_LANE(2)
+                NUTO_GROUP_LANE(3)
+                NUTO_GROUP_LANE(4)
+                NUTO_GROUP_LANE(5)
+                NUTO_GROUP_LANE(6)
+                NUTO_GROUP_LANE(7)
+#undef NUTO_GROUP_LANE
+            }
 #pragma unroll
             for (int i = 0; i < 3; i++)
 #pragma unroll
                 for (int k = 0; k < 3; k++)
-                {
-                    J[i][k] += X[i] * d[k];
                     Hu[i][k] += U[i] * d[k];
-                }
             he[0] += en * d[0];
             he[1] += en * d[1];
             he[2] += en * d[2];
             eN += Nv * en;
         }
-        double inv[3][3];
-        const double detJ = Inverse<3>(J, inv);
-        const double s = detJ * w;
+        const double inv[3][3] = {{f.g[0], f.g[1], f.g[2]}, {f.g[3], f.g[4], f.g[5]}, {f.g[6], f.g[7], f.g[8]}};
+        const double s = f.g[9];
         // displacement gradient Gu = Hu J^-1, strain in Voigt order [xx, yy, zz, yz, zx, xy]
         double Gu[3][3];
 #pragma unroll
@@ -345,11 +358,14 @@ __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, const GdmHex
                 rec[sHa + i] = c1 * ga[i];
                 rec[sVa + i] = va[i];
                 rec[sCg + i] = P.c * s * ga[i];
-                rec[sKa + i] = FULLDMG ? 0.0 : 0.5 * s * d2G * ga[i];
+                if (!FULLDMG)
+                    rec[sKa + i] = 0.5 * s * d2G * ga[i];
             }
             rec[sSn] = s * Na;
-            rec[28] = 0.0;
-            rec[29] = 0.0;
+            if (FULLDMG)
+                rec[25] = 0.0;
+            else
+                rec[28] = rec[29] = 0.0;
 #pragma unroll
             for (int k = 0; k < kSt; k += 2)
                 *reinterpret_cast<double2*>(st + k) = make_double2(rec[k], rec[k + 1]);
@@ -371,9 +387,10 @@ __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, const GdmHex
             double db[3], Nb;
             ShapeAt(b, ip, db, Nb);
             const double* st = state + (ip * 8 + r) * kSt;
-            double rec[28];
+            constexpr int kRd = FULLDMG ? 26 : 28;
+            double rec[kRd];
 #pragma unroll
-            for (int k = 0; k < 28; k += 2)
+            for (int k = 0; k < kRd; k += 2)
             {
                 const double2 v = *reinterpret_cast<const double2*>(st + k);
                 rec[k] = v.x;
@@ -426,13 +443,16 @@ __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, const GdmHex
     }
     __syncthreads();
 
-    // ---- partial blocks and R terms to LDS -----------------------------------------------------------
+    // ---- partial blocks, R terms and the slot map to LDS (the records are dead now) -----------------
     double* part = state;                   // [r][b][kPs]
     double* rterm = state + 8 * 8 * kPs;    // [4][r][ip]
     double* rcell = rterm + 4 * 64;         // [4][r]: R terms summed over the IPs of cell r
     double* zero = rcell + 32;              // 8 zeros: the block of an absent (cell, slot) pair
+    // slotOf[q][r] = local index b of neighbour q in cell r, 0xFF if q is not a node of cell r
+    unsigned char(*slotOf)[kMaxInc] = reinterpret_cast<unsigned char(*)[kMaxInc]>(state + kRed);
     if (WK)
     {
+        reinterpret_cast<unsigned int*>(slotOf)[lane] = 0xFFFFFFFFu;  // 64 x 4 bytes = the whole map
 #pragma unroll
         for (int i = 0; i < 16; i++)
             part[lane * kPs + i] = acc[i];
@@ -447,9 +467,23 @@ __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, const GdmHex
         rterm[3 * 64 + lane] = Re;
     }
     __syncthreads();
+    if (WK && valid)
+        slotOf[f.slot][r] = (unsigned char)j;
+    // R: per cell over its IPs in order (Cell.h:108-117), then over the cells in order
+    if (WR && lane < 32)
+    {
+        const double* t = rterm + (lane >> 3) * 64 + (lane & 7) * 8;
+        double s = t[0];
+#pragma unroll
+        for (int ip = 1; ip < 8; ip++)
+            s += t[ip];
+        rcell[lane] = s;
+    }
+    __syncthreads();
 
-    // ---- reduce: lane (q, half) sums its 8 block entries over the cells in order ---------------------
-    // (absent cells add the zero block, so every lane runs the same 8 steps)
+    // ---- reduce: lane (q, half) sums its 8 block entries over the cells in order and writes them ----
+    // (absent cells add the zero block, so every lane runs the same 8 steps). Row layout: the three
+    // displacement rows of p are consecutive in the CSR (4 nA columns each), then the eeq row.
     if (WK)
     {
         const int q = lane >> 1, h = lane & 1;
@@ -474,51 +508,9 @@ __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, const GdmHex
                 const int e = h * 8 + k;
                 const int i = e >> 2, c = e & 3;
                 const int col = c < 3 ? 3 * q + c : 3 * nA + q;
-                rowbuf[i * L + col] = sum[k];
+                const size_t at = (i < 3 ? (size_t)ru0 + (size_t)i * L : (size_t)re0) + col;
+                A.K[at] = sum[k];
             }
-        }
-    }
-    // R: per cell over its IPs in order (Cell.h:108-117), then over the cells in order
-    if (WR && lane < 32)
-    {
-        const double* t = rterm + (lane >> 3) * 64 + (lane & 7) * 8;
-        double s = t[0];
-#pragma unroll
-        for (int ip = 1; ip < 8; ip++)
-            s += t[ip];
-        rcell[lane] = s;
-    }
-    __syncthreads();
-
-    // ---- write p's rows (displacement rows are consecutive in the CSR) -------------------------------
-    if (WK)
-    {
-        double v[6], ve[2];
-#pragma unroll
-        for (int it = 0; it < 6; it++)
-        {
-            const int i = lane + 64 * it;
-            v[it] = i < 3 * L ? rowbuf[i] : 0.0;
-        }
-#pragma unroll
-        for (int it = 0; it < 2; it++)
-        {
-            const int i = lane + 64 * it;
-            ve[it] = i < L ? rowbuf[3 * L + i] : 0.0;
-        }
-#pragma unroll
-        for (int it = 0; it < 6; it++)
-        {
-            const int i = lane + 64 * it;
-            if (i < 3 * L)
-                A.K[(size_t)ru0 + i] = v[it];
-        }
-#pragma unroll
-        for (int it = 0; it < 2; it++)
-        {
-            const int i = lane + 64 * it;
-            if (i < L)
-                A.K[(size_t)re0 + i] = ve[it];
         }
     }
     if (WR && lane < 4)
@@ -535,19 +527,55 @@ __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, const GdmHex
     }
 }
 
+// Geometry of every (cell, IP): J = sum_n X_n dN_n/dxi^T over the coordinate nodes in local
+// order (Jacobian.h:16-48), J^-1 and detJ (Jacobian.h:35-47), detJ * w (Cell.h:108-117).
+__global__ void __launch_bounds__(256) kGdmHex8Geometry(const double* x, const double* y, const double* z,
+                                                         const int* cconn, const double* tab, int nCells,
+                                                         double* geo)
+{
+    using namespace gdm8;
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;  // (cell, ip)
+    if (t >= (long long)nCells * 8)
+        return;
+    const int c = (int)(t >> 3), ip = (int)(t & 7);
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+    for (int n = 0; n < 8; n++)
+    {
+        double d[3], Nv;
+        ShapeAt(n, ip, d, Nv);
+        const int cn = cconn[(size_t)c * 8 + n];
+        const double X[3] = {x[cn], y[cn], z[cn]};
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                J[i][k] += X[i] * d[k];
+    }
+    double inv[3][3];
+    const double detJ = Inverse<3>(J, inv);
+    double* g = geo + (size_t)t * kGeo;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+            g[3 * a + b] = inv[a][b];
+    g[9] = detJ * tab[ip * kTab];
+}
+
 struct GdmHex8Plan
 {
     bool ready = false;
     int nNodes = 0;
     int grid = 0;     // workgroups: one per node (rounded up to a multiple of 8 with the XCD mapping)
     int xcdMap = 1;   // contiguous node ranges per XCD (env NUTO_AMD_GDM_XCD_MAP=0 switches it off)
-    DevBuf nodeCell, nodeHist, nodeLa, nodeSlot, xd, yd, zd;
+    DevBuf nodeCell, nodeHist, nodeLa, nodeSlot, geo;
 
     void Release()
     {
         ready = false;
         nNodes = 0;
-        for (DevBuf* b : {&nodeCell, &nodeHist, &nodeLa, &nodeSlot, &xd, &yd, &zd})
+        for (DevBuf* b : {&nodeCell, &nodeHist, &nodeLa, &nodeSlot, &geo})
             b->Release();
     }
 
@@ -556,11 +584,11 @@ struct GdmHex8Plan
         return ready && kind == kGradientDamage && coordType == kHex8 && dofType == kHex8 && nip == 8;
     }
 
-    // Builds the per-node tables from the node graph. Returns false (plan not used) when the mesh
-    // exceeds the kernel's bounds or a dof node sits on coordinate nodes with different
-    // coordinates; returns a HIP error through `err` when an upload fails.
-    bool Prepare(int kind, int coordType, int dofType, int nip, const NodeGraph& g, int nCells, const int* dconn,
-                 const int* cconn, const double* hx, const double* hy, const double* hz, const int* cellIds,
+    // Builds the per-node tables from the node graph and the geometry records from the device
+    // coordinates (x, y, z by coordinate node, cconn) and the IP table. Returns false (plan not
+    // used) when the mesh exceeds the kernel's bounds; a HIP failure is returned through `err`.
+    bool Prepare(int kind, int coordType, int dofType, int nip, const NodeGraph& g, int nCells, const int* cellIds,
+                 const double* x, const double* y, const double* z, const int* cconn, const double* tab,
                  hipStream_t stream, hipError_t& err)
     {
         Release();
@@ -571,22 +599,6 @@ struct GdmHex8Plan
         for (int p = 0; p < n; p++)
         {
             if (g.incPtr[p + 1] - g.incPtr[p] > gdm8::kMaxInc || g.adjPtr[p + 1] - g.adjPtr[p] > gdm8::kMaxAdj)
-                return false;
-        }
-        // coordinates by dof node: every coordinate node behind one dof node must agree exactly
-        std::vector<double> X(n), Y(n), Z(n);
-        std::vector<unsigned char> seen(n, 0);
-        for (size_t i = 0; i < (size_t)nCells * 8; i++)
-        {
-            const int q = dconn[i], c = cconn[i];
-            if (!seen[q])
-            {
-                seen[q] = 1;
-                X[q] = hx[c];
-                Y[q] = hy[c];
-                Z[q] = hz[c];
-            }
-            else if (X[q] != hx[c] || Y[q] != hy[c] || Z[q] != hz[c])
                 return false;
         }
         std::vector<int> cell((size_t)n * 8, -1), hist(cellIds ? (size_t)n * 8 : 0, 0);
@@ -607,13 +619,22 @@ struct GdmHex8Plan
         if ((err = nodeCell.Upload(cell.data(), cell.size() * sizeof(int), stream)) != hipSuccess ||
             (err = nodeLa.Upload(la.data(), la.size(), stream)) != hipSuccess ||
             (err = nodeSlot.Upload(slot.data(), slot.size(), stream)) != hipSuccess ||
-            (err = xd.Upload(X.data(), X.size() * sizeof(double), stream)) != hipSuccess ||
-            (err = yd.Upload(Y.data(), Y.size() * sizeof(double), stream)) != hipSuccess ||
-            (err = zd.Upload(Z.data(), Z.size() * sizeof(double), stream)) != hipSuccess ||
-            (cellIds && (err = nodeHist.Upload(hist.data(), hist.size() * sizeof(int), stream)) != hipSuccess))
+            (cellIds && (err = nodeHist.Upload(hist.data(), hist.size() * sizeof(int), stream)) != hipSuccess) ||
+            (err = geo.Alloc((size_t)nCells * 8 * gdm8::kGeo * sizeof(double))) != hipSuccess)
         {
             Release();
             return false;
+        }
+        if (nCells > 0)
+        {
+            const long long items = (long long)nCells * 8;
+            hipLaunchKernelGGL(kGdmHex8Geometry, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, stream, x, y,
+                               z, cconn, tab, nCells, geo.As<double>());
+            if ((err = hipGetLastError()) != hipSuccess || (err = hipStreamSynchronize(stream)) != hipSuccess)
+            {
+                Release();
+                return false;
+            }
         }
         nNodes = n;
         const char* env = std::getenv("NUTO_AMD_GDM_XCD_MAP");
@@ -626,8 +647,7 @@ struct GdmHex8Plan
     GdmHex8Tables Tables() const
     {
         return GdmHex8Tables{nodeCell.As<int>(), nodeHist.p ? nodeHist.As<int>() : nullptr,
-                             nodeLa.As<unsigned char>(), nodeSlot.As<unsigned char>(), xd.As<double>(),
-                             yd.As<double>(), zd.As<double>(), xcdMap};
+                             nodeLa.As<unsigned char>(), nodeSlot.As<unsigned char>(), geo.As<double>(), xcdMap};
     }
 
     template <bool FULLDMG>

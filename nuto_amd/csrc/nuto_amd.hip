@@ -44,8 +44,7 @@ struct nuto_ctx
     bool cellsSet = false;
     int coordType = 0, dim = 0, nCells = 0, nCoordNodes = 0;
     DevBuf x, y, z, cconn, cellId;
-    std::vector<double> hx, hy, hz;  // host copies of the cell group (pattern-time tables)
-    std::vector<int> hcconn, hcellId;
+    std::vector<int> hcellId;  // host copy of the cell ids (pattern-time tables)
     bool hasCellIds = false;
     int maxCellId = 0;
     Field f[2];
@@ -352,8 +351,8 @@ void Assemble(nuto_ctx* ctx, const double* u, const double* e, const double* kap
     {
         hipError_t err = hipSuccess;
         ctx->gdm.Prepare(ctx->params.kind, ctx->coordType, ctx->f[0].type, ctx->nip, ctx->graph, ctx->nCells,
-                         ctx->f[0].ranks.data(), ctx->hcconn.data(), ctx->hx.data(), ctx->hy.data(), ctx->hz.data(),
-                         ctx->hasCellIds ? ctx->hcellId.data() : nullptr, ctx->stream, err);
+                         ctx->hasCellIds ? ctx->hcellId.data() : nullptr, ctx->x.As<double>(), ctx->y.As<double>(),
+                         ctx->z.As<double>(), ctx->cconn.As<int>(), ctx->tab.As<double>(), ctx->stream, err);
         Check(err, "hex8 node tables");
         ctx->gdmTried = true;
     }
@@ -508,10 +507,6 @@ int nuto_set_cells(nuto_ctx* ctx, int coord_type, int n_cells, int n_coord_nodes
         if (dim == 3)
             Upload(ctx, ctx->z, sz.data(), sz.size(), "coords");
         Upload(ctx, ctx->cconn, coord_conn, (size_t)n_cells * nn, "connectivity");
-        ctx->hx = std::move(sx);
-        ctx->hy = std::move(sy);
-        ctx->hz = std::move(sz);
-        ctx->hcconn.assign(coord_conn, coord_conn + (size_t)n_cells * nn);
         ctx->hcellId.clear();
         if (cell_ids)
             ctx->hcellId.assign(cell_ids, cell_ids + n_cells);
