@@ -60,6 +60,54 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes_read": 8 * n, "copy_calibration_bytes_written": 8 * n}), flush=True)
 
 
+def adjoint_ratio(n, steps, warmup):
+    """Adjoint vs forward on the differentiation workload: hex8 gradient damage on an n^3 mesh of an
+    n mm cube with per-cell fields E ~ LN(30000, 0.1), f_t ~ LN(3, 0.1), G_f ~ LN(0.1, 0.15);
+    forward = fused R+K assembly, adjoint = nuto_vjp producing K^T lambda and dL/dtheta."""
+    import numpy as np
+    import torch
+
+    import nuto_amd as N
+    from nuto_amd import synthetic as S
+
+    X, C = N.structured_mesh(N.HEX8, n, n, n, (float(n), float(n), float(n)))
+    ranks, nn = N.dof_layer(N.HEX8, C, N.HEX8)
+    ctx = N.Context()
+    ctx.set_cells(N.HEX8, X, C)
+    ctx.set_dof_type(0, N.HEX8, 3, 3 * nn, N.cell_dofs(ranks, 3))
+    ctx.set_dof_type(1, N.HEX8, 1, nn, N.cell_dofs(ranks, 1))
+    ctx.set_integrand(N.gradient_damage_params())
+    ctx.set_rule(0, 2)
+    nnz, n_rows = ctx.build_pattern()
+    nc = C.shape[0]
+    E, ft, gf = S.material_fields(nc)
+    ctx.set_material_fields(E, ft, gf)
+    u = torch.from_numpy(S.uniform(S.SEED_U, 3 * nn, -5e-3, 5e-3)).cuda()
+    e = torch.from_numpy(S.uniform(S.SEED_E, nn, 0.0, 5e-4)).cuda()
+    kap = torch.from_numpy(S.uniform(S.SEED_KAPPA, nc * 8, 0.0, 3e-4)).cuda()
+    lam = torch.from_numpy(S.uniform(0x3001, n_rows, -1.0, 1.0)).cuda()
+    R = torch.empty(n_rows, dtype=torch.float64, device="cuda")
+    K = torch.empty(nnz, dtype=torch.float64, device="cuda")
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        for a, b in evs:
+            a.record()
+            fn()
+            b.record()
+        torch.cuda.synchronize()
+        return float(np.median([a.elapsed_time(b) for a, b in evs]))
+
+    fwd = timed(lambda: ctx.build_vector_matrix(u, e, kap, R, K))
+    adj = timed(lambda: ctx.vjp(u, e, kap, lam))
+    return {"workload": f"hex8 gradient damage {n}x{n}x{n} cells, per-cell lognormal (E, f_t, G_f)",
+            "forward_ms": fwd, "adjoint_ms": adj, "ratio": adj / fwd,
+            "adjoint_outputs": "K^T lambda (u, eeq) and dR/dtheta^T lambda (3 per cell)"}
+
+
 def cpu_baseline(n, threads, budget_s=30.0):
     """Times the oracle (NuTo's algorithm restated: per-thread triplet lists, critical splice,
     setFromTriplets) on an n^3 sample of the same workload."""
@@ -98,6 +146,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=24, help="cells per direction of the CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0, help="OpenMP threads for the CPU baseline (0: auto)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--adjoint-n", type=int, default=64, help="cells per direction of the adjoint workload")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -184,6 +234,10 @@ def main():
                      "kernel_ms": kernel_ms, "algorithmic_bytes": bytes_alg},
         "setup_s": setup_s,
     }
+    if not args.no_adjoint:
+        del R, K
+        torch.cuda.empty_cache()
+        rec["adjoint"] = adjoint_ratio(args.adjoint_n, args.steps, args.warmup)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         try:

@@ -113,6 +113,9 @@ int nuto_set_history_ld(nuto_ctx* ctx, int history_ld);
  * Host arrays of n_cells, or NULL to return to the scalar parameters. */
 int nuto_set_material_fields(nuto_ctx* ctx, const double* E, const double* ft, const double* gf);
 
+/* The same fields from device memory (n_cells doubles each), e.g. tensors that carry gradients. */
+int nuto_set_material_fields_device(nuto_ctx* ctx, const double* E, const double* ft, const double* gf);
+
 /* Sparsity of BuildMatrix's result (setFromTriplets pattern: every dof pair of every cell, explicit
  * zeros kept; SimpleAssembler.cpp:100-156), built once and kept on the device. */
 int nuto_build_pattern(nuto_ctx* ctx, long long* nnz, int* n_rows);
@@ -134,6 +137,18 @@ int nuto_build_vector_matrix(nuto_ctx* ctx, const double* u, const double* e, co
  * GradientDamage.h:122-125) or EvolutionImplicit::Update (LocalIsotropicDamage.h:130-133):
  * kappa(cell, ip) = max(kappa(cell, ip), eeq(ip)), in place. */
 int nuto_update_history(nuto_ctx* ctx, const double* u, const double* e, double* kappa);
+
+/* Adjoint of BuildVector for the gradient-damage integrand (no counterpart in the reference, which
+ * has no derivatives w.r.t. material parameters; its K^T lambda is the transpose of BuildMatrix,
+ * SimpleAssembler.cpp:100-156 with GradientDamage::Hessian0, GradientDamage.h:92-120).
+ * Given the cotangent lambda (n_rows, layout of R):
+ *   d_state (n_rows)       = K^T lambda = (dR / d(u, eeq))^T lambda, computed without forming K;
+ *   d_theta (3 x n_cells)  = (dR / d theta)^T lambda for theta = (E, f_t, G_f) per cell, stored
+ *                            [E row; f_t row; G_f row]; NULL skips it. Without material fields the
+ *                            scalar parameters are the point (f_t = kappa0 E, G_f = f_t / beta).
+ * kappa (the history) is a constant. All pointers are device pointers. Hex8 cells and dofs only. */
+int nuto_vjp(nuto_ctx* ctx, const double* u, const double* e, const double* kappa, const double* lambda,
+             double* d_state, double* d_theta);
 
 /* ---- device memory helpers for hosts without a HIP toolchain -------------------------------- */
 int nuto_device_alloc(nuto_ctx* ctx, void** ptr, unsigned long long bytes);

@@ -93,6 +93,8 @@ def lib():
         L.nuto_set_rule.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.nuto_set_history_ld.argtypes = [vp, ctypes.c_int]
         L.nuto_set_material_fields.argtypes = [vp, vp, vp, vp]
+        L.nuto_set_material_fields_device.argtypes = [vp, vp, vp, vp]
+        L.nuto_vjp.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.nuto_build_pattern.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ip]
         L.nuto_copy_pattern.argtypes = [vp, vp, vp]
         L.nuto_pattern_device.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp)]
@@ -228,6 +230,18 @@ class Context:
         a = [None if v is None else np.ascontiguousarray(v, np.float64) for v in (E, ft, gf)]
         self._chk(lib().nuto_set_material_fields(self.h, *[_np_ptr(v) for v in a]))
 
+    def set_material_fields_device(self, theta):
+        """Per-cell (E, f_t, G_f) from a (3, n_cells) float64 device tensor (None: scalar parameters)."""
+        if theta is None:
+            self._chk(lib().nuto_set_material_fields_device(self.h, None, None, None))
+            return
+        if theta.shape != (3, self.n_cells) or theta.dtype != self._torch.float64:
+            raise ValueError("theta must be a (3, n_cells) float64 tensor")
+        theta = theta.contiguous()
+        self._chk(lib().nuto_set_material_fields_device(self.h, _dev_ptr(theta[0]), _dev_ptr(theta[1]),
+                                                        _dev_ptr(theta[2])))
+        self._theta_keep = theta  # the copy is stream-ordered; keep the source alive until it ran
+
     def build_pattern(self):
         nnz, nr = ctypes.c_longlong(), ctypes.c_int()
         self._chk(lib().nuto_build_pattern(self.h, ctypes.byref(nnz), ctypes.byref(nr)))
@@ -265,6 +279,14 @@ class Context:
         self._chk(lib().nuto_build_vector_matrix(self.h, _dev_ptr(u), _dev_ptr(e), _dev_ptr(kappa), _dev_ptr(R),
                                                  _dev_ptr(K)))
         return R, K
+
+    def vjp(self, u, e, kappa, lam, want_theta=True):
+        """Adjoint of build_vector: (K^T lam, dR/dtheta^T lam as (3, n_cells) or None)."""
+        d_state = self._new(self.n_rows)
+        d_theta = self._new(3 * self.n_cells) if want_theta else None
+        self._chk(lib().nuto_vjp(self.h, _dev_ptr(u), _dev_ptr(e), _dev_ptr(kappa), _dev_ptr(lam), _dev_ptr(d_state),
+                                 _dev_ptr(d_theta)))
+        return d_state, (None if d_theta is None else d_theta.view(3, self.n_cells))
 
     def update_history(self, u, e, kappa):
         self._chk(lib().nuto_update_history(self.h, _dev_ptr(u), _dev_ptr(e), _dev_ptr(kappa)))

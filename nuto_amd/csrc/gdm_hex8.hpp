@@ -107,6 +107,159 @@ __device__ inline void GradOf(const double d[3], const double inv[3][3], double 
 }
 }  // namespace gdm8
 
+// Integration-point state of GradientDamage<3> (GradientDamage.h:70-135) from the displacement
+// gradient w.r.t. xi (Hu), the nonlocal-strain gradient w.r.t. xi (he), the nonlocal strain (eN),
+// J^-1 and the history kappa.
+struct GdmIp
+{
+    double eps[6];   // engineering strain, Voigt [xx, yy, zz, yz, zx, xy]
+    double ge[3];    // grad eeq
+    double eN;       // eeq at the IP
+    double dk;       // d kappa / d eeq
+    double kap, om, dom;  // kappa, damage and its derivative
+    double H;        // 1, or 0 for UNILATERAL application under volumetric compression
+    double d2G;      // (1 - omega) 2G
+    double sig[6], dsw[6];  // stress and d stress / d omega
+    double I1, Am, eq;      // modified Mises: I1, sqrt term, value
+};
+
+template <bool FULLDMG>
+__device__ inline void GdmIpState(const double Hu[3][3], const double he[3], double eN, const double inv[3][3],
+                                  double kh, double m3K, double m2G, double k0, double beta, const LawParams& P,
+                                  GdmIp& o)
+{
+    using namespace gdm8;
+    // displacement gradient Gu = Hu J^-1
+    double Gu[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            Gu[i][c] = Hu[i][0] * inv[0][c] + Hu[i][1] * inv[1][c] + Hu[i][2] * inv[2][c];
+    o.eps[0] = Gu[0][0];
+    o.eps[1] = Gu[1][1];
+    o.eps[2] = Gu[2][2];
+    o.eps[3] = Gu[1][2] + Gu[2][1];
+    o.eps[4] = Gu[2][0] + Gu[0][2];
+    o.eps[5] = Gu[0][1] + Gu[1][0];
+    GradOf(he, inv, o.ge);
+    o.eN = eN;
+    // kappa = max(history, eeq), d kappa / d eeq (GradientDamage.h:127-135)
+    o.kap = kh < eN ? eN : kh;
+    o.dk = eN >= kh ? 1.0 : 0.0;
+    // DamageLawExponential (DamageLawExponential.h:39-52)
+    o.om = 0.0;
+    o.dom = 0.0;
+    if (!(o.kap < k0))
+    {
+        const double ik = 1.0 / o.kap;
+        const double ex = exp(beta * (k0 - o.kap));
+        const double rr = k0 * ik;
+        o.om = 1.0 - rr * (1.0 - P.alpha + P.alpha * ex);
+        o.dom = rr * ((ik + beta) * P.alpha * ex + (1.0 - P.alpha) * ik);
+    }
+    // LinearElasticDamage<3> stress and dStress/dOmega (LinearElasticDamage.h:65-91)
+    const double* eps = o.eps;
+    const double eV = (eps[0] + eps[1] + eps[2]) * kThird;
+    o.H = (!FULLDMG && eV < 0.0) ? 0.0 : 1.0;
+    const double sV = (1.0 - o.om * o.H) * m3K * eV;
+    o.d2G = (1.0 - o.om) * m2G;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+    {
+        const double dev = eps[i] - eV;
+        o.sig[i] = sV + o.d2G * dev;
+        o.dsw[i] = -m3K * eV * o.H - m2G * dev;
+    }
+#pragma unroll
+    for (int i = 3; i < 6; i++)
+    {
+        o.sig[i] = 0.5 * o.d2G * eps[i];
+        o.dsw[i] = -0.5 * m2G * eps[i];
+    }
+    // ModifiedMisesStrainNorm<3> value (ModifiedMisesStrainNorm.h:136-146)
+    o.I1 = eps[0] + eps[1] + eps[2];
+    const double a0 = eps[0] - eps[1], a1 = eps[1] - eps[2], a2 = eps[2] - eps[0];
+    const double J2 =
+        (a0 * a0 + a1 * a1 + a2 * a2) * kSixth + 0.25 * (eps[3] * eps[3] + eps[4] * eps[4] + eps[5] * eps[5]);
+    o.Am = sqrt(P.K1 * P.K1 * o.I1 * o.I1 + P.K2 * J2);
+    o.eq = P.K1 * o.I1 + o.Am;
+}
+
+// d eps_eq / d eps of the modified Mises norm (ModifiedMisesStrainNorm.h:231-268, A == 0 branch)
+__device__ inline void GdmMisesDerivative(const GdmIp& o, const LawParams& P, double m[6])
+{
+    using namespace gdm8;
+    const double* eps = o.eps;
+    if (o.Am == 0.0)
+    {
+        m[0] = m[1] = m[2] = P.K1;
+        m[3] = m[4] = m[5] = 0.0;
+    }
+    else
+    {
+        const double h = 0.5 / o.Am;
+        const double t2 = 2.0 * P.K1 * P.K1 * o.I1;
+        m[0] = P.K1 + h * (t2 + P.K2 * (2.0 * eps[0] - eps[1] - eps[2]) * kThird);
+        m[1] = P.K1 + h * (t2 + P.K2 * (2.0 * eps[1] - eps[0] - eps[2]) * kThird);
+        m[2] = P.K1 + h * (t2 + P.K2 * (2.0 * eps[2] - eps[0] - eps[1]) * kThird);
+        m[3] = h * P.K2 * 0.5 * eps[3];
+        m[4] = h * P.K2 * 0.5 * eps[4];
+        m[5] = h * P.K2 * 0.5 * eps[5];
+    }
+}
+
+// Hu, he, eN of cell r at IP ip from the node values held by the 8 lanes of the cell
+// (node n by lane (r, n)): Hu = sum_n u_n dN_n/dxi^T, he = sum_n e_n dN_n/dxi, eN = sum_n N_n e_n
+__device__ inline void GatherGradients(const double v[4], int ip, double Hu[3][3], double he[3], double& eN)
+{
+    using namespace gdm8;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+    {
+        he[i] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            Hu[i][k] = 0.0;
+    }
+    eN = 0.0;
+#pragma unroll
+    for (int n = 0; n < 8; n++)
+    {
+        double d[3], Nv;
+        ShapeAt(n, ip, d, Nv);
+        double U[3], en;
+        switch (n)
+        {
+#define NUTO_GROUP_LANE(N)                                                                                 \
+    case N:                                                                                                \
+        U[0] = GroupLane<N>(v[0]);                                                                         \
+        U[1] = GroupLane<N>(v[1]);                                                                         \
+        U[2] = GroupLane<N>(v[2]);                                                                         \
+        en = GroupLane<N>(v[3]);                                                                           \
+        break;
+            NUTO_GROUP_LANE(0)
+            NUTO_GROUP_LANE(1)
+            NUTO_GROUP_LANE(2)
+            NUTO_GROUP_LANE(3)
+            NUTO_GROUP_LANE(4)
+            NUTO_GROUP_LANE(5)
+            NUTO_GROUP_LANE(6)
+            NUTO_GROUP_LANE(7)
+#undef NUTO_GROUP_LANE
+        }
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                Hu[i][k] += U[i] * d[k];
+        he[0] += en * d[0];
+        he[1] += en * d[1];
+        he[2] += en * d[2];
+        eN += Nv * en;
+    }
+}
+
 // Per-node tables of the node kernel (built once per pattern, see GdmHex8Plan::Prepare).
 struct GdmHex8Tables
 {
@@ -122,6 +275,7 @@ struct GdmHex8Tables
 struct NodeFetch
 {
     int cell, la, slot;
+    int q;  // dof node of (r, j)
     double v[4];  // u0 u1 u2 e of node j of cell r
     double g[gdm8::kGeo];  // geometry of cell r at IP j
     double th[3];
@@ -135,10 +289,12 @@ __device__ inline void FetchNode(const RowArgs& A, const GdmHex8Tables& T, int p
     f.la = T.nodeLa[pr];
     f.slot = T.nodeSlot[pr * 8 + j];
     f.kh = 0.0;
+    f.q = 0;
     if (f.cell >= 0)
     {
         const int hist = T.nodeHist ? T.nodeHist[pr] : f.cell;
         const int q = A.dconn[(size_t)f.cell * 8 + j];
+        f.q = q;
         f.kh = A.kappa[(size_t)hist + (size_t)j * A.kappaLd];
         const double2* gr = reinterpret_cast<const double2*>(T.geo + ((size_t)f.cell * 8 + j) * gdm8::kGeo);
 #pragma unroll
@@ -209,100 +365,16 @@ __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, const GdmHex
     if (valid)
     {
         const int ip = j;
-        // Hu = sum_n u_n dN_n^T, he = sum_n e_n dN_n, eN = sum_n N_n e_n (dN/dxi at the IP)
-        double Hu[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-        double he[3] = {0, 0, 0};
-        double eN = 0.0;
-#pragma unroll
-        for (int n = 0; n < 8; n++)
-        {
-            double d[3], Nv;
-            ShapeAt(n, ip, d, Nv);
-            // node n of cell r is held by lane (r, n)
-            double U[3], en;
-            switch (n)
-            {
-#define NUTO_GROUP_LANE(N)                                                                                 \
-    case N:                                                                                                \
-        U[0] = GroupLane<N>(f.v[0]);                                                                       \
-        U[1] = GroupLane<N>(f.v[1]);                                                                       \
-        U[2] = GroupLane<N>(f.v[2]);                                                                       \
-        en = GroupLane<N>(f.v[3]);                                                                         \
-        break;
-                NUTO_GROUP_LANE(0)
-                NUTO_GROUP_LANE(1)
-                NUTO_GROUP_LANE(2)
-                NUTO_GROUP_LANE(3)
-                NUTO_GROUP_LANE(4)
-                NUTO_GROUP_LANE(5)
-                NUTO_GROUP_LANE(6)
-                NUTO_GROUP_LANE(7)
-#undef NUTO_GROUP_LANE
-            }
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int k = 0; k < 3; k++)
-                    Hu[i][k] += U[i] * d[k];
-            he[0] += en * d[0];
-            he[1] += en * d[1];
-            he[2] += en * d[2];
-            eN += Nv * en;
-        }
+        double Hu[3][3], he[3], eN;
+        GatherGradients(f.v, ip, Hu, he, eN);
         const double inv[3][3] = {{f.g[0], f.g[1], f.g[2]}, {f.g[3], f.g[4], f.g[5]}, {f.g[6], f.g[7], f.g[8]}};
         const double s = f.g[9];
-        // displacement gradient Gu = Hu J^-1, strain in Voigt order [xx, yy, zz, yz, zx, xy]
-        double Gu[3][3];
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int c = 0; c < 3; c++)
-                Gu[i][c] = Hu[i][0] * inv[0][c] + Hu[i][1] * inv[1][c] + Hu[i][2] * inv[2][c];
-        const double eps[6] = {Gu[0][0], Gu[1][1], Gu[2][2], Gu[1][2] + Gu[2][1], Gu[2][0] + Gu[0][2],
-                               Gu[0][1] + Gu[1][0]};
-        double ge[3];
-        GradOf(he, inv, ge);
-
-        // kappa = max(history, eeq), d kappa / d eeq (GradientDamage.h:127-135)
-        const double kh = f.kh;
-        const double kap = kh < eN ? eN : kh;
-        const double dk = eN >= kh ? 1.0 : 0.0;
-        // DamageLawExponential (DamageLawExponential.h:39-52)
-        double om = 0.0, dom = 0.0;
-        if (!(kap < k0))
-        {
-            const double ik = 1.0 / kap;
-            const double ex = exp(beta * (k0 - kap));
-            const double rr = k0 * ik;
-            om = 1.0 - rr * (1.0 - P.alpha + P.alpha * ex);
-            dom = rr * ((ik + beta) * P.alpha * ex + (1.0 - P.alpha) * ik);
-        }
-        // LinearElasticDamage<3> stress and dStress/dOmega (LinearElasticDamage.h:65-91)
-        const double eV = (eps[0] + eps[1] + eps[2]) * kThird;
-        const double H = (!FULLDMG && eV < 0.0) ? 0.0 : 1.0;
-        const double sV = (1.0 - om * H) * m3K * eV;
-        const double d2G = (1.0 - om) * m2G;
-        double sig[6], dsw[6];
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-        {
-            const double dev = eps[i] - eV;
-            sig[i] = sV + d2G * dev;
-            dsw[i] = -m3K * eV * H - m2G * dev;
-        }
-#pragma unroll
-        for (int i = 3; i < 6; i++)
-        {
-            sig[i] = 0.5 * d2G * eps[i];
-            dsw[i] = -0.5 * m2G * eps[i];
-        }
-        // ModifiedMisesStrainNorm<3> value and derivative (ModifiedMisesStrainNorm.h:136-146,231-268)
-        const double I1 = eps[0] + eps[1] + eps[2];
-        const double a0 = eps[0] - eps[1], a1 = eps[1] - eps[2], a2 = eps[2] - eps[0];
-        const double J2 = (a0 * a0 + a1 * a1 + a2 * a2) * kSixth +
-                          0.25 * (eps[3] * eps[3] + eps[4] * eps[4] + eps[5] * eps[5]);
-        const double Am = sqrt(P.K1 * P.K1 * I1 * I1 + P.K2 * J2);
-        const double eq = P.K1 * I1 + Am;
+        GdmIp st;
+        GdmIpState<FULLDMG>(Hu, he, eN, inv, f.kh, m3K, m2G, k0, beta, P, st);
+        const double* sig = st.sig;
+        const double* dsw = st.dsw;
+        const double* ge = st.ge;
+        const double om = st.om, dom = st.dom, dk = st.dk, H = st.H, d2G = st.d2G, eq = st.eq;
 
         double da[3], Na, ga[3];
         ShapeAt(la, ip, da, Na);
@@ -319,23 +391,8 @@ __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, const GdmHex
         if (WK)
         {
             double m[6];
-            if (Am == 0.0)
-            {
-                m[0] = m[1] = m[2] = P.K1;
-                m[3] = m[4] = m[5] = 0.0;
-            }
-            else
-            {
-                const double h = 0.5 / Am;
-                const double t2 = 2.0 * P.K1 * P.K1 * I1;
-                m[0] = P.K1 + h * (t2 + P.K2 * (2.0 * eps[0] - eps[1] - eps[2]) * kThird);
-                m[1] = P.K1 + h * (t2 + P.K2 * (2.0 * eps[1] - eps[0] - eps[2]) * kThird);
-                m[2] = P.K1 + h * (t2 + P.K2 * (2.0 * eps[2] - eps[0] - eps[1]) * kThird);
-                m[3] = h * P.K2 * 0.5 * eps[3];
-                m[4] = h * P.K2 * 0.5 * eps[4];
-                m[5] = h * P.K2 * 0.5 * eps[5];
-            }
-            double* st = state + (ip * 8 + r) * kSt;
+            GdmMisesDerivative(st, P, m);
+            double* rp = state + (ip * 8 + r) * kSt;
             double rec[kSt];
 #pragma unroll
             for (int a = 0; a < 3; a++)
@@ -368,7 +425,7 @@ __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, const GdmHex
                 rec[28] = rec[29] = 0.0;
 #pragma unroll
             for (int k = 0; k < kSt; k += 2)
-                *reinterpret_cast<double2*>(st + k) = make_double2(rec[k], rec[k + 1]);
+                *reinterpret_cast<double2*>(rp + k) = make_double2(rec[k], rec[k + 1]);
         }
     }
     __syncthreads();

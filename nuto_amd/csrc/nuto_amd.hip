@@ -12,6 +12,7 @@
 #include "assemble_rows.hpp"
 #include "elements.hpp"
 #include "gdm_hex8.hpp"
+#include "gdm_hex8_vjp.hpp"
 #include "mesh.hpp"
 
 using namespace nuto_amd;
@@ -336,6 +337,19 @@ void CheckReady(nuto_ctx* ctx, const double* u, const double* e, const double* k
     }
 }
 
+// Builds the hex8 node kernel's tables once per pattern (a no-op for other meshes / integrands).
+void PrepareGdm(nuto_ctx* ctx)
+{
+    if (ctx->gdmTried)
+        return;
+    hipError_t err = hipSuccess;
+    ctx->gdm.Prepare(ctx->params.kind, ctx->coordType, ctx->f[0].type, ctx->nip, ctx->graph, ctx->nCells,
+                     ctx->hasCellIds ? ctx->hcellId.data() : nullptr, ctx->x.As<double>(), ctx->y.As<double>(),
+                     ctx->z.As<double>(), ctx->cconn.As<int>(), ctx->tab.As<double>(), ctx->stream, err);
+    Check(err, "hex8 node tables");
+    ctx->gdmTried = true;
+}
+
 void Assemble(nuto_ctx* ctx, const double* u, const double* e, const double* kappa, double* R, double* K,
               bool needR, bool needK)
 {
@@ -347,15 +361,7 @@ void Assemble(nuto_ctx* ctx, const double* u, const double* e, const double* kap
     const bool wr = R != nullptr, wk = K != nullptr;
     if (!wr && !wk)
         return;
-    if (!ctx->gdmTried)
-    {
-        hipError_t err = hipSuccess;
-        ctx->gdm.Prepare(ctx->params.kind, ctx->coordType, ctx->f[0].type, ctx->nip, ctx->graph, ctx->nCells,
-                         ctx->hasCellIds ? ctx->hcellId.data() : nullptr, ctx->x.As<double>(), ctx->y.As<double>(),
-                         ctx->z.As<double>(), ctx->cconn.As<int>(), ctx->tab.As<double>(), ctx->stream, err);
-        Check(err, "hex8 node tables");
-        ctx->gdmTried = true;
-    }
+    PrepareGdm(ctx);
     if (!ForceGeneric() && ctx->gdm.Usable(ctx->params.kind, ctx->coordType, ctx->f[0].type, ctx->nip))
     {
         RowArgs a = MakeRowArgs(ctx, u, e, kappa, R, K);
@@ -626,6 +632,32 @@ int nuto_set_material_fields(nuto_ctx* ctx, const double* E, const double* ft, c
     });
 }
 
+int nuto_set_material_fields_device(nuto_ctx* ctx, const double* E, const double* ft, const double* gf)
+{
+    return Guard(ctx, [&] {
+        Require(ctx->cellsSet, NUTO_ERR_STATE, "set the cells first");
+        if (!E && !ft && !gf)
+        {
+            ctx->hasTheta = false;
+            return;
+        }
+        Require(E && ft && gf, NUTO_ERR_ARGUMENT, "E, ft and gf must all be given");
+        const size_t bytes = (size_t)ctx->nCells * sizeof(double);
+        for (DevBuf* b : {&ctx->thE, &ctx->thFt, &ctx->thGf})
+        {
+            if (b->bytes != bytes || !b->p)
+                Check(b->Alloc(bytes), "material fields");
+        }
+        if (bytes)
+        {
+            Check(hipMemcpyAsync(ctx->thE.p, E, bytes, hipMemcpyDeviceToDevice, ctx->stream), "material fields");
+            Check(hipMemcpyAsync(ctx->thFt.p, ft, bytes, hipMemcpyDeviceToDevice, ctx->stream), "material fields");
+            Check(hipMemcpyAsync(ctx->thGf.p, gf, bytes, hipMemcpyDeviceToDevice, ctx->stream), "material fields");
+        }
+        ctx->hasTheta = true;
+    });
+}
+
 int nuto_build_pattern(nuto_ctx* ctx, long long* nnz, int* n_rows)
 {
     return Guard(ctx, [&] {
@@ -724,6 +756,27 @@ int nuto_update_history(nuto_ctx* ctx, const double* u, const double* e, double*
             BuildTables(ctx);
         RowArgs a = MakeRowArgs(ctx, u, e, kappa, nullptr, nullptr);
         Dispatch(ctx, a, false, false, 1, kappa);
+    });
+}
+
+int nuto_vjp(nuto_ctx* ctx, const double* u, const double* e, const double* kappa, const double* lambda,
+             double* d_state, double* d_theta)
+{
+    return Guard(ctx, [&] {
+        CheckReady(ctx, u, e, kappa);
+        Require(ctx->params.kind == kGradientDamage, NUTO_ERR_UNSUPPORTED,
+                "the adjoint is provided for the gradient-damage integrand");
+        Require(lambda != nullptr && d_state != nullptr, NUTO_ERR_ARGUMENT, "lambda / d_state is NULL");
+        if (!ctx->tabValid)
+            BuildTables(ctx);
+        PrepareGdm(ctx);
+        Require(ctx->gdm.Usable(ctx->params.kind, ctx->coordType, ctx->f[0].type, ctx->nip), NUTO_ERR_UNSUPPORTED,
+                "the adjoint needs hex8 cells and dofs, 2x2x2 Gauss points and at most 8 cells and 32 "
+                "neighbours per node");
+        RowArgs a = MakeRowArgs(ctx, u, e, kappa, nullptr, nullptr);
+        LaunchGdmHex8Vjp(ctx->gdm, a, lambda, d_state, d_theta, ctx->stream);
+        Check(hipGetLastError(), "vjp launch");
+        SyncCheck(ctx, "vjp kernel");
     });
 }
 

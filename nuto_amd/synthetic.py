@@ -50,3 +50,17 @@ SEED_E_MOD = 0x2001
 SEED_FT = 0x2002
 SEED_GF = 0x2003
 SEED_PERTURB = 0x2004
+
+
+def material_fields(n_cells):
+    """Per-cell (E, f_t, G_f): E ~ LN(mean 30000, CV 0.1), f_t ~ LN(3, 0.1), G_f ~ LN(0.1, 0.15)."""
+    return (lognormal_mean_cv(SEED_E_MOD, n_cells, 30000.0, 0.1), lognormal_mean_cv(SEED_FT, n_cells, 3.0, 0.1),
+            lognormal_mean_cv(SEED_GF, n_cells, 0.1, 0.15))
+
+
+def perturbed_fields(E, ft, gf, scale=0.05):
+    """Seeded perturbation of the fields (the point r_obs is observed at): each value times
+    exp(scale * z), z standard normal from stream SEED_PERTURB (one counter range per field)."""
+    n = len(E)
+    z = normal(SEED_PERTURB, 3 * n).reshape(3, n)
+    return E * np.exp(scale * z[0]), ft * np.exp(scale * z[1]), gf * np.exp(scale * z[2])

@@ -1,0 +1,153 @@
+"""Adjoint (nuto_vjp, torch.autograd) of the hex8 gradient-damage residual.
+
+References: dL/d(u, eeq) against K^T lambda with K = BuildMatrix of the oracle (the reference
+gradient, GradientDamage.h:92-120 is the tangent of GradientDamage.h:70-90); dL/dtheta against
+central finite differences of the oracle's R with relative step 1e-6 (the reference has no
+derivatives w.r.t. material parameters, so this part is pinned by FD only).
+"""
+import numpy as np
+import pytest
+import torch
+
+import nuto_amd as N
+import oracle as O
+from nuto_amd import synthetic as S
+from pairs import Pair, make_mesh, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def gdm_pair(n, app, jitter=0.0, fields=True):
+    X, C = make_mesh(N.HEX8, n, lengths=tuple(float(v) for v in n), jitter=jitter)
+    dev = N.gradient_damage_params(damage_application=app)
+    orc = O.Params(kind=1, dim=3, E=30000.0, nu=0.2, k=10.0, kappa0=1e-4, beta=100.0, alpha=0.99, c=4.0,
+                   damage_application=app)
+    p = Pair(N.HEX8, N.HEX8, X, C, dev, orc, True)
+    th = S.material_fields(p.ncells) if fields else None
+    return p, th
+
+
+def loss_parts(p, th, u, e, kap):
+    """r_obs = R at the perturbed fields (oracle), lambda = R(theta) - r_obs."""
+    if th is not None:
+        p.orc.set_theta(*S.perturbed_fields(*th))
+    r_obs = p.orc.assemble(u, e, kap, want_k=False)["R"]
+    if th is not None:
+        p.orc.set_theta(*th)
+    ref = p.orc.assemble(u, e, kap)
+    return r_obs, ref
+
+
+def kt_lambda(ref, lam, n_rows):
+    rp, ci, val, bnd = ref["row_ptr"], ref["col"], ref["val"], ref["bound"]
+    rows = np.repeat(np.arange(n_rows), np.diff(rp))
+    kt = np.zeros(n_rows)
+    np.add.at(kt, ci, val * lam[rows])
+    kb = np.zeros(n_rows)
+    np.add.at(kb, ci, bnd * np.abs(lam[rows]))
+    return kt, kb
+
+
+@pytest.mark.parametrize("app", [N.FULL, N.UNILATERAL])
+def test_vjp_state_is_K_transpose_lambda(app):
+    p, th = gdm_pair((4, 3, 3), app, jitter=0.1)
+    u, e, kap = p.inputs()
+    r_obs, ref = loss_parts(p, th, u, e, kap)
+    lam = ref["R"] - r_obs
+    ctx = p.context()
+    ctx.set_material_fields(*th)
+    d_state, _ = ctx.vjp(to_dev(u), to_dev(e), to_dev(kap), to_dev(lam), want_theta=False)
+    got = d_state.cpu().numpy()
+    kt, kb = kt_lambda(ref, lam, len(lam))
+    err = np.abs(got - kt)
+    assert (err <= 1e-12 * kb + 1e-300).all(), f"max err/bound {np.max(err / (kb + 1e-300)):.3e}"
+
+
+def test_vjp_theta_matches_central_fd():
+    p, th = gdm_pair((2, 2, 2), N.FULL)
+    u, e, kap = p.inputs()
+    r_obs, ref = loss_parts(p, th, u, e, kap)
+    lam = ref["R"] - r_obs
+    ctx = p.context()
+    ctx.set_material_fields(*th)
+    _, d_theta = ctx.vjp(to_dev(u), to_dev(e), to_dev(kap), to_dev(lam))
+    got = d_theta.cpu().numpy()
+
+    def L(E, ft, gf):
+        p.orc.set_theta(E, ft, gf)
+        R = p.orc.assemble(u, e, kap, want_k=False)["R"]
+        return 0.5 * np.sum((R - r_obs) ** 2)
+
+    base = [np.array(v, dtype=np.float64) for v in th]
+    fd = np.zeros_like(got)
+    for k in range(3):
+        for c in range(p.ncells):
+            h = 1e-6 * base[k][c]
+            plus = [v.copy() for v in base]
+            minus = [v.copy() for v in base]
+            plus[k][c] += h
+            minus[k][c] -= h
+            fd[k, c] = (L(*plus) - L(*minus)) / (2 * h)
+    p.orc.set_theta(*th)
+    scale = np.abs(fd).max(axis=1, keepdims=True)
+    assert (np.abs(got - fd) <= 1e-6 * scale).all(), f"max rel {np.max(np.abs(got - fd) / scale):.3e}"
+
+
+def test_vjp_scalar_parameters_theta_point():
+    """Without fields the gradient is taken at f_t = kappa0 E, G_f = f_t / beta."""
+    p, _ = gdm_pair((3, 2, 2), N.FULL, fields=False)
+    u, e, kap = p.inputs()
+    n = p.ncells
+    E, ft = np.full(n, 30000.0), np.full(n, 1e-4 * 30000.0)
+    gf = ft / 100.0
+    ctx = p.context()
+    lam = S.uniform(0x77, p.n_u + p.n_e, -1.0, 1.0)
+    _, d0 = ctx.vjp(to_dev(u), to_dev(e), to_dev(kap), to_dev(lam))
+    ctx.set_material_fields(E, ft, gf)
+    _, d1 = ctx.vjp(to_dev(u), to_dev(e), to_dev(kap), to_dev(lam))
+    a, b = d0.cpu().numpy(), d1.cpu().numpy()
+    assert np.allclose(a, b, rtol=1e-13, atol=1e-13 * np.abs(b).max())
+
+
+def test_autograd_function_against_KT_lambda_and_directional_fd():
+    p, th = gdm_pair((5, 4, 3), N.FULL)
+    u, e, kap = p.inputs()
+    r_obs, ref = loss_parts(p, th, u, e, kap)
+    ctx = p.context()
+    tu = to_dev(u).requires_grad_(True)
+    te = to_dev(e).requires_grad_(True)
+    tk = to_dev(kap)
+    theta = torch.from_numpy(np.stack(th)).cuda().requires_grad_(True)
+    robs = to_dev(r_obs)
+    R = N.autograd.residual(ctx, tu, te, tk, theta)
+    loss = 0.5 * ((R - robs) ** 2).sum()
+    loss.backward()
+    lam = R.detach().cpu().numpy() - r_obs
+    kt, kb = kt_lambda(ref, lam, len(lam))
+    got = np.concatenate([tu.grad.cpu().numpy(), te.grad.cpu().numpy()])
+    assert (np.abs(got - kt) <= 1e-12 * kb + 1e-300).all()
+    # directional derivative along a seeded direction in theta, central FD on the device residual
+    v = np.stack([S.uniform(0x91 + k, p.ncells, -1.0, 1.0) for k in range(3)]) * np.stack(th)
+    tv = torch.from_numpy(v).cuda()
+    h = 1e-6
+    with torch.no_grad():
+        def Lat(t):
+            Rt = N.autograd.residual(ctx, tu.detach(), te.detach(), tk, t)
+            return 0.5 * float(((Rt - robs) ** 2).sum())
+        fd = (Lat(theta.detach() + h * tv) - Lat(theta.detach() - h * tv)) / (2 * h)
+    ad = float((theta.grad * tv).sum())
+    assert abs(ad - fd) <= 1e-6 * abs(fd)
+
+
+def test_vjp_rejects_other_integrands():
+    X, C = make_mesh(N.HEX8, (2, 2, 2))
+    ranks, nn = N.dof_layer(N.HEX8, C, N.HEX8)
+    ctx = N.Context()
+    ctx.set_cells(N.HEX8, X, C)
+    ctx.set_dof_type(0, N.HEX8, 3, 3 * nn, N.cell_dofs(ranks, 3))
+    ctx.set_integrand(N.linear_elastic_params(3, 30000.0, 0.2))
+    ctx.set_rule(0, 2)
+    ctx.build_pattern()
+    u = to_dev(np.zeros(3 * nn))
+    with pytest.raises(N.NutoError):
+        ctx.vjp(u, None, None, u)
