@@ -150,6 +150,58 @@ int nuto_update_history(nuto_ctx* ctx, const double* u, const double* e, double*
 int nuto_vjp(nuto_ctx* ctx, const double* u, const double* e, const double* kappa, const double* lambda,
              double* d_state, double* d_theta);
 
+/* ---- batches of independent meshes -------------------------------------------------------------
+ * No counterpart in the reference, which assembles one cell group per SimpleAssembler call
+ * (SimpleAssembler.cpp:38-156); a batch runs that call for many meshes at once. Every mesh is one
+ * cell group with its own dof numbering (as nuto_set_cells + nuto_set_dof_type would take it) and
+ * the batch's integrand; its rule is the default of its element pair (tet4/tet4: tetrahedron order
+ * 1, tet4/tet10: order 2, hex8/hex8: 2x2x2 Gauss, hex8/hex20: 3x3x3 Gauss).
+ *
+ * Layout: the batch's u, eeq, R, K, history and d_theta arrays hold all meshes; nuto_batch_mesh_info
+ * gives each mesh's offsets:
+ *   info[0] group       info[1] dof nodes   info[2] cells          info[3] IPs per cell
+ *   info[4] u offset    info[5] eeq offset (-1)                    info[6] R offset (displacement rows)
+ *   info[7] R offset (eeq rows, -1)         info[8] rows of the mesh
+ *   info[9] K offset (displacement rows)    info[10] their nonzeros
+ *   info[11] K offset (eeq rows, -1)        info[12] their nonzeros
+ *   info[13] history offset of (cell 0, ip 0); (cell c, ip i) at info[13] + c + i * info[14]
+ *   info[15] d_theta offset of cell 0 (E); f_t and G_f follow at strides of info[16]
+ * Within its slices a mesh's R and K are exactly nuto_build_vector / nuto_build_matrix of that mesh
+ * (monolithic CSR of nuto_batch_copy_pattern). */
+typedef struct nuto_batch nuto_batch;
+int nuto_batch_create(nuto_batch** batch, int device);
+int nuto_batch_destroy(nuto_batch* batch);
+const char* nuto_batch_last_error(const nuto_batch* batch);
+int nuto_batch_set_stream(nuto_batch* batch, void* stream); /* non-NULL for graph replay (mode -1) */
+int nuto_batch_synchronize(nuto_batch* batch);
+int nuto_batch_set_integrand(nuto_batch* batch, const nuto_params* params); /* dim 3 integrands */
+/* cell_ranks: n_cells x nodes(dof_type) dof node ranks (nuto_dof_layer); mesh receives its index */
+int nuto_batch_add_mesh(nuto_batch* batch, int coord_type, int dof_type, int n_cells, int n_coord_nodes,
+                        const double* coords, const int* coord_conn, int n_dof_nodes, const int* cell_ranks,
+                        int* mesh);
+int nuto_batch_build(nuto_batch* batch); /* patterns of all meshes, once */
+/* sizes[6]: lengths of u, eeq, R, K, history and d_theta */
+int nuto_batch_sizes(nuto_batch* batch, long long* sizes);
+int nuto_batch_mesh_info(nuto_batch* batch, int mesh, long long* info /* 17 */);
+int nuto_batch_copy_pattern(nuto_batch* batch, int mesh, int* row_ptr, int* col);
+/* per-cell (E, f_t, G_f) over all cells, meshes in the order they were added (host arrays) */
+int nuto_batch_set_material_fields(nuto_batch* batch, const double* E, const double* ft, const double* gf);
+/* R and K of every mesh. mode -1: the tuned launch (mode 3 until tuned) replayed from a HIP graph; 0..2: one fused launch
+ * over all meshes with 64 / 128 / 256 nodes per workgroup; 3: one launch per element type over all
+ * meshes of that type; 4: one launch per mesh (what separate nuto_build_vector_matrix calls do). */
+int nuto_batch_assemble(nuto_batch* batch, const double* u, const double* e, const double* kappa, double* R,
+                        double* K, int mode);
+/* Times modes 0..4 (ms[5], -1 where not applicable) and makes the fastest of 0..3 the tuned mode. */
+int nuto_batch_autotune(nuto_batch* batch, const double* u, const double* e, const double* kappa, double* R,
+                        double* K, double* ms, int* best);
+/* Gradient damage: nuto_vjp for every mesh (d_state layout of R, d_theta layout of info[15]). */
+int nuto_batch_vjp(nuto_batch* batch, const double* u, const double* e, const double* kappa, const double* lambda,
+                   double* d_state, double* d_theta);
+/* Gradient damage, loss L = 0.5 ||R - r_obs||^2 of every mesh in one graph replay: R and K, then
+ * lambda = R - r_obs, then d_state = K^T lambda and d_theta = dR/dtheta^T lambda. */
+int nuto_batch_residual_adjoint(nuto_batch* batch, const double* u, const double* e, const double* kappa,
+                                const double* r_obs, double* R, double* K, double* d_state, double* d_theta);
+
 /* ---- device memory helpers for hosts without a HIP toolchain -------------------------------- */
 int nuto_device_alloc(nuto_ctx* ctx, void** ptr, unsigned long long bytes);
 int nuto_device_free(nuto_ctx* ctx, void* ptr);

@@ -95,6 +95,24 @@ def lib():
         L.nuto_set_material_fields.argtypes = [vp, vp, vp, vp]
         L.nuto_set_material_fields_device.argtypes = [vp, vp, vp, vp]
         L.nuto_vjp.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.nuto_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int]
+        L.nuto_batch_destroy.argtypes = [vp]
+        L.nuto_batch_last_error.restype = ctypes.c_char_p
+        L.nuto_batch_last_error.argtypes = [vp]
+        L.nuto_batch_set_stream.argtypes = [vp, vp]
+        L.nuto_batch_synchronize.argtypes = [vp]
+        L.nuto_batch_set_integrand.argtypes = [vp, ctypes.POINTER(Params)]
+        L.nuto_batch_add_mesh.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp,
+                                          ctypes.c_int, vp, ip]
+        L.nuto_batch_build.argtypes = [vp]
+        L.nuto_batch_sizes.argtypes = [vp, vp]
+        L.nuto_batch_mesh_info.argtypes = [vp, ctypes.c_int, vp]
+        L.nuto_batch_copy_pattern.argtypes = [vp, ctypes.c_int, vp, vp]
+        L.nuto_batch_set_material_fields.argtypes = [vp, vp, vp, vp]
+        L.nuto_batch_assemble.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_int]
+        L.nuto_batch_autotune.argtypes = [vp, vp, vp, vp, vp, vp, vp, ip]
+        L.nuto_batch_vjp.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.nuto_batch_residual_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.nuto_build_pattern.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ip]
         L.nuto_copy_pattern.argtypes = [vp, vp, vp]
         L.nuto_pattern_device.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp)]
@@ -291,3 +309,135 @@ class Context:
     def update_history(self, u, e, kappa):
         self._chk(lib().nuto_update_history(self.h, _dev_ptr(u), _dev_ptr(e), _dev_ptr(kappa)))
         return kappa
+
+
+class Batch:
+    """Many independent meshes assembled together (the C ABI nuto_batch).
+
+    The batch runs on a torch stream of its own (graph replay needs a non-null stream); every call
+    first orders that stream after torch's current stream and afterwards orders the current stream
+    after it, so results can be used by torch right away.
+    """
+
+    MESH_INFO = ("group", "dof_nodes", "cells", "nip", "u", "e", "R_u", "R_e", "rows", "K_u", "nnz_u", "K_e", "nnz_e",
+                 "history", "history_ld", "theta", "theta_stride")
+
+    def __init__(self, device=0):
+        import torch
+
+        if not torch.cuda.is_available():
+            raise RuntimeError("nuto_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+        self._torch = torch
+        L = lib()
+        self._L = L
+        h = ctypes.c_void_p()
+        r = L.nuto_batch_create(ctypes.byref(h), device)
+        if r:
+            raise NutoError(r, "nuto_batch_create failed")
+        self.h = h
+        self.stream = torch.cuda.Stream(device=device)
+        self._chk(L.nuto_batch_set_stream(self.h, ctypes.c_void_p(self.stream.cuda_stream)))
+        self.n_meshes = 0
+        self.sizes = None
+
+    def close(self):
+        h, L = getattr(self, "h", None), getattr(self, "_L", None)
+        if h and L is not None:
+            L.nuto_batch_destroy(h)
+        self.h = None
+
+    __del__ = close
+
+    def _chk(self, r):
+        if r:
+            raise NutoError(r, lib().nuto_batch_last_error(self.h).decode())
+
+    def _enter(self):
+        self.stream.wait_stream(self._torch.cuda.current_stream())
+
+    def _leave(self):
+        self._torch.cuda.current_stream().wait_stream(self.stream)
+
+    def set_integrand(self, params):
+        self._chk(lib().nuto_batch_set_integrand(self.h, ctypes.byref(params)))
+
+    def add_mesh(self, coord_type, dof_type, coords, conn, ranks, n_dof_nodes):
+        coords = np.ascontiguousarray(coords, np.float64)
+        conn = np.ascontiguousarray(conn, np.int32)
+        ranks = np.ascontiguousarray(ranks, np.int32)
+        m = ctypes.c_int()
+        self._chk(lib().nuto_batch_add_mesh(self.h, coord_type, dof_type, conn.shape[0], coords.shape[0],
+                                            _np_ptr(coords), _np_ptr(conn), n_dof_nodes, _np_ptr(ranks),
+                                            ctypes.byref(m)))
+        self.n_meshes = m.value + 1
+        return m.value
+
+    def build(self):
+        self._chk(lib().nuto_batch_build(self.h))
+        sz = np.zeros(6, np.int64)
+        self._chk(lib().nuto_batch_sizes(self.h, _np_ptr(sz)))
+        self.sizes = dict(zip(("u", "e", "R", "K", "history", "theta"), (int(v) for v in sz)))
+        return self.sizes
+
+    def mesh_info(self, m):
+        info = np.zeros(17, np.int64)
+        self._chk(lib().nuto_batch_mesh_info(self.h, m, _np_ptr(info)))
+        return dict(zip(self.MESH_INFO, (int(v) for v in info)))
+
+    def pattern(self, m):
+        i = self.mesh_info(m)
+        rp = np.zeros(i["rows"] + 1, np.int32)
+        ci = np.zeros(i["nnz_u"] + i["nnz_e"], np.int32)
+        self._chk(lib().nuto_batch_copy_pattern(self.h, m, _np_ptr(rp), _np_ptr(ci)))
+        return rp, ci
+
+    def set_material_fields(self, E=None, ft=None, gf=None):
+        a = [None if v is None else np.ascontiguousarray(v, np.float64) for v in (E, ft, gf)]
+        self._chk(lib().nuto_batch_set_material_fields(self.h, *[_np_ptr(v) for v in a]))
+
+    def _new(self, n):
+        return self._torch.empty(max(n, 1), dtype=self._torch.float64, device="cuda")
+
+    def assemble(self, u, e=None, kappa=None, R=None, K=None, mode=-1):
+        R = self._new(self.sizes["R"]) if R is None else R
+        K = self._new(self.sizes["K"]) if K is None else K
+        self._enter()
+        self._chk(lib().nuto_batch_assemble(self.h, _dev_ptr(u), _dev_ptr(e), _dev_ptr(kappa), _dev_ptr(R),
+                                            _dev_ptr(K), mode))
+        self._leave()
+        return R, K
+
+    def autotune(self, u, e=None, kappa=None, R=None, K=None):
+        R = self._new(self.sizes["R"]) if R is None else R
+        K = self._new(self.sizes["K"]) if K is None else K
+        ms = np.zeros(5)
+        best = ctypes.c_int()
+        self._enter()
+        self._chk(lib().nuto_batch_autotune(self.h, _dev_ptr(u), _dev_ptr(e), _dev_ptr(kappa), _dev_ptr(R),
+                                            _dev_ptr(K), _np_ptr(ms), ctypes.byref(best)))
+        self._leave()
+        return ms, best.value
+
+    def vjp(self, u, e, kappa, lam):
+        d_state = self._new(self.sizes["R"])
+        d_theta = self._new(self.sizes["theta"])
+        self._enter()
+        self._chk(lib().nuto_batch_vjp(self.h, _dev_ptr(u), _dev_ptr(e), _dev_ptr(kappa), _dev_ptr(lam),
+                                       _dev_ptr(d_state), _dev_ptr(d_theta)))
+        self._leave()
+        return d_state, d_theta
+
+    def residual_adjoint(self, u, e, kappa, r_obs, R=None, K=None, d_state=None, d_theta=None):
+        R = self._new(self.sizes["R"]) if R is None else R
+        K = self._new(self.sizes["K"]) if K is None else K
+        d_state = self._new(self.sizes["R"]) if d_state is None else d_state
+        d_theta = self._new(self.sizes["theta"]) if d_theta is None else d_theta
+        self._enter()
+        self._chk(lib().nuto_batch_residual_adjoint(self.h, _dev_ptr(u), _dev_ptr(e), _dev_ptr(kappa),
+                                                    _dev_ptr(r_obs), _dev_ptr(R), _dev_ptr(K), _dev_ptr(d_state),
+                                                    _dev_ptr(d_theta)))
+        self._leave()
+        return R, K, d_state, d_theta
+
+    def synchronize(self):
+        self._chk(lib().nuto_batch_synchronize(self.h))

@@ -98,13 +98,13 @@ __device__ inline void Grad(const double* dNd, int b, const double inv[DIM][DIM]
     }
 }
 
+// Rows of dof node p (p < pEnd <= A.nNodes): one lane per node, incident cells in group order.
 template <int KIND, int DIM, int NNC, int NND, int NIP, int BH, bool WR, bool WK>
-__global__ void __launch_bounds__(64) kAssembleRows(const RowArgs A)
+__device__ inline void AssembleRowsBody(const RowArgs& A, int p, int pEnd)
 {
     constexpr int F = KIND == kGradientDamage ? 4 : DIM;
     constexpr int S = 1 + NNC * DIM + NND + NND * DIM;
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= A.nNodes)
+    if (p >= pEnd)
         return;
     const int beg = A.incPtr[p], end = A.incPtr[p + 1];
     const int nA = A.adjPtr[p + 1] - A.adjPtr[p];
@@ -434,6 +434,12 @@ __global__ void __launch_bounds__(64) kAssembleRows(const RowArgs A)
         if constexpr (KIND == kGradientDamage)
             A.R[(size_t)A.nU + p] = racc[3];
     }
+}
+
+template <int KIND, int DIM, int NNC, int NND, int NIP, int BH, bool WR, bool WK>
+__global__ void __launch_bounds__(64) kAssembleRows(const RowArgs A)
+{
+    AssembleRowsBody<KIND, DIM, NNC, NND, NIP, BH, WR, WK>(A, blockIdx.x * blockDim.x + threadIdx.x, A.nNodes);
 }
 
 // History update (GradientDamage::Update, GradientDamage.h:122-125, or EvolutionImplicit::Update,

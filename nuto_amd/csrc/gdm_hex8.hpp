@@ -269,6 +269,7 @@ struct GdmHex8Tables
     const unsigned char* nodeSlot;  // [node][8][8]: neighbour slot (column block) of local node j
     const double* geo;              // [cell][ip][kGeo] geometry records
     int xcdMap;                     // 1: contiguous node ranges per XCD
+    int pBegin, pEnd;               // nodes [pBegin, pEnd) of the launch
 };
 
 // Inputs of lane (r, j) of node p.
@@ -337,9 +338,10 @@ __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, const GdmHex
     {
         const int per = (int)((gridDim.x + 7) >> 3);
         p = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
-        if (p >= A.nNodes)
-            return;
     }
+    p += T.pBegin;
+    if (p >= T.pEnd)
+        return;
     // p's rows: 3 displacement rows (consecutive in the CSR) and the eeq row, each 4 nA long
     const int ru0 = A.rowPtr[(size_t)p * 3];
     const int L = A.rowPtr[(size_t)p * 3 + 1] - ru0;
@@ -704,7 +706,8 @@ struct GdmHex8Plan
     GdmHex8Tables Tables() const
     {
         return GdmHex8Tables{nodeCell.As<int>(), nodeHist.p ? nodeHist.As<int>() : nullptr,
-                             nodeLa.As<unsigned char>(), nodeSlot.As<unsigned char>(), geo.As<double>(), xcdMap};
+                             nodeLa.As<unsigned char>(), nodeSlot.As<unsigned char>(), geo.As<double>(), xcdMap, 0,
+                             nNodes};
     }
 
     template <bool FULLDMG>
@@ -727,6 +730,21 @@ struct GdmHex8Plan
             LaunchT<true>(a, wr, wk, stream);
         else
             LaunchT<false>(a, wr, wk, stream);
+    }
+
+    // R + K of the nodes [first, end) only (one mesh of a batch), plain block order
+    void LaunchRange(const RowArgs& a, int first, int end, hipStream_t stream)
+    {
+        if (end <= first)
+            return;
+        GdmHex8Tables t = Tables();
+        t.xcdMap = 0;
+        t.pBegin = first;
+        t.pEnd = end;
+        if (a.P.damage_application == 0)
+            hipLaunchKernelGGL((kGdmHex8Node<true, true, true>), dim3(end - first), dim3(64), 0, stream, a, t);
+        else
+            hipLaunchKernelGGL((kGdmHex8Node<true, true, false>), dim3(end - first), dim3(64), 0, stream, a, t);
     }
 };
 

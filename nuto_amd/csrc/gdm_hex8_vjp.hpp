@@ -44,9 +44,10 @@ __global__ void __launch_bounds__(64) kGdmHex8Vjp(const RowArgs A, const GdmHex8
     {
         const int per = (int)((gridDim.x + 7) >> 3);
         p = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
-        if (p >= A.nNodes)
-            return;
     }
+    p += T.pBegin;
+    if (p >= T.pEnd)
+        return;
     NodeFetch f;
     FetchNode(A, T, p, r, j, f);
     const bool valid = f.cell >= 0;

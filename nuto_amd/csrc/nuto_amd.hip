@@ -13,6 +13,7 @@
 #include "elements.hpp"
 #include "gdm_hex8.hpp"
 #include "gdm_hex8_vjp.hpp"
+#include "batch.hpp"
 #include "mesh.hpp"
 
 using namespace nuto_amd;
@@ -804,6 +805,744 @@ int nuto_copy_to_device(nuto_ctx* ctx, void* dst, const void* src, unsigned long
 int nuto_copy_to_host(nuto_ctx* ctx, void* dst, const void* src, unsigned long long bytes)
 {
     return Guard(ctx, [&] { Check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "copy to host"); });
+}
+
+}  // extern "C"
+
+// ================================================================================================
+// Batches of independent meshes (declared in include/nuto_amd.h, "batches")
+// ================================================================================================
+
+namespace
+{
+
+struct BatchMesh
+{
+    int ct = 0, dt = 0, nCells = 0, nCoord = 0, nDof = 0;
+    std::vector<double> coords;
+    std::vector<int> conn, ranks;
+    int group = -1, nodeOff = 0, cellOff = 0, coordOff = 0;
+};
+
+struct BatchGroup
+{
+    int ct = 0, dt = 0, type = 0;
+    nuto_ctx* ctx = nullptr;
+    std::vector<int> meshes;
+    int nNodes = 0, nCells = 0, nCoord = 0, nip = 0, nRows = 0;
+    long long nnz = 0;
+    long long uOff = 0, eOff = 0, ROff = 0, KOff = 0, hOff = 0, tOff = 0;  // offsets into the batch arrays
+};
+
+// default rule of each (coordinate, dof) element pair: tensor Gauss 2 / 3, tetrahedron order 1 / 2
+bool BatchTypeOf(int ct, int dt, int& type, int& ruleKind, int& ruleN)
+{
+    if (ct == kTet4 && dt == kTet4)
+        type = kBtTet4, ruleKind = 1, ruleN = 1;
+    else if (ct == kTet4 && dt == kTet10)
+        type = kBtTet10, ruleKind = 1, ruleN = 2;
+    else if (ct == kHex8 && dt == kHex8)
+        type = kBtHex8, ruleKind = 0, ruleN = 2;
+    else if (ct == kHex8 && dt == kHex20)
+        type = kBtHex20, ruleKind = 0, ruleN = 3;
+    else
+        return false;
+    return true;
+}
+
+constexpr int kBatchModes = 5;  // 0..2 fused launch, 64/128/256 nodes per tile; 3 one launch per type; 4 per mesh
+constexpr int kBatchBlock[3] = {64, 128, 256};
+
+}  // namespace
+
+struct nuto_batch
+{
+    int device = 0;
+    hipStream_t own = nullptr, stream = nullptr;
+    std::string err;
+    nuto_params params{};
+    bool integrandSet = false;
+    std::vector<BatchMesh> meshes;
+    std::vector<BatchGroup> groups;
+    bool built = false;
+    long long uLen = 0, eLen = 0, RLen = 0, KLen = 0, hLen = 0, tLen = 0;
+    DevBuf args;  // RowArgs per group for the fused kernel
+    const void* argPtrs[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    DevBuf tiles[3];
+    int nTiles[3] = {0, 0, 0};
+    int tuned = -1;  // launch mode chosen by nuto_batch_autotune (-1: not tuned)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graphKey = -1;  // mode the graph was captured for (+100 for forward + adjoint)
+    const void* graphPtrs[10] = {};
+    DevBuf lambda;      // forward + adjoint: lambda = R - r_obs
+};
+
+namespace
+{
+
+template <class F>
+int GuardB(nuto_batch* b, F&& fn)
+{
+    if (!b)
+        return NUTO_ERR_ARGUMENT;
+    try
+    {
+        Check(hipSetDevice(b->device), "hipSetDevice");
+        fn();
+        b->err.clear();
+        return NUTO_OK;
+    }
+    catch (const Failure& f)
+    {
+        b->err = f.msg;
+        return f.code;
+    }
+    catch (const std::bad_alloc&)
+    {
+        b->err = "host allocation failed";
+        return NUTO_ERR_MEMORY;
+    }
+    catch (const std::exception& e)
+    {
+        b->err = e.what();
+        return NUTO_ERR_ARGUMENT;
+    }
+}
+
+// a failing call on a group context becomes a failure of the batch
+void CheckCtx(nuto_ctx* ctx, int status)
+{
+    if (status != NUTO_OK)
+        throw Failure{status, std::string("batch group: ") + ctx->err};
+}
+
+void DropGraph(nuto_batch* b)
+{
+    if (b->exec)
+        (void)hipGraphExecDestroy(b->exec);
+    if (b->graph)
+        (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->graphKey = -1;
+}
+
+void ReleaseGroups(nuto_batch* b)
+{
+    DropGraph(b);
+    for (BatchGroup& g : b->groups)
+        if (g.ctx)
+            nuto_destroy(g.ctx);
+    b->groups.clear();
+    b->built = false;
+    b->tuned = -1;
+    for (auto& p : b->argPtrs)
+        p = nullptr;
+}
+
+RowArgs GroupArgs(nuto_batch* b, const BatchGroup& g, const double* u, const double* e, const double* kappa,
+                  double* R, double* K)
+{
+    return MakeRowArgs(g.ctx, u + g.uOff, e ? e + g.eOff : nullptr, kappa ? kappa + g.hOff : nullptr, R + g.ROff,
+                       K + g.KOff);
+}
+
+// Device copy of the per-group arguments of the fused kernel, refreshed when the caller's arrays change.
+void UploadArgs(nuto_batch* b, const double* u, const double* e, const double* kappa, double* R, double* K)
+{
+    const void* now[5] = {u, e, kappa, R, K};
+    if (std::equal(now, now + 5, b->argPtrs) && b->args.p)
+        return;
+    std::vector<RowArgs> a;
+    for (const BatchGroup& g : b->groups)
+        a.push_back(GroupArgs(b, g, u, e, kappa, R, K));
+    const size_t bytes = a.size() * sizeof(RowArgs);
+    if (!b->args.p || b->args.bytes != bytes)
+    {
+        DropGraph(b);  // a captured graph refers to the old buffer
+        Check(b->args.Alloc(bytes), "batch arguments");
+    }
+    // the buffer is rewritten in place: graphs captured with it stay valid
+    Check(hipMemcpyAsync(b->args.p, a.data(), bytes, hipMemcpyHostToDevice, b->stream), "batch arguments");
+    Check(hipStreamSynchronize(b->stream), "batch arguments");
+    std::copy(now, now + 5, b->argPtrs);
+}
+
+template <int KIND>
+void LaunchFused(nuto_batch* b, int which)
+{
+    const int n = b->nTiles[which];
+    if (n == 0)
+        return;
+    const RowArgs* a = b->args.As<RowArgs>();
+    const BatchTile* t = b->tiles[which].As<BatchTile>();
+    if (which == 0)
+        hipLaunchKernelGGL((kBatchRows<KIND, 64>), dim3(n), dim3(64), 0, b->stream, a, t);
+    else if (which == 1)
+        hipLaunchKernelGGL((kBatchRows<KIND, 128>), dim3(n), dim3(128), 0, b->stream, a, t);
+    else
+        hipLaunchKernelGGL((kBatchRows<KIND, 256>), dim3(n), dim3(256), 0, b->stream, a, t);
+}
+
+template <int KIND>
+void LaunchRange(const RowArgs& a, int type, int first, int end, hipStream_t stream)
+{
+    if (end <= first)
+        return;
+    const dim3 grid((end - first + 63) / 64), block(64);
+    if (type == kBtTet4)
+        hipLaunchKernelGGL((kBatchRowsRange<KIND, kBtTet4>), grid, block, 0, stream, a, first, end);
+    else if (type == kBtTet10)
+        hipLaunchKernelGGL((kBatchRowsRange<KIND, kBtTet10>), grid, block, 0, stream, a, first, end);
+    else if (type == kBtHex8)
+        hipLaunchKernelGGL((kBatchRowsRange<KIND, kBtHex8>), grid, block, 0, stream, a, first, end);
+    else
+        hipLaunchKernelGGL((kBatchRowsRange<KIND, kBtHex20>), grid, block, 0, stream, a, first, end);
+}
+
+bool IsGdm(const nuto_batch* b)
+{
+    return b->params.kind == kGradientDamage;
+}
+
+// The kernels of one batched R + K assembly in launch mode `mode` (no synchronisation).
+void LaunchBatch(nuto_batch* b, int mode, const double* u, const double* e, const double* kappa, double* R, double* K)
+{
+    const int kind = b->params.kind;
+    if (IsGdm(b) && mode < 3)
+        mode = 3;
+    if (mode < 3)
+    {
+        UploadArgs(b, u, e, kappa, R, K);
+        if (kind == kLinearElastic)
+            LaunchFused<kLinearElastic>(b, mode);
+        else
+            LaunchFused<kLocalDamage>(b, mode);
+    }
+    else
+    {
+        for (BatchGroup& g : b->groups)
+        {
+            RowArgs a = GroupArgs(b, g, u, e, kappa, R, K);
+            if (mode == 3)
+            {
+                if (IsGdm(b))
+                    g.ctx->gdm.Launch(a, true, true, b->stream);
+                else if (kind == kLinearElastic)
+                    LaunchRange<kLinearElastic>(a, g.type, 0, g.nNodes, b->stream);
+                else
+                    LaunchRange<kLocalDamage>(a, g.type, 0, g.nNodes, b->stream);
+                continue;
+            }
+            for (int m : g.meshes)
+            {
+                const BatchMesh& bm = b->meshes[m];
+                const int first = bm.nodeOff, end = bm.nodeOff + bm.nDof;
+                if (IsGdm(b))
+                    g.ctx->gdm.LaunchRange(a, first, end, b->stream);
+                else if (kind == kLinearElastic)
+                    LaunchRange<kLinearElastic>(a, g.type, first, end, b->stream);
+                else
+                    LaunchRange<kLocalDamage>(a, g.type, first, end, b->stream);
+            }
+        }
+    }
+    Check(hipGetLastError(), "batch launch");
+}
+
+__global__ void kResidualMinus(const double* __restrict__ R, const double* __restrict__ robs, double* __restrict__ lam,
+                               long long n)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n)
+        lam[i] = R[i] - robs[i];
+}
+
+void LaunchVjp(nuto_batch* b, const double* u, const double* e, const double* kappa, const double* lam,
+               double* dstate, double* dtheta)
+{
+    for (BatchGroup& g : b->groups)
+    {
+        RowArgs a = MakeRowArgs(g.ctx, u + g.uOff, e + g.eOff, kappa + g.hOff, nullptr, nullptr);
+        LaunchGdmHex8Vjp(g.ctx->gdm, a, lam + g.ROff, dstate + g.ROff, dtheta ? dtheta + g.tOff : nullptr, b->stream);
+    }
+    Check(hipGetLastError(), "batch vjp launch");
+}
+
+// Replays (capturing first if needed) a graph of `capture`, keyed by `key` and the pointers.
+template <class F>
+void GraphRun(nuto_batch* b, int key, const void* const* ptrs, int nptr, F&& capture)
+{
+    Require(b->stream != nullptr, NUTO_ERR_STATE, "graph replay needs a non-null stream (nuto_batch_set_stream)");
+    bool same = b->exec && b->graphKey == key;
+    for (int i = 0; same && i < nptr; i++)
+        same = b->graphPtrs[i] == ptrs[i];
+    if (!same)
+    {
+        DropGraph(b);
+        Check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+        try
+        {
+            capture();
+        }
+        catch (...)
+        {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(b->stream, &g);
+            if (g)
+                (void)hipGraphDestroy(g);
+            throw;
+        }
+        Check(hipStreamEndCapture(b->stream, &b->graph), "graph capture");
+        Check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "graph instantiate");
+        b->graphKey = key;
+        for (int i = 0; i < nptr; i++)
+            b->graphPtrs[i] = ptrs[i];
+    }
+    Check(hipGraphLaunch(b->exec, b->stream), "graph launch");
+}
+
+void RequireBuilt(nuto_batch* b, const double* u, const double* e, const double* kappa)
+{
+    Require(b->built, NUTO_ERR_STATE, "batch not built (nuto_batch_build)");
+    Require(u != nullptr, NUTO_ERR_ARGUMENT, "u is NULL");
+    Require(!IsGdm(b) || e != nullptr, NUTO_ERR_ARGUMENT, "nonlocal equivalent strain vector is NULL");
+    Require(b->params.kind == kLinearElastic || kappa != nullptr, NUTO_ERR_ARGUMENT, "history (kappa) is NULL");
+}
+
+}  // namespace
+
+extern "C" {
+
+int nuto_batch_create(nuto_batch** out, int device)
+{
+    if (!out)
+        return NUTO_ERR_ARGUMENT;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        return NUTO_ERR_DEVICE;
+    if (device < 0 || device >= count)
+        return NUTO_ERR_ARGUMENT;
+    nuto_batch* b = new (std::nothrow) nuto_batch;
+    if (!b)
+        return NUTO_ERR_MEMORY;
+    b->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&b->own, hipStreamNonBlocking) != hipSuccess)
+    {
+        delete b;
+        return NUTO_ERR_DEVICE;
+    }
+    b->stream = b->own;
+    *out = b;
+    return NUTO_OK;
+}
+
+int nuto_batch_destroy(nuto_batch* b)
+{
+    if (!b)
+        return NUTO_ERR_ARGUMENT;
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->stream);
+    ReleaseGroups(b);
+    if (b->own)
+        (void)hipStreamDestroy(b->own);
+    delete b;
+    return NUTO_OK;
+}
+
+const char* nuto_batch_last_error(const nuto_batch* b)
+{
+    return b ? b->err.c_str() : "null batch";
+}
+
+int nuto_batch_set_stream(nuto_batch* b, void* stream)
+{
+    return GuardB(b, [&] {
+        DropGraph(b);
+        b->stream = (hipStream_t)stream;
+        for (BatchGroup& g : b->groups)
+            CheckCtx(g.ctx, nuto_set_stream(g.ctx, stream));
+    });
+}
+
+int nuto_batch_synchronize(nuto_batch* b)
+{
+    return GuardB(b, [&] { Check(hipStreamSynchronize(b->stream), "synchronize"); });
+}
+
+int nuto_batch_set_integrand(nuto_batch* b, const nuto_params* params)
+{
+    return GuardB(b, [&] {
+        Require(params != nullptr, NUTO_ERR_ARGUMENT, "params is NULL");
+        Require(params->dim == 3, NUTO_ERR_UNSUPPORTED, "batches hold 3D meshes");
+        Require(params->kind == kLinearElastic || params->kind == kLocalDamage || params->kind == kGradientDamage,
+                NUTO_ERR_ARGUMENT, "unknown integrand kind");
+        ReleaseGroups(b);
+        b->params = *params;
+        b->integrandSet = true;
+    });
+}
+
+int nuto_batch_add_mesh(nuto_batch* b, int coord_type, int dof_type, int n_cells, int n_coord_nodes,
+                        const double* coords, const int* coord_conn, int n_dof_nodes, const int* cell_ranks, int* mesh)
+{
+    return GuardB(b, [&] {
+        int type, rk, rn;
+        Require(BatchTypeOf(coord_type, dof_type, type, rk, rn), NUTO_ERR_UNSUPPORTED,
+                "batches hold tet4/tet4, tet4/tet10, hex8/hex8 and hex8/hex20 meshes");
+        Require(n_cells >= 0 && n_coord_nodes >= 0 && n_dof_nodes >= 0, NUTO_ERR_ARGUMENT, "negative sizes");
+        Require(coords && coord_conn && cell_ranks, NUTO_ERR_ARGUMENT, "NULL mesh array");
+        const int nnc = NodesOf(coord_type), nnd = NodesOf(dof_type);
+        BatchMesh m;
+        m.ct = coord_type;
+        m.dt = dof_type;
+        m.nCells = n_cells;
+        m.nCoord = n_coord_nodes;
+        m.nDof = n_dof_nodes;
+        m.coords.assign(coords, coords + (size_t)n_coord_nodes * 3);
+        m.conn.assign(coord_conn, coord_conn + (size_t)n_cells * nnc);
+        m.ranks.assign(cell_ranks, cell_ranks + (size_t)n_cells * nnd);
+        for (int v : m.conn)
+            Require(v >= 0 && v < n_coord_nodes, NUTO_ERR_ARGUMENT, "coordinate connectivity out of range");
+        for (int v : m.ranks)
+            Require(v >= 0 && v < n_dof_nodes, NUTO_ERR_ARGUMENT, "dof node rank out of range");
+        ReleaseGroups(b);
+        b->meshes.push_back(std::move(m));
+        if (mesh)
+            *mesh = (int)b->meshes.size() - 1;
+    });
+}
+
+int nuto_batch_build(nuto_batch* b)
+{
+    return GuardB(b, [&] {
+        Require(b->integrandSet, NUTO_ERR_STATE, "no integrand set");
+        ReleaseGroups(b);
+        const bool gdm = IsGdm(b);
+        // groups in the order hex20, tet10, hex8, tet4 (heaviest rows first)
+        const int order[4][2] = {{kHex8, kHex20}, {kTet4, kTet10}, {kHex8, kHex8}, {kTet4, kTet4}};
+        for (const auto& ctdt : order)
+        {
+            BatchGroup g;
+            g.ct = ctdt[0];
+            g.dt = ctdt[1];
+            int rk, rn;
+            BatchTypeOf(g.ct, g.dt, g.type, rk, rn);
+            for (size_t m = 0; m < b->meshes.size(); m++)
+                if (b->meshes[m].ct == g.ct && b->meshes[m].dt == g.dt)
+                    g.meshes.push_back((int)m);
+            if (g.meshes.empty())
+                continue;
+            Require(!gdm || (g.ct == kHex8 && g.dt == kHex8), NUTO_ERR_UNSUPPORTED,
+                    "gradient-damage batches hold hex8 meshes only");
+            const int gi = (int)b->groups.size();
+            std::vector<double> X;
+            std::vector<int> C, Rk;
+            for (int m : g.meshes)
+            {
+                BatchMesh& bm = b->meshes[m];
+                bm.group = gi;
+                bm.nodeOff = g.nNodes;
+                bm.cellOff = g.nCells;
+                bm.coordOff = g.nCoord;
+                X.insert(X.end(), bm.coords.begin(), bm.coords.end());
+                for (int v : bm.conn)
+                    C.push_back(v + bm.coordOff);
+                for (int v : bm.ranks)
+                    Rk.push_back(v + bm.nodeOff);
+                g.nNodes += bm.nDof;
+                g.nCells += bm.nCells;
+                g.nCoord += bm.nCoord;
+            }
+            int st = nuto_create(&g.ctx, b->device);
+            if (st != NUTO_OK)
+                throw Failure{st, "batch group context"};
+            b->groups.push_back(g);
+            BatchGroup& G = b->groups.back();
+            nuto_ctx* c = G.ctx;
+            CheckCtx(c, nuto_set_stream(c, b->stream));
+            CheckCtx(c, nuto_set_cells(c, G.ct, G.nCells, G.nCoord, X.data(), C.data(), nullptr));
+            std::vector<int> cd(Rk.size() * 3);
+            for (size_t i = 0; i < Rk.size(); i++)
+                for (int k = 0; k < 3; k++)
+                    cd[i * 3 + k] = Rk[i] * 3 + k;
+            CheckCtx(c, nuto_set_dof_type(c, 0, G.dt, 3, 3 * G.nNodes, cd.data()));
+            if (gdm)
+                CheckCtx(c, nuto_set_dof_type(c, 1, G.dt, 1, G.nNodes, Rk.data()));
+            CheckCtx(c, nuto_set_integrand(c, &b->params));
+            CheckCtx(c, nuto_set_rule(c, rk, rn));
+            long long nnz = 0;
+            int rows = 0;
+            CheckCtx(c, nuto_build_pattern(c, &nnz, &rows));
+            CheckCtx(c, Guard(c, [&] {
+                BuildTables(c);
+                if (gdm)
+                {
+                    PrepareGdm(c);
+                    Require(c->gdm.Usable(c->params.kind, c->coordType, c->f[0].type, c->nip), NUTO_ERR_UNSUPPORTED,
+                            "gradient-damage batch meshes need at most 8 cells and 32 neighbours per node");
+                }
+            }));
+            G.nnz = nnz;
+            G.nRows = rows;
+            G.nip = c->nip;
+        }
+        // offsets of every group in the batch arrays
+        b->uLen = b->eLen = b->RLen = b->KLen = b->hLen = b->tLen = 0;
+        for (BatchGroup& g : b->groups)
+        {
+            g.uOff = b->uLen;
+            g.eOff = b->eLen;
+            g.ROff = b->RLen;
+            g.KOff = b->KLen;
+            g.hOff = b->hLen;
+            g.tOff = b->tLen;
+            b->uLen += 3LL * g.nNodes;
+            b->eLen += gdm ? g.nNodes : 0;
+            b->RLen += g.nRows;
+            b->KLen += g.nnz;
+            b->hLen += (long long)g.nCells * g.nip;
+            b->tLen += 3LL * g.nCells;
+        }
+        // tiles of the fused kernel (groups are already heaviest first)
+        for (int w = 0; w < 3; w++)
+        {
+            std::vector<BatchTile> t;
+            for (size_t gi = 0; gi < b->groups.size() && !gdm; gi++)
+            {
+                const BatchGroup& g = b->groups[gi];
+                for (int first = 0; first < g.nNodes; first += kBatchBlock[w])
+                    t.push_back(BatchTile{(int)gi, first, std::min(first + kBatchBlock[w], g.nNodes), g.type});
+            }
+            b->nTiles[w] = (int)t.size();
+            Check(b->tiles[w].Upload(t.data(), t.size() * sizeof(BatchTile), b->stream), "batch tiles");
+        }
+        b->built = true;
+    });
+}
+
+int nuto_batch_sizes(nuto_batch* b, long long* sizes)
+{
+    return GuardB(b, [&] {
+        Require(b->built, NUTO_ERR_STATE, "batch not built (nuto_batch_build)");
+        Require(sizes != nullptr, NUTO_ERR_ARGUMENT, "sizes is NULL");
+        sizes[0] = b->uLen;
+        sizes[1] = b->eLen;
+        sizes[2] = b->RLen;
+        sizes[3] = b->KLen;
+        sizes[4] = b->hLen;
+        sizes[5] = b->tLen;
+    });
+}
+
+int nuto_batch_mesh_info(nuto_batch* b, int mesh, long long* info)
+{
+    return GuardB(b, [&] {
+        Require(b->built, NUTO_ERR_STATE, "batch not built (nuto_batch_build)");
+        Require(mesh >= 0 && mesh < (int)b->meshes.size() && info, NUTO_ERR_ARGUMENT, "bad mesh index / info");
+        const BatchMesh& m = b->meshes[mesh];
+        const BatchGroup& g = b->groups[m.group];
+        const int* rp = g.ctx->graph.adjPtr.data();  // neighbour counts give the row lengths
+        const bool gdm = IsGdm(b);
+        const int w = gdm ? 4 : 3;  // columns per neighbour
+        // nonzeros before the mesh's first node and within the mesh, per row group
+        long long before = 0, inside = 0;
+        for (int p = 0; p < m.nodeOff; p++)
+            before += (long long)(rp[p + 1] - rp[p]) * w;
+        for (int p = m.nodeOff; p < m.nodeOff + m.nDof; p++)
+            inside += (long long)(rp[p + 1] - rp[p]) * w;
+        const long long nnzU = 3 * inside, nnzE = gdm ? inside : 0;
+        info[0] = m.group;
+        info[1] = m.nDof;
+        info[2] = m.nCells;
+        info[3] = g.nip;
+        info[4] = g.uOff + 3LL * m.nodeOff;                     // u
+        info[5] = gdm ? g.eOff + m.nodeOff : -1;                // eeq
+        info[6] = g.ROff + 3LL * m.nodeOff;                     // R, displacement rows
+        info[7] = gdm ? g.ROff + 3LL * g.nNodes + m.nodeOff : -1;  // R, eeq rows
+        info[8] = (long long)m.nDof * (gdm ? 4 : 3);            // rows of the mesh
+        info[9] = g.KOff + 3 * before;                          // K, displacement rows
+        info[10] = nnzU;
+        info[11] = gdm ? g.KOff + 3LL * (g.nnz / 4) + before : -1;  // K, eeq rows (after all displacement rows)
+        info[12] = nnzE;
+        info[13] = g.hOff + m.cellOff;                          // history of (cell 0, ip 0)
+        info[14] = g.nCells;                                    // history leading dimension
+        info[15] = g.tOff + m.cellOff;                          // d_theta of cell 0 (E row)
+        info[16] = g.nCells;                                    // d_theta stride between E, f_t, G_f
+    });
+}
+
+int nuto_batch_copy_pattern(nuto_batch* b, int mesh, int* row_ptr, int* col)
+{
+    return GuardB(b, [&] {
+        Require(b->built, NUTO_ERR_STATE, "batch not built (nuto_batch_build)");
+        Require(mesh >= 0 && mesh < (int)b->meshes.size() && row_ptr && col, NUTO_ERR_ARGUMENT, "bad arguments");
+        const BatchMesh& m = b->meshes[mesh];
+        const BatchGroup& g = b->groups[m.group];
+        const NodeGraph& G = g.ctx->graph;
+        const bool gdm = IsGdm(b);
+        const int nU = 3 * m.nDof;
+        // mesh-local monolithic CSR: rows [u ; e], columns [u, e], ascending (the group graph's
+        // neighbours of a node are the mesh's nodes, offset by nodeOff)
+        long long k = 0;
+        int row = 0;
+        auto emitRow = [&](int p) {
+            row_ptr[row++] = (int)k;
+            for (int a = G.adjPtr[p]; a < G.adjPtr[p + 1]; a++)
+                for (int c = 0; c < 3; c++)
+                    col[k++] = 3 * (G.adj[a] - m.nodeOff) + c;
+            if (gdm)
+                for (int a = G.adjPtr[p]; a < G.adjPtr[p + 1]; a++)
+                    col[k++] = nU + (G.adj[a] - m.nodeOff);
+        };
+        for (int p = m.nodeOff; p < m.nodeOff + m.nDof; p++)
+            for (int c = 0; c < 3; c++)
+                emitRow(p);
+        if (gdm)
+            for (int p = m.nodeOff; p < m.nodeOff + m.nDof; p++)
+                emitRow(p);
+        row_ptr[row] = (int)k;
+    });
+}
+
+int nuto_batch_set_material_fields(nuto_batch* b, const double* E, const double* ft, const double* gf)
+{
+    return GuardB(b, [&] {
+        Require(b->built, NUTO_ERR_STATE, "batch not built (nuto_batch_build)");
+        Require((E && ft && gf) || (!E && !ft && !gf), NUTO_ERR_ARGUMENT, "E, ft and gf must all be given");
+        // host arrays over all cells, meshes in the order they were added
+        DropGraph(b);  // captured launches carry the old field buffers
+        std::vector<long long> base(b->meshes.size(), 0);
+        for (size_t m = 1; m < b->meshes.size(); m++)
+            base[m] = base[m - 1] + b->meshes[m - 1].nCells;
+        for (BatchGroup& g : b->groups)
+        {
+            if (!E)
+            {
+                CheckCtx(g.ctx, nuto_set_material_fields(g.ctx, nullptr, nullptr, nullptr));
+                continue;
+            }
+            std::vector<double> e(g.nCells), f(g.nCells), h(g.nCells);
+            for (int m : g.meshes)
+            {
+                const BatchMesh& bm = b->meshes[m];
+                for (int c = 0; c < bm.nCells; c++)
+                {
+                    e[bm.cellOff + c] = E[base[m] + c];
+                    f[bm.cellOff + c] = ft[base[m] + c];
+                    h[bm.cellOff + c] = gf[base[m] + c];
+                }
+            }
+            CheckCtx(g.ctx, nuto_set_material_fields(g.ctx, e.data(), f.data(), h.data()));
+        }
+        b->argPtrs[0] = nullptr;  // the group arguments carry the field pointers
+    });
+}
+
+int nuto_batch_assemble(nuto_batch* b, const double* u, const double* e, const double* kappa, double* R, double* K,
+                        int mode)
+{
+    return GuardB(b, [&] {
+        RequireBuilt(b, u, e, kappa);
+        Require(R && K, NUTO_ERR_ARGUMENT, "R / K is NULL");
+        Require(mode >= -1 && mode < kBatchModes, NUTO_ERR_ARGUMENT, "unknown launch mode");
+        if (mode == -1)
+        {
+            // the tuned (or default fused) launch, replayed from a graph
+            const int m = b->tuned >= 0 ? b->tuned : 3;
+            if (m < 3)
+                UploadArgs(b, u, e, kappa, R, K);  // outside the capture
+            const void* p[5] = {u, e, kappa, R, K};
+            GraphRun(b, m, p, 5, [&] { LaunchBatch(b, m, u, e, kappa, R, K); });
+        }
+        else
+            LaunchBatch(b, mode, u, e, kappa, R, K);
+    });
+}
+
+int nuto_batch_autotune(nuto_batch* b, const double* u, const double* e, const double* kappa, double* R, double* K,
+                        double* ms, int* best)
+{
+    return GuardB(b, [&] {
+        RequireBuilt(b, u, e, kappa);
+        Require(R && K, NUTO_ERR_ARGUMENT, "R / K is NULL");
+        hipEvent_t t0, t1;
+        Check(hipEventCreate(&t0), "event");
+        Check(hipEventCreate(&t1), "event");
+        double bestMs = 1e300;
+        int bestMode = 3;
+        for (int mode = 0; mode < kBatchModes; mode++)
+        {
+            double v = -1.0;
+            if (!(IsGdm(b) && mode < 3))
+            {
+                LaunchBatch(b, mode, u, e, kappa, R, K);  // warm-up
+                std::vector<float> rep;
+                for (int it = 0; it < 5; it++)
+                {
+                    Check(hipEventRecord(t0, b->stream), "event");
+                    LaunchBatch(b, mode, u, e, kappa, R, K);
+                    Check(hipEventRecord(t1, b->stream), "event");
+                    Check(hipEventSynchronize(t1), "event");
+                    float x = 0.0f;
+                    Check(hipEventElapsedTime(&x, t0, t1), "event");
+                    rep.push_back(x);
+                }
+                std::sort(rep.begin(), rep.end());
+                v = rep[rep.size() / 2];
+                if (mode != 4 && v < bestMs)  // the per-mesh launch is the reference, not a candidate
+                {
+                    bestMs = v;
+                    bestMode = mode;
+                }
+            }
+            if (ms)
+                ms[mode] = v;
+        }
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        b->tuned = bestMode;
+        DropGraph(b);
+        if (best)
+            *best = bestMode;
+    });
+}
+
+int nuto_batch_vjp(nuto_batch* b, const double* u, const double* e, const double* kappa, const double* lambda,
+                   double* d_state, double* d_theta)
+{
+    return GuardB(b, [&] {
+        RequireBuilt(b, u, e, kappa);
+        Require(IsGdm(b), NUTO_ERR_UNSUPPORTED, "the adjoint is provided for the gradient-damage integrand");
+        Require(lambda && d_state, NUTO_ERR_ARGUMENT, "lambda / d_state is NULL");
+        LaunchVjp(b, u, e, kappa, lambda, d_state, d_theta);
+    });
+}
+
+int nuto_batch_residual_adjoint(nuto_batch* b, const double* u, const double* e, const double* kappa,
+                                const double* r_obs, double* R, double* K, double* d_state, double* d_theta)
+{
+    return GuardB(b, [&] {
+        RequireBuilt(b, u, e, kappa);
+        Require(IsGdm(b), NUTO_ERR_UNSUPPORTED, "the adjoint is provided for the gradient-damage integrand");
+        Require(r_obs && R && K && d_state, NUTO_ERR_ARGUMENT, "NULL array");
+        if (b->lambda.bytes != (size_t)b->RLen * sizeof(double))
+        {
+            DropGraph(b);
+            Check(b->lambda.Alloc((size_t)b->RLen * sizeof(double)), "lambda");
+        }
+        double* lam = b->lambda.As<double>();
+        const void* p[9] = {u, e, kappa, r_obs, R, K, d_state, d_theta, lam};
+        GraphRun(b, 100, p, 9, [&] {
+            LaunchBatch(b, 3, u, e, kappa, R, K);
+            if (b->RLen > 0)
+                hipLaunchKernelGGL(kResidualMinus, dim3((unsigned)((b->RLen + 255) / 256)), dim3(256), 0, b->stream, R,
+                                   r_obs, lam, b->RLen);
+            LaunchVjp(b, u, e, kappa, lam, d_state, d_theta);
+        });
+    });
 }
 
 }  // extern "C"

@@ -1,0 +1,156 @@
+"""Batched assembly: every mesh of a batch against the oracle (pattern bit-equal, R and K within
+1e-12 of the contribution magnitudes), all launch modes bit-identical, graph replays identical to
+the eager launch, and the batched gradient-damage residual + adjoint against single meshes.
+"""
+import numpy as np
+import pytest
+import torch
+
+import nuto_amd as N
+import oracle as O
+from nuto_amd import synthetic as S
+from nuto_amd import workloads as W
+from pairs import check_values, to_dev
+
+pytestmark = pytest.mark.gpu
+
+RULE = {(N.TET4, N.TET4): (1, 1), (N.TET4, N.TET10): (1, 2), (N.HEX8, N.HEX8): (0, 2), (N.HEX8, N.HEX20): (0, 3)}
+
+
+def ld_oracle_params():
+    return O.Params(kind=2, dim=3, E=30000.0, nu=0.2, k=10.0, kappa0=1e-4, beta=100.0, alpha=0.99)
+
+
+def build(meshes, params):
+    b = N.Batch()
+    b.set_integrand(params)
+    for ct, dt, X, C, R, nn in meshes:
+        b.add_mesh(ct, dt, X, C, R, nn)
+    sizes = b.build()
+    return b, sizes
+
+
+def mesh_history(kap, info):
+    """History of one mesh in NuTo's layout (cell + ip * n_cells) from the batch array."""
+    c = np.arange(info["cells"])
+    return np.concatenate([kap[info["history"] + c + i * info["history_ld"]] for i in range(info["nip"])])
+
+
+def batch_inputs(sizes, seed_shift=0):
+    u = S.uniform(S.SEED_U + seed_shift, sizes["u"], -5e-3, 5e-3)
+    e = S.uniform(S.SEED_E + seed_shift, sizes["e"], 0.0, 5e-4) if sizes["e"] else None
+    kap = S.uniform(S.SEED_KAPPA + seed_shift, sizes["history"], 0.0, 3e-4)
+    return u, e, kap
+
+
+@pytest.fixture(scope="module")
+def mixed():
+    meshes = W.mixed_batch(7, 60.0, 400.0)
+    assert {(m[0], m[1]) for m in meshes} == set(W.TYPES)  # every element pair present
+    b, sizes = build(meshes, N.local_damage_params())
+    u, _, kap = batch_inputs(sizes)
+    return meshes, b, sizes, u, kap
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
+def test_every_mesh_matches_oracle(mixed, mode):
+    meshes, b, sizes, u, kap = mixed
+    R, K = b.assemble(to_dev(u), None, to_dev(kap), mode=mode)
+    R, K = R.cpu().numpy(), K.cpu().numpy()
+    for m, (ct, dt, X, C, ranks, nn) in enumerate(meshes):
+        info = b.mesh_info(m)
+        om = O.Mesh.from_arrays(ct, X, C)
+        Ld = om.add_dof(dt, 3)
+        lay = om.dof_layer(Ld)
+        assert np.array_equal(lay["numbers"][lay["conn"], 0] // 3, ranks), "numbering differs"
+        P = O.Problem(om, ld_oracle_params(), RULE[(ct, dt)][0], RULE[(ct, dt)][1], Ld, -1)
+        um = u[info["u"]:info["u"] + 3 * nn]
+        ref = P.assemble(um, None, mesh_history(kap, info))
+        rp, ci = b.pattern(m)
+        assert np.array_equal(rp, ref["row_ptr"]) and np.array_equal(ci, ref["col"]), f"pattern of mesh {m}"
+        check_values(R[info["R_u"]:info["R_u"] + info["rows"]], ref["R"], ref["R_bound"], f"R of mesh {m}")
+        check_values(K[info["K_u"]:info["K_u"] + info["nnz_u"]], ref["val"], ref["bound"], f"K of mesh {m}")
+
+
+def test_launch_modes_are_deterministic(mixed):
+    """Each mode reproduces its own bits; the per-type and per-mesh launches share one kernel per
+    element type and agree bit for bit (the fused kernel is compiled separately, FMA contraction
+    may differ in the last bit; it is held to the oracle bar above)."""
+    _, b, sizes, u, kap = mixed
+    per_mesh = [t.cpu().numpy() for t in b.assemble(to_dev(u), None, to_dev(kap), mode=4)]
+    for mode in (0, 1, 2, 3):
+        a = [t.cpu().numpy() for t in b.assemble(to_dev(u), None, to_dev(kap), mode=mode)]
+        c = [t.cpu().numpy() for t in b.assemble(to_dev(u), None, to_dev(kap), mode=mode)]
+        assert np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1]), f"mode {mode}"
+        if mode == 3:
+            assert np.array_equal(a[0], per_mesh[0]) and np.array_equal(a[1], per_mesh[1])
+
+
+def test_graph_replays_match_eager(mixed):
+    _, b, sizes, u, kap = mixed
+    du, dk = to_dev(u), to_dev(kap)
+    R = torch.empty(sizes["R"], dtype=torch.float64, device="cuda")
+    K = torch.empty(sizes["K"], dtype=torch.float64, device="cuda")
+    eager = [t.cpu().numpy() for t in b.assemble(du, None, dk, mode=3)]  # the untuned graph replays mode 3
+    for rep in range(4):
+        R.fill_(float("nan"))
+        K.fill_(float("nan"))
+        du.mul_(1.0)  # eager work on torch's stream between replays
+        b.assemble(du, None, dk, R, K, mode=-1)
+        torch.cuda.synchronize()
+        assert np.array_equal(R.cpu().numpy(), eager[0]) and np.array_equal(K.cpu().numpy(), eager[1]), rep
+
+
+def test_autotune_picks_a_batched_mode(mixed):
+    _, b, sizes, u, kap = mixed
+    ms, best = b.autotune(to_dev(u), None, to_dev(kap))
+    assert best in (0, 1, 2, 3) and all(v > 0 for v in ms)
+    for rep in range(3):
+        R, K = b.assemble(to_dev(u), None, to_dev(kap), mode=-1)
+        ref = b.assemble(to_dev(u), None, to_dev(kap), mode=best)
+        assert torch.equal(R, ref[0]) and torch.equal(K, ref[1])
+
+
+def test_gdm_batch_residual_adjoint_matches_single_meshes():
+    dims = [(2, 2, 2), (3, 2, 2), (2, 3, 4)]
+    meshes = []
+    for n in dims:
+        X, C = N.structured_mesh(N.HEX8, *n, tuple(float(v) for v in n))
+        ranks, nn = N.dof_layer(N.HEX8, C, N.HEX8)
+        meshes.append((N.HEX8, N.HEX8, X, C, ranks, nn))
+    b, sizes = build(meshes, N.gradient_damage_params())
+    n_cells = sum(m[3].shape[0] for m in meshes)
+    E, ft, gf = S.material_fields(n_cells)
+    b.set_material_fields(E, ft, gf)
+    u, e, kap = batch_inputs(sizes)
+    r_obs = S.uniform(0x5151, sizes["R"], -1e-2, 1e-2)
+    R, K, ds, dt = b.residual_adjoint(to_dev(u), to_dev(e), to_dev(kap), to_dev(r_obs))
+    R2, K2, ds2, dt2 = b.residual_adjoint(to_dev(u), to_dev(e), to_dev(kap), to_dev(r_obs))  # replay
+    assert torch.equal(R, R2) and torch.equal(ds, ds2) and torch.equal(dt, dt2)
+    R, K, ds, dt = (t.cpu().numpy() for t in (R, K, ds, dt))
+    c0 = 0
+    for m, (ct, dtp, X, C, ranks, nn) in enumerate(meshes):
+        info = b.mesh_info(m)
+        nc = C.shape[0]
+        ctx = N.Context()
+        ctx.set_cells(ct, X, C)
+        ctx.set_dof_type(0, dtp, 3, 3 * nn, N.cell_dofs(ranks, 3))
+        ctx.set_dof_type(1, dtp, 1, nn, N.cell_dofs(ranks, 1))
+        ctx.set_integrand(N.gradient_damage_params())
+        ctx.set_rule(0, 2)
+        ctx.build_pattern()
+        ctx.set_material_fields(E[c0:c0 + nc], ft[c0:c0 + nc], gf[c0:c0 + nc])
+        um = to_dev(u[info["u"]:info["u"] + 3 * nn])
+        em = to_dev(e[info["e"]:info["e"] + nn])
+        km = to_dev(mesh_history(kap, info))
+        Rm, Km = ctx.build_vector_matrix(um, em, km)
+        rows = np.r_[info["R_u"]:info["R_u"] + 3 * nn, info["R_e"]:info["R_e"] + nn]
+        assert np.array_equal(R[rows], Rm.cpu().numpy())
+        kk = np.r_[info["K_u"]:info["K_u"] + info["nnz_u"], info["K_e"]:info["K_e"] + info["nnz_e"]]
+        assert np.array_equal(K[kk], Km.cpu().numpy())
+        lam = to_dev(R[rows] - r_obs[rows])
+        dsm, dtm = ctx.vjp(um, em, km, lam)
+        assert np.array_equal(ds[rows], dsm.cpu().numpy())
+        th = np.concatenate([dt[info["theta"] + k * info["theta_stride"] + np.arange(nc)] for k in range(3)])
+        assert np.array_equal(th, dtm.cpu().numpy().ravel())
+        c0 += nc
