@@ -108,6 +108,99 @@ def adjoint_ratio(n, steps, warmup):
             "adjoint_outputs": "K^T lambda (u, eeq) and dR/dtheta^T lambda (3 per cell)"}
 
 
+def batched(n_meshes, steps, warmup):
+    """Mixed-type batch: meshes with log-uniform element counts in [1e3, 5e4], tet4 / tet10 / hex8 /
+    hex20 drawn uniformly, jittered grids (10 % of the spacing), local isotropic damage. Compares the
+    tuned batched launch (graph replay) with one launch per mesh on the same data."""
+    import numpy as np
+    import torch
+
+    import nuto_amd as N
+    from nuto_amd import synthetic as S
+    from nuto_amd import workloads as W
+
+    t0 = time.time()
+    meshes = W.mixed_batch(n_meshes)
+    b = N.Batch()
+    b.set_integrand(N.local_damage_params())
+    for ct, dt, X, C, ranks, nn in meshes:
+        b.add_mesh(ct, dt, X, C, ranks, nn)
+    sizes = b.build()
+    cells = sum(m[3].shape[0] for m in meshes)
+    u = torch.from_numpy(S.uniform(S.SEED_U, sizes["u"], -5e-3, 5e-3)).cuda()
+    kap = torch.from_numpy(S.uniform(S.SEED_KAPPA, sizes["history"], 0.0, 3e-4)).cuda()
+    R = torch.empty(sizes["R"], dtype=torch.float64, device="cuda")
+    K = torch.empty(sizes["K"], dtype=torch.float64, device="cuda")
+    setup = time.time() - t0
+    ms_modes, best = b.autotune(u, None, kap, R, K)
+
+    def timed(mode):
+        for _ in range(warmup):
+            b.assemble(u, None, kap, R, K, mode=mode)
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        for a, z in evs:
+            a.record(b.stream)
+            b.assemble(u, None, kap, R, K, mode=mode)
+            z.record(b.stream)
+        torch.cuda.synchronize()
+        return float(np.median([a.elapsed_time(z) for a, z in evs]))
+
+    graph_ms = timed(-1)
+    per_mesh_ms = timed(4)
+    counts = {}
+    for ct, dt, *_ in meshes:
+        k = {(N.TET4, N.TET4): "tet4", (N.TET4, N.TET10): "tet10", (N.HEX8, N.HEX8): "hex8",
+             (N.HEX8, N.HEX20): "hex20"}[(ct, dt)]
+        counts[k] = counts.get(k, 0) + 1
+    return {"workload": f"{n_meshes} meshes, cells log-uniform in [1e3, 5e4], types {counts}, jitter 10 %, "
+                        f"local isotropic damage R+K", "cells": cells, "nnz": sizes["K"],
+            "batched_ms": graph_ms, "per_mesh_launch_ms": per_mesh_ms, "speedup": per_mesh_ms / graph_ms,
+            "cells_per_s": cells / (graph_ms * 1e-3), "tuned_mode": best,
+            "autotune_ms": {"fused64": ms_modes[0], "fused128": ms_modes[1], "fused256": ms_modes[2],
+                            "per_type": ms_modes[3], "per_mesh": ms_modes[4]}, "setup_s": setup}
+
+
+def batched_adjoint(n_meshes, steps, warmup):
+    """Gradient-damage batch: hex8 meshes with log-uniform element counts in [1e4, 2e5], per-cell
+    lognormal fields; one graph replay computes R, K, lambda = R - r_obs, K^T lambda and dL/dtheta."""
+    import numpy as np
+    import torch
+
+    import nuto_amd as N
+    from nuto_amd import synthetic as S
+    from nuto_amd import workloads as W
+
+    t0 = time.time()
+    meshes = W.gdm_batch(n_meshes)
+    b = N.Batch()
+    b.set_integrand(N.gradient_damage_params())
+    for ct, dt, X, C, ranks, nn in meshes:
+        b.add_mesh(ct, dt, X, C, ranks, nn)
+    sizes = b.build()
+    cells = sum(m[3].shape[0] for m in meshes)
+    b.set_material_fields(*S.material_fields(cells))
+    u = torch.from_numpy(S.uniform(S.SEED_U, sizes["u"], -5e-3, 5e-3)).cuda()
+    e = torch.from_numpy(S.uniform(S.SEED_E, sizes["e"], 0.0, 5e-4)).cuda()
+    kap = torch.from_numpy(S.uniform(S.SEED_KAPPA, sizes["history"], 0.0, 3e-4)).cuda()
+    robs = torch.from_numpy(S.uniform(0x5151, sizes["R"], -1e-2, 1e-2)).cuda()
+    out = [torch.empty(sizes[k], dtype=torch.float64, device="cuda") for k in ("R", "K", "R", "theta")]
+    setup = time.time() - t0
+    for _ in range(warmup):
+        b.residual_adjoint(u, e, kap, robs, *out)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    for a, z in evs:
+        a.record(b.stream)
+        b.residual_adjoint(u, e, kap, robs, *out)
+        z.record(b.stream)
+    torch.cuda.synchronize()
+    ms = float(np.median([a.elapsed_time(z) for a, z in evs]))
+    return {"workload": f"{n_meshes} hex8 gradient-damage meshes, cells log-uniform in [1e4, 2e5], per-cell "
+                        f"lognormal (E, f_t, G_f); R+K, lambda = R - r_obs, K^T lambda, dL/dtheta in one graph",
+            "cells": cells, "nnz": sizes["K"], "ms": ms, "cells_per_s": cells / (ms * 1e-3), "setup_s": setup}
+
+
 def cpu_baseline(n, threads, budget_s=30.0):
     """Times the oracle (NuTo's algorithm restated: per-thread triplet lists, critical splice,
     setFromTriplets) on an n^3 sample of the same workload."""
@@ -148,6 +241,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--adjoint-n", type=int, default=64, help="cells per direction of the adjoint workload")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--batch-meshes", type=int, default=256, help="meshes of the mixed-type batch (0: skip)")
+    ap.add_argument("--batch-gdm-meshes", type=int, default=64,
+                    help="meshes of the gradient-damage residual + adjoint batch (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -238,6 +334,13 @@ def main():
         del R, K
         torch.cuda.empty_cache()
         rec["adjoint"] = adjoint_ratio(args.adjoint_n, args.steps, args.warmup)
+        torch.cuda.empty_cache()
+    if args.batch_meshes:
+        rec["batched"] = batched(args.batch_meshes, min(args.steps, 10), args.warmup)
+        torch.cuda.empty_cache()
+    if args.batch_gdm_meshes:
+        rec["batched_adjoint"] = batched_adjoint(args.batch_gdm_meshes, min(args.steps, 10), args.warmup)
+        torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         try:
