@@ -1,0 +1,196 @@
+// C++ host mirror of SimpleAssembler over the nuto_amd C ABI (see nuto_assembler.h).
+#include "nuto_assembler.h"
+
+#include <atomic>
+#include <memory>
+
+namespace nuto_amd
+{
+namespace host
+{
+
+namespace
+{
+std::atomic<int> gNextDofTypeId{0};  // UniqueId<DofType> (nuto/base/UniqueId.h:9-29): creation order
+
+// device buffer owned through the C ABI allocator
+struct DeviceArray
+{
+    nuto_ctx* ctx = nullptr;
+    void* p = nullptr;
+    DeviceArray(nuto_ctx* c, size_t bytes) : ctx(c)
+    {
+        if (nuto_device_alloc(ctx, &p, bytes) != NUTO_OK)
+            throw Exception("nuto_amd::host", std::string("device allocation: ") + nuto_last_error(ctx));
+    }
+    ~DeviceArray()
+    {
+        if (p)
+            nuto_device_free(ctx, p);
+    }
+    DeviceArray(const DeviceArray&) = delete;
+    DeviceArray& operator=(const DeviceArray&) = delete;
+    double* d() const { return static_cast<double*>(p); }
+};
+}  // namespace
+
+DofType::DofType(std::string name, int num) : mName(std::move(name)), mNum(num), mId(gNextDofTypeId++)
+{
+}
+
+int DofInfo::Total(const DofType& d) const
+{
+    auto a = numIndependent.find(d.Id());
+    auto b = numDependent.find(d.Id());
+    return (a == numIndependent.end() ? 0 : a->second) + (b == numDependent.end() ? 0 : b->second);
+}
+
+int CellGroup::NumCells() const
+{
+    const int nn = coordType == NUTO_QUAD4 ? 4 : coordType == NUTO_HEX8 ? 8 : 4;
+    return conn.empty() ? 0 : (int)(conn.size() / nn);
+}
+
+SimpleAssembler::SimpleAssembler(DofInfo dofInfo, int device) : mDofInfo(std::move(dofInfo))
+{
+    const int s = nuto_create(&mCtx, device);
+    if (s != NUTO_OK)
+        throw Exception("SimpleAssembler", "no usable device (nuto_create status " + std::to_string(s) + ")");
+}
+
+SimpleAssembler::~SimpleAssembler()
+{
+    if (mCtx)
+        nuto_destroy(mCtx);
+}
+
+void SimpleAssembler::Check(int status, const char* caller) const
+{
+    if (status != NUTO_OK)
+        throw Exception(caller, nuto_last_error(mCtx));
+}
+
+void SimpleAssembler::Setup(const CellGroup& cells, const std::vector<DofType>& dofs, const nuto_params& p, int dim)
+{
+    // SimpleAssembler::ThrowOnZeroDofNumbering (SimpleAssembler.cpp:16-23)
+    for (const DofType& d : dofs)
+        if (mDofInfo.Total(d) == 0)
+            throw Exception("SimpleAssembler", "You did not provide a dof numbering for DofType " + d.GetName() +
+                                                   ". Please call SetDofInfo(...) first.");
+    const int nCells = cells.NumCells();
+    Check(nuto_set_cells(mCtx, cells.coordType, nCells, (int)(cells.coords.size() / dim), cells.coords.data(),
+                         cells.conn.data(), cells.cellIds.empty() ? nullptr : cells.cellIds.data()),
+          "SimpleAssembler::SetCells");
+    for (size_t s = 0; s < dofs.size(); s++)
+    {
+        auto it = cells.dofNumbering.find(dofs[s].Id());
+        if (it == cells.dofNumbering.end())
+            throw Exception("SimpleAssembler", "the cell group has no numbering for DofType " + dofs[s].GetName());
+        Check(nuto_set_dof_type(mCtx, (int)s, it->second.first, dofs[s].GetNum(), mDofInfo.Total(dofs[s]),
+                                it->second.second.data()),
+              "SimpleAssembler::SetDofType");
+    }
+    Check(nuto_set_integrand(mCtx, &p), "SimpleAssembler::SetIntegrand");
+    const bool tet = cells.coordType == NUTO_TET4;
+    Check(nuto_set_rule(mCtx, tet ? 1 : 0, tet ? 1 : 2), "SimpleAssembler::SetRule");
+    Check(nuto_build_pattern(mCtx, &mNnz, &mRows), "SimpleAssembler::BuildPattern");
+}
+
+void SimpleAssembler::Run(const std::vector<DofType>& dofs, const DofVector<double>& state,
+                          const std::vector<double>* kappas, bool wantR, bool wantK, DofVector<double>* R,
+                          DofMatrixSparse<double>* K)
+{
+    std::vector<int> n;
+    for (const DofType& d : dofs)
+        n.push_back(mDofInfo.Total(d));
+    DeviceArray u(mCtx, n[0] * sizeof(double));
+    Check(nuto_copy_to_device(mCtx, u.p, state[dofs[0]].data(), n[0] * sizeof(double)), "BuildVector");
+    std::unique_ptr<DeviceArray> e, k;
+    if (dofs.size() > 1)
+    {
+        e.reset(new DeviceArray(mCtx, n[1] * sizeof(double)));
+        Check(nuto_copy_to_device(mCtx, e->p, state[dofs[1]].data(), n[1] * sizeof(double)), "BuildVector");
+    }
+    if (kappas)
+    {
+        k.reset(new DeviceArray(mCtx, kappas->size() * sizeof(double)));
+        Check(nuto_copy_to_device(mCtx, k->p, kappas->data(), kappas->size() * sizeof(double)), "BuildVector");
+    }
+    DeviceArray r(mCtx, (wantR ? mRows : 1) * sizeof(double));
+    DeviceArray kv(mCtx, (wantK ? mNnz : 1) * sizeof(double));
+    const double* ed = e ? e->d() : nullptr;
+    const double* kd = k ? k->d() : nullptr;
+    if (wantR && wantK)
+        Check(nuto_build_vector_matrix(mCtx, u.d(), ed, kd, r.d(), kv.d()), "SimpleAssembler::BuildVectorMatrix");
+    else if (wantR)
+        Check(nuto_build_vector(mCtx, u.d(), ed, kd, r.d()), "SimpleAssembler::BuildVector");
+    else
+        Check(nuto_build_matrix(mCtx, u.d(), ed, kd, kv.d()), "SimpleAssembler::BuildMatrix");
+    Check(nuto_synchronize(mCtx), "SimpleAssembler");
+    if (wantR)
+    {
+        std::vector<double> all(mRows);
+        Check(nuto_copy_to_host(mCtx, all.data(), r.p, mRows * sizeof(double)), "BuildVector");
+        size_t off = 0;
+        for (size_t s = 0; s < dofs.size(); s++)
+        {
+            (*R)[dofs[s]].assign(all.begin() + off, all.begin() + off + n[s]);
+            off += n[s];
+        }
+    }
+    if (wantK)
+    {
+        K->rowPtr.resize(mRows + 1);
+        K->col.resize(mNnz);
+        K->val.resize(mNnz);
+        Check(nuto_copy_pattern(mCtx, K->rowPtr.data(), K->col.data()), "BuildMatrix");
+        Check(nuto_copy_to_host(mCtx, K->val.data(), kv.p, mNnz * sizeof(double)), "BuildMatrix");
+    }
+}
+
+DofVector<double> SimpleAssembler::BuildVector(const CellGroup& cells, const GradientDamage& f,
+                                               const DofVector<double>& state)
+{
+    const nuto_params p{NUTO_GRADIENT_DAMAGE, 3, f.E, f.nu, 0, f.k, f.kappa0, f.beta, f.alpha, f.c,
+                        f.damageApplication};
+    Setup(cells, {f.disp, f.eeq}, p, 3);
+    DofVector<double> R;
+    Run({f.disp, f.eeq}, state, &f.kappas, true, false, &R, nullptr);
+    return R;
+}
+
+DofMatrixSparse<double> SimpleAssembler::BuildMatrix(const CellGroup& cells, const GradientDamage& f,
+                                                     const DofVector<double>& state)
+{
+    const nuto_params p{NUTO_GRADIENT_DAMAGE, 3, f.E, f.nu, 0, f.k, f.kappa0, f.beta, f.alpha, f.c,
+                        f.damageApplication};
+    Setup(cells, {f.disp, f.eeq}, p, 3);
+    DofMatrixSparse<double> K;
+    Run({f.disp, f.eeq}, state, &f.kappas, false, true, nullptr, &K);
+    return K;
+}
+
+DofVector<double> SimpleAssembler::BuildVector(const CellGroup& cells, const MomentumBalance& f,
+                                               const DofVector<double>& state)
+{
+    const int dim = cells.coordType == NUTO_QUAD4 ? 2 : 3;
+    const nuto_params p{NUTO_MOMENTUM_LINEAR_ELASTIC, dim, f.E, f.nu, f.planeState, 0, 0, 0, 0, 0, 0};
+    Setup(cells, {f.disp}, p, dim);
+    DofVector<double> R;
+    Run({f.disp}, state, nullptr, true, false, &R, nullptr);
+    return R;
+}
+
+DofMatrixSparse<double> SimpleAssembler::BuildMatrix(const CellGroup& cells, const MomentumBalance& f,
+                                                     const DofVector<double>& state)
+{
+    const int dim = cells.coordType == NUTO_QUAD4 ? 2 : 3;
+    const nuto_params p{NUTO_MOMENTUM_LINEAR_ELASTIC, dim, f.E, f.nu, f.planeState, 0, 0, 0, 0, 0, 0};
+    Setup(cells, {f.disp}, p, dim);
+    DofMatrixSparse<double> K;
+    Run({f.disp}, state, nullptr, false, true, nullptr, &K);
+    return K;
+}
+
+}  // namespace host
+}  // namespace nuto_amd

@@ -1,0 +1,158 @@
+// C++ host mirror of NuTo's SimpleAssembler interface over the nuto_amd C ABI (no Eigen, no Boost).
+//
+// Types mirror the reference's (names, argument meaning, error behaviour):
+//   Exception          NuTo::Exception (nuto/base/Exception.h:9-26), thrown for every failing call
+//   DofType            NuTo::DofType (nuto/mechanics/dofs/DofType.h:8-50): name, components, unique id
+//   DofVector<T>       NuTo::DofVector (DofVector.h:11-149): one value vector per DofType
+//   DofMatrixSparse<T> NuTo::DofMatrixSparse (DofMatrixSparse.h:9-10) exported monolithically as
+//                      ToEigen(K, {d0, d1}) does (DofMatrixSparseConvertEigen.h:40-67): CSR here
+//   DofInfo            NuTo::DofInfo (DofInfo.h:7-17): numIndependent / numDependent per DofType
+//   SimpleAssembler    BuildVector / BuildMatrix (SimpleAssembler.h:14-21, SimpleAssembler.cpp:38-156)
+// NuTo's VectorFunction / MatrixFunction are opaque std::functions; here the integrand is a typed
+// descriptor (GradientDamage, MomentumBalance), and the cell group is the mesh itself (CellGroup).
+// The node values that NuTo reads from its nodes (NodalValueMerger) are passed as a DofVector.
+#pragma once
+
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/nuto_amd.h"
+
+namespace nuto_amd
+{
+namespace host
+{
+
+class Exception : public std::runtime_error
+{
+public:
+    Exception(const std::string& caller, const std::string& message) : std::runtime_error(caller + ":\n" + message)
+    {
+    }
+};
+
+class DofType
+{
+public:
+    DofType(std::string name, int num);
+    const std::string& GetName() const { return mName; }
+    int GetNum() const { return mNum; }
+    int Id() const { return mId; }
+
+private:
+    std::string mName;
+    int mNum;
+    int mId;
+};
+
+template <typename T>
+class DofVector
+{
+public:
+    std::vector<T>& operator[](const DofType& d) { return mValues[d.Id()]; }
+    const std::vector<T>& operator[](const DofType& d) const;
+    bool Has(const DofType& d) const { return mValues.count(d.Id()) != 0; }
+
+private:
+    std::map<int, std::vector<T>> mValues;
+};
+
+template <typename T>
+const std::vector<T>& DofVector<T>::operator[](const DofType& d) const
+{
+    auto it = mValues.find(d.Id());
+    if (it == mValues.end())
+        throw std::out_of_range("DofVector: no values for dof type " + d.GetName());  // std::map::at in NuTo
+    return it->second;
+}
+
+// Monolithic CSR of the blocks (d0, d0) (d0, d1) (d1, d0) (d1, d1): rows [d0 ; d1], columns [d0, d1].
+template <typename T>
+struct DofMatrixSparse
+{
+    std::vector<int> rowPtr, col;
+    std::vector<T> val;
+    int Rows() const { return (int)rowPtr.size() - 1; }
+    long long NonZeros() const { return (long long)val.size(); }
+    T Coeff(int row, int column) const;  // 0 outside the pattern
+};
+
+template <typename T>
+T DofMatrixSparse<T>::Coeff(int row, int column) const
+{
+    for (int k = rowPtr[row]; k < rowPtr[row + 1]; k++)
+        if (col[k] == column)
+            return val[k];
+    return T(0);
+}
+
+struct DofInfo
+{
+    std::map<int, int> numIndependent, numDependent;
+    int Total(const DofType& d) const;
+};
+
+// The cell group: coordinate elements and, per DofType, the cell dof numbering
+// (Cell::DofNumbering, node-major, component-minor, Cell.h:60-63).
+struct CellGroup
+{
+    int coordType = NUTO_HEX8;
+    std::vector<double> coords;  // nodes x dim
+    std::vector<int> conn;       // cells x nodes(coordType)
+    std::vector<int> cellIds;    // Cell::Id per cell (empty: 0..n-1)
+    std::map<int, std::pair<int, std::vector<int>>> dofNumbering;  // DofType id -> (interpolation, numbering)
+    int NumCells() const;
+};
+
+// GradientDamage<3> with LinearElasticDamage, ModifiedMisesStrainNorm and DamageLawExponential
+// (GradientDamage.h:36, LinearElasticDamage.h:32, ModifiedMisesStrainNorm.h:38, DamageLawExponential.h:26)
+struct GradientDamage
+{
+    DofType disp, eeq;
+    double c, E, nu, k, kappa0, beta, alpha;
+    int damageApplication = 0;   // 0 FULL, 1 UNILATERAL
+    std::vector<double> kappas;  // history, cells x IPs column-major (mKappas, GradientDamage.h:137)
+};
+
+// MomentumBalance<D> with LinearElastic<D> (MomentumBalance.h:16, LinearElastic.h:16)
+struct MomentumBalance
+{
+    DofType disp;
+    double E, nu;
+    int planeState = 0;  // 2D: 0 PLANE_STRESS, 1 PLANE_STRAIN
+};
+
+class SimpleAssembler
+{
+public:
+    explicit SimpleAssembler(DofInfo dofInfo = DofInfo(), int device = 0);
+    ~SimpleAssembler();
+    SimpleAssembler(const SimpleAssembler&) = delete;
+    SimpleAssembler& operator=(const SimpleAssembler&) = delete;
+
+    void SetDofInfo(DofInfo dofInfo) { mDofInfo = std::move(dofInfo); }
+
+    // SimpleAssembler::BuildVector / BuildMatrix for the given integrand and node values
+    DofVector<double> BuildVector(const CellGroup& cells, const GradientDamage& f, const DofVector<double>& state);
+    DofMatrixSparse<double> BuildMatrix(const CellGroup& cells, const GradientDamage& f,
+                                        const DofVector<double>& state);
+    DofVector<double> BuildVector(const CellGroup& cells, const MomentumBalance& f, const DofVector<double>& state);
+    DofMatrixSparse<double> BuildMatrix(const CellGroup& cells, const MomentumBalance& f,
+                                        const DofVector<double>& state);
+
+private:
+    void Check(int status, const char* caller) const;
+    void Setup(const CellGroup& cells, const std::vector<DofType>& dofs, const nuto_params& p, int dim);
+    void Run(const std::vector<DofType>& dofs, const DofVector<double>& state, const std::vector<double>* kappas,
+             bool wantR, bool wantK, DofVector<double>* R, DofMatrixSparse<double>* K);
+
+    DofInfo mDofInfo;
+    nuto_ctx* mCtx = nullptr;
+    long long mNnz = 0;
+    int mRows = 0;
+};
+
+}  // namespace host
+}  // namespace nuto_amd

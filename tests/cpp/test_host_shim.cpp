@@ -1,0 +1,205 @@
+// C++ test of nuto_amd::host::SimpleAssembler (the SimpleAssembler mirror over the C ABI) against
+// the oracle's C API: pattern bit-equal, R and K within 1e-12 of the contribution magnitudes, and
+// the reference's error behaviour (missing dof numbering, missing DofVector entry). Needs a GPU.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../nuto_amd/host/nuto_assembler.h"
+
+extern "C" {
+void* orc_mesh_from_arrays(int coordType, int nNodes, const double* X, int nCells, const int* conn);
+void orc_mesh_free(void* m);
+int orc_mesh_add_dof(void* mp, int type, int ncomp);
+void* orc_problem(void* mp, const nuto_params* p, int ruleKind, int ruleN, int layer0, int layer1);
+void orc_problem_free(void* P);
+long long orc_problem_assemble(void* Pp, const double* u, const double* e, const double* kappa, int wantR, int wantK,
+                               int nthreads);
+void orc_problem_R(void* Pp, double* R, double* Rb);
+void orc_problem_K(void* Pp, int* rp, int* ci, double* v, double* b);
+}
+
+using namespace nuto_amd::host;
+
+static int gFailures = 0;
+#define EXPECT(cond, what)                                                                                         \
+    do                                                                                                             \
+    {                                                                                                              \
+        if (!(cond))                                                                                               \
+        {                                                                                                          \
+            std::printf("FAIL %s (%s:%d)\n", what, __FILE__, __LINE__);                                            \
+            gFailures++;                                                                                           \
+        }                                                                                                          \
+    } while (0)
+
+static double Uniform(unsigned long long& s, double lo, double hi)  // SplitMix64, top 53 bits
+{
+    unsigned long long z = (s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    return lo + (hi - lo) * ((z >> 11) * (1.0 / 9007199254740992.0));
+}
+
+struct Mesh
+{
+    int coordType, nNodes, nCells, nDofNodes;
+    std::vector<double> X;
+    std::vector<int> C, ranks;
+};
+
+static Mesh MakeMesh(int coordType, int nx, int ny, int nz)
+{
+    Mesh m;
+    m.coordType = coordType;
+    nuto_mesh_structured(coordType, nx, ny, nz, nx, ny, nz, 0.0, 0, &m.nNodes, &m.nCells, nullptr, nullptr);
+    const int dim = coordType == NUTO_QUAD4 ? 2 : 3, nn = coordType == NUTO_QUAD4 ? 4 : 8;
+    m.X.resize((size_t)m.nNodes * dim);
+    m.C.resize((size_t)m.nCells * nn);
+    nuto_mesh_structured(coordType, nx, ny, nz, nx, ny, nz, 0.0, 0, &m.nNodes, &m.nCells, m.X.data(), m.C.data());
+    m.ranks.resize(m.C.size());
+    nuto_dof_layer(coordType, m.nCells, m.C.data(), coordType, m.ranks.data(), &m.nDofNodes);
+    return m;
+}
+
+static std::vector<int> CellDofs(const std::vector<int>& ranks, int ncomp)
+{
+    std::vector<int> d;
+    for (int r : ranks)
+        for (int c = 0; c < ncomp; c++)
+            d.push_back(r * ncomp + c);
+    return d;
+}
+
+static void Compare(const char* what, void* P, int nRows, const std::vector<double>& R,
+                    const DofMatrixSparse<double>& K)
+{
+    std::vector<double> rR(nRows), rB(nRows);
+    orc_problem_R(P, rR.data(), rB.data());
+    for (int i = 0; i < nRows; i++)
+        EXPECT(std::fabs(R[i] - rR[i]) <= 1e-12 * rB[i], (std::string(what) + " R").c_str());
+    std::vector<int> rp(nRows + 1), ci(K.col.size());
+    std::vector<double> v(K.col.size()), b(K.col.size());
+    orc_problem_K(P, rp.data(), ci.data(), v.data(), b.data());
+    EXPECT(rp == K.rowPtr, (std::string(what) + " row_ptr").c_str());
+    EXPECT(ci == K.col, (std::string(what) + " col").c_str());
+    for (size_t k = 0; k < v.size(); k++)
+        if (!(std::fabs(K.val[k] - v[k]) <= 1e-12 * b[k]))
+        {
+            EXPECT(false, (std::string(what) + " K").c_str());
+            break;
+        }
+}
+
+static void GradientDamageCase()
+{
+    Mesh m = MakeMesh(NUTO_HEX8, 3, 3, 2);
+    DofType disp("Displacements", 3), eeq("NonlocalEqStrain", 1);
+    DofInfo info;
+    info.numIndependent[disp.Id()] = 3 * m.nDofNodes;
+    info.numIndependent[eeq.Id()] = m.nDofNodes;
+    CellGroup cells;
+    cells.coordType = NUTO_HEX8;
+    cells.coords = m.X;
+    cells.conn = m.C;
+    cells.dofNumbering[disp.Id()] = {NUTO_HEX8, CellDofs(m.ranks, 3)};
+    cells.dofNumbering[eeq.Id()] = {NUTO_HEX8, CellDofs(m.ranks, 1)};
+    GradientDamage f{disp, eeq, 4.0, 30000.0, 0.2, 10.0, 1e-4, 100.0, 0.99, 0, {}};
+    unsigned long long seed = 12345;
+    DofVector<double> state;
+    for (int i = 0; i < 3 * m.nDofNodes; i++)
+        state[disp].push_back(Uniform(seed, -5e-3, 5e-3));
+    for (int i = 0; i < m.nDofNodes; i++)
+        state[eeq].push_back(Uniform(seed, 0.0, 5e-4));
+    for (int i = 0; i < m.nCells * 8; i++)
+        f.kappas.push_back(Uniform(seed, 0.0, 3e-4));
+
+    // NuTo throws when the dof numbering is missing (SimpleAssembler.cpp:16-23)
+    {
+        SimpleAssembler none;
+        bool threw = false;
+        try
+        {
+            none.BuildVector(cells, f, state);
+        }
+        catch (const Exception& e)
+        {
+            threw = std::string(e.what()).find("dof numbering") != std::string::npos;
+        }
+        EXPECT(threw, "missing dof info throws Exception");
+    }
+    SimpleAssembler asmbl(info);
+    DofVector<double> R = asmbl.BuildVector(cells, f, state);
+    DofMatrixSparse<double> K = asmbl.BuildMatrix(cells, f, state);
+    std::vector<double> Rall = R[disp];
+    Rall.insert(Rall.end(), R[eeq].begin(), R[eeq].end());
+
+    void* om = orc_mesh_from_arrays(NUTO_HEX8, m.nNodes, m.X.data(), m.nCells, m.C.data());
+    const int Ld = orc_mesh_add_dof(om, NUTO_HEX8, 3), Le = orc_mesh_add_dof(om, NUTO_HEX8, 1);
+    nuto_params p{NUTO_GRADIENT_DAMAGE, 3, 30000.0, 0.2, 0, 10.0, 1e-4, 100.0, 0.99, 4.0, 0};
+    void* P = orc_problem(om, &p, 0, 2, Ld, Le);
+    orc_problem_assemble(P, state[disp].data(), state[eeq].data(), f.kappas.data(), 1, 1, 1);
+    Compare("gradient damage", P, 4 * m.nDofNodes, Rall, K);
+    orc_problem_free(P);
+    orc_mesh_free(om);
+
+    bool outOfRange = false;
+    try
+    {
+        DofType other("Temperature", 1);
+        (void)static_cast<const DofVector<double>&>(R)[other];
+    }
+    catch (const std::out_of_range&)
+    {
+        outOfRange = true;
+    }
+    EXPECT(outOfRange, "missing DofVector entry throws std::out_of_range");
+}
+
+static void MomentumBalanceCase()
+{
+    Mesh m = MakeMesh(NUTO_QUAD4, 5, 4, 0);
+    DofType disp("Displacements", 2);
+    DofInfo info;
+    info.numIndependent[disp.Id()] = 2 * m.nDofNodes;
+    CellGroup cells;
+    cells.coordType = NUTO_QUAD4;
+    cells.coords = m.X;
+    cells.conn = m.C;
+    cells.dofNumbering[disp.Id()] = {NUTO_QUAD4, CellDofs(m.ranks, 2)};
+    MomentumBalance f{disp, 30000.0, 0.2, 0};
+    unsigned long long seed = 777;
+    DofVector<double> state;
+    for (int i = 0; i < 2 * m.nDofNodes; i++)
+        state[disp].push_back(Uniform(seed, -1e-3, 1e-3));
+    SimpleAssembler asmbl(info);
+    DofVector<double> R = asmbl.BuildVector(cells, f, state);
+    DofMatrixSparse<double> K = asmbl.BuildMatrix(cells, f, state);
+    void* om = orc_mesh_from_arrays(NUTO_QUAD4, m.nNodes, m.X.data(), m.nCells, m.C.data());
+    const int Ld = orc_mesh_add_dof(om, NUTO_QUAD4, 2);
+    nuto_params p{NUTO_MOMENTUM_LINEAR_ELASTIC, 2, 30000.0, 0.2, 0, 0, 0, 0, 0, 0, 0};
+    void* P = orc_problem(om, &p, 0, 2, Ld, -1);
+    orc_problem_assemble(P, state[disp].data(), nullptr, nullptr, 1, 1, 1);
+    Compare("momentum balance", P, 2 * m.nDofNodes, R[disp], K);
+    orc_problem_free(P);
+    orc_mesh_free(om);
+}
+
+int main()
+{
+    try
+    {
+        GradientDamageCase();
+        MomentumBalanceCase();
+    }
+    catch (const std::exception& e)
+    {
+        std::printf("FAIL exception: %s\n", e.what());
+        return 1;
+    }
+    std::printf(gFailures ? "FAILED (%d)\n" : "ok\n", gFailures);
+    return gFailures ? 1 : 0;
+}
