@@ -55,6 +55,18 @@ constexpr int kMaxAdj = 32;  // neighbours per node
 template <bool FULLDMG>
 constexpr int kStOf = FULLDMG ? 26 : 30;
 constexpr int kGeo = 10;     // doubles per (cell, IP) geometry record: J^-1 (row-major), detJ * w
+// per (cell, IP) state record of the pre-pass (kGdmHex8Ip): with e'' = [e0, e1, e2, e3/2, e4/2, e5/2]
+// (tensor strain) the stress, t = s omega' kappa' d sigma / d omega and m = d eps_eq / d eps are all
+// of the form alpha [1, 1, 1, 0, 0, 0] + beta e'', so B_a^T of each is alpha g_a + beta (e'' g_a)
+constexpr int kSr = 18;
+constexpr int rEps = 0;   // e'' (6)
+constexpr int rSig = 6;   // s alpha_sigma, s beta_sigma
+constexpr int rT = 8;     // alpha_t, beta_t (s included)
+constexpr int rM = 10;    // s alpha_m, s beta_m
+constexpr int rC1 = 12;   // FULL: s (1 - omega); UNILATERAL: s (3K (1 - omega H) - 2G (1 - omega)) / 3
+constexpr int rC2 = 13;   // UNILATERAL: s G (1 - omega); FULL: 0
+constexpr int rRe = 14;   // s (eeq - eps_eq)
+constexpr int rGe = 15;   // s c grad eeq (3)
 constexpr int kPs = 17;      // doubles per partial block (odd: 64 lanes spread over the banks)
 constexpr int kTab = 57;     // per-IP table row: w, dNc[8][3], N[8], dNd[8][3]
 
@@ -66,7 +78,7 @@ constexpr int sMn = 15;   // -N_a s d eps_eq / d eps (6)
 constexpr int sCg = 21;   // c s g_a (3)
 constexpr int sSn = 24;   // s N_a (1)
 constexpr int sKa = 25;   // UNILATERAL: c2 g_a (3)
-constexpr int kRed = 1384;  // doubles of partial blocks, R terms and the zero block after phase B
+constexpr int kRed = 1160;  // doubles after phase B: partial blocks 64 x kPs, R sums 32, zero block 8, slot map 32
 
 // value of lane (lane & ~7) + n within each group of 8 lanes (ds_swizzle bit mode:
 // and-mask 0x18 keeps the group, or-mask n selects the lane)
@@ -268,6 +280,7 @@ struct GdmHex8Tables
     const unsigned char* nodeLa;    // [node][8]: local index of the node in that cell
     const unsigned char* nodeSlot;  // [node][8][8]: neighbour slot (column block) of local node j
     const double* geo;              // [cell][ip][kGeo] geometry records
+    const double* ipState;          // [cell][ip][kSr] pre-pass state records (nullptr: recompute)
     int xcdMap;                     // 1: contiguous node ranges per XCD
     int pBegin, pEnd;               // nodes [pBegin, pEnd) of the launch
 };
@@ -279,10 +292,12 @@ struct NodeFetch
     int q;  // dof node of (r, j)
     double v[4];  // u0 u1 u2 e of node j of cell r
     double g[gdm8::kGeo];  // geometry of cell r at IP j
+    double sr[gdm8::kSr];  // pre-pass state of cell r at IP j
     double th[3];
     double kh;  // history kappa of cell r at IP j
 };
 
+template <bool PRE>
 __device__ inline void FetchNode(const RowArgs& A, const GdmHex8Tables& T, int p, int r, int j, NodeFetch& f)
 {
     const size_t pr = (size_t)p * 8 + r;
@@ -293,10 +308,6 @@ __device__ inline void FetchNode(const RowArgs& A, const GdmHex8Tables& T, int p
     f.q = 0;
     if (f.cell >= 0)
     {
-        const int hist = T.nodeHist ? T.nodeHist[pr] : f.cell;
-        const int q = A.dconn[(size_t)f.cell * 8 + j];
-        f.q = q;
-        f.kh = A.kappa[(size_t)hist + (size_t)j * A.kappaLd];
         const double2* gr = reinterpret_cast<const double2*>(T.geo + ((size_t)f.cell * 8 + j) * gdm8::kGeo);
 #pragma unroll
         for (int k = 0; k < gdm8::kGeo / 2; k++)
@@ -305,10 +316,28 @@ __device__ inline void FetchNode(const RowArgs& A, const GdmHex8Tables& T, int p
             f.g[2 * k] = v.x;
             f.g[2 * k + 1] = v.y;
         }
-        f.v[0] = A.u[(size_t)q * 3];
-        f.v[1] = A.u[(size_t)q * 3 + 1];
-        f.v[2] = A.u[(size_t)q * 3 + 2];
-        f.v[3] = A.e[q];
+        if constexpr (PRE)
+        {
+            const double2* sp = reinterpret_cast<const double2*>(T.ipState + ((size_t)f.cell * 8 + j) * gdm8::kSr);
+#pragma unroll
+            for (int k = 0; k < gdm8::kSr / 2; k++)
+            {
+                const double2 v = sp[k];
+                f.sr[2 * k] = v.x;
+                f.sr[2 * k + 1] = v.y;
+            }
+        }
+        else
+        {
+            const int hist = T.nodeHist ? T.nodeHist[pr] : f.cell;
+            const int q = A.dconn[(size_t)f.cell * 8 + j];
+            f.q = q;
+            f.kh = A.kappa[(size_t)hist + (size_t)j * A.kappaLd];
+            f.v[0] = A.u[(size_t)q * 3];
+            f.v[1] = A.u[(size_t)q * 3 + 1];
+            f.v[2] = A.u[(size_t)q * 3 + 2];
+            f.v[3] = A.e[q];
+        }
         if (A.thE)
         {
             f.th[0] = A.thE[f.cell];
@@ -318,17 +347,171 @@ __device__ inline void FetchNode(const RowArgs& A, const GdmHex8Tables& T, int p
     }
 }
 
+// value of lane i ^ k within each group of 8 lanes (ds_swizzle bit mode: and-mask 0x1F, xor-mask k)
+template <int k>
+__device__ inline double GroupXor(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_ds_swizzle((int)(b & 0xffffffffLL), 0x1F | (k << 10));
+    const int hi = __builtin_amdgcn_ds_swizzle((int)(b >> 32), 0x1F | (k << 10));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// sum over the 8 lanes of a group, ((t0 + t1) + (t2 + t3)) + ((t4 + t5) + (t6 + t7)) in every lane
+// (each butterfly step adds the same two values in every lane, so all lanes hold the same bits)
+__device__ inline double GroupSum(double v)
+{
+    v += GroupXor<1>(v);
+    v += GroupXor<2>(v);
+    v += GroupXor<4>(v);
+    return v;
+}
+
+// geometry and pre-pass state records of (cell, ip)
+__device__ inline void LoadIpRecords(const GdmHex8Tables& T, int cell, int ip, double g[gdm8::kGeo],
+                                     double sr[gdm8::kSr])
+{
+    const double2* gr = reinterpret_cast<const double2*>(T.geo + ((size_t)cell * 8 + ip) * gdm8::kGeo);
+#pragma unroll
+    for (int k = 0; k < gdm8::kGeo / 2; k++)
+    {
+        const double2 v = gr[k];
+        g[2 * k] = v.x;
+        g[2 * k + 1] = v.y;
+    }
+    const double2* sp = reinterpret_cast<const double2*>(T.ipState + ((size_t)cell * 8 + ip) * gdm8::kSr);
+#pragma unroll
+    for (int k = 0; k < gdm8::kSr / 2; k++)
+    {
+        const double2 v = sp[k];
+        sr[2 * k] = v.x;
+        sr[2 * k + 1] = v.y;
+    }
+}
+
+// Terms of row node a (local index la) at IP ip of a cell from its pre-pass state record: the LDS
+// record for phase B (J^-1, s(1-w) g_a or c1 g_a, B_a^T t, -N_a s m, c s g_a, s N_a, c2 g_a) and the
+// R terms s B_a^T sigma, s (N_a (eeq - eps_eq) + c g_a . grad eeq).
+template <bool FULLDMG, int kSt>
+__device__ inline void PreRecord(const double* sr, const double* g, int la, int ip, const LawParams& P, double rec[kSt],
+                                 double Rv[4])
+{
+    using namespace gdm8;
+    const double inv[3][3] = {{g[0], g[1], g[2]}, {g[3], g[4], g[5]}, {g[6], g[7], g[8]}};
+    const double s = g[9];
+    double da[3], Na, ga[3];
+    ShapeAt(la, ip, da, Na);
+    GradOf(da, inv, ga);
+    // e'' g_a: B_a^T of the Voigt vector e''
+    const double Eg[3] = {sr[rEps] * ga[0] + sr[rEps + 5] * ga[1] + sr[rEps + 4] * ga[2],
+                          sr[rEps + 5] * ga[0] + sr[rEps + 1] * ga[1] + sr[rEps + 3] * ga[2],
+                          sr[rEps + 4] * ga[0] + sr[rEps + 3] * ga[1] + sr[rEps + 2] * ga[2]};
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+        Rv[i] = sr[rSig] * ga[i] + sr[rSig + 1] * Eg[i];
+    Rv[3] = Na * sr[rRe] + (ga[0] * sr[rGe] + ga[1] * sr[rGe + 1] + ga[2] * sr[rGe + 2]);
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+            rec[sInv + 3 * a + b] = inv[a][b];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+    {
+        rec[sHa + i] = sr[rC1] * ga[i];
+        rec[sVa + i] = sr[rT] * ga[i] + sr[rT + 1] * Eg[i];
+        rec[sCg + i] = P.c * s * ga[i];
+        if (!FULLDMG)
+            rec[sKa + i] = sr[rC2] * ga[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+        rec[sMn + i] = -Na * ((i < 3 ? sr[rM] : 0.0) + sr[rM + 1] * sr[rEps + i]);
+    rec[sSn] = s * Na;
+    if (FULLDMG)
+        rec[25] = 0.0;
+    else
+        rec[28] = rec[29] = 0.0;
+}
+
+template <int kSt>
+__device__ inline void StoreRecord(double* dst, const double rec[kSt])
+{
+#pragma unroll
+    for (int k = 0; k < kSt; k += 2)
+        *reinterpret_cast<double2*>(dst + k) = make_double2(rec[k], rec[k + 1]);
+}
+
+// Phase B: lane (r, b) accumulates block (a_r, b) over IPs ip0 .. ip0 + n - 1 (records at
+// state[(i * 8 + r) * kSt], i = ip - ip0), in IP order.
+template <bool FULLDMG, int kSt>
+__device__ inline void PhaseB(const double* state, int r, int b, int ip0, int n, double acc[16])
+{
+    using namespace gdm8;
+#pragma unroll 1
+    for (int i = 0; i < n; i++)
+    {
+        const int ip = ip0 + i;
+        double db[3], Nb;
+        ShapeAt(b, ip, db, Nb);
+        const double* st = state + (i * 8 + r) * kSt;
+        constexpr int kRd = FULLDMG ? 26 : 28;
+        double rec[kRd];
+#pragma unroll
+        for (int k = 0; k < kRd; k += 2)
+        {
+            const double2 v = *reinterpret_cast<const double2*>(st + k);
+            rec[k] = v.x;
+            rec[k + 1] = v.y;
+        }
+        const double inv[3][3] = {{rec[0], rec[1], rec[2]}, {rec[3], rec[4], rec[5]}, {rec[6], rec[7], rec[8]}};
+        double gb[3];
+        GradOf(db, inv, gb);
+        const double* ha = rec + sHa;
+#pragma unroll
+        for (int ii = 0; ii < 3; ii++)
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                acc[ii * 4 + c] += ha[ii] * gb[c];
+        if (!FULLDMG)
+        {
+            const double* ka = rec + sKa;
+            const double d = ka[0] * gb[0] + ka[1] * gb[1] + ka[2] * gb[2];
+#pragma unroll
+            for (int ii = 0; ii < 3; ii++)
+            {
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+                    acc[ii * 4 + c] += gb[ii] * ka[c];
+                acc[ii * 5] += d;
+            }
+        }
+        acc[3] += rec[sVa] * Nb;
+        acc[7] += rec[sVa + 1] * Nb;
+        acc[11] += rec[sVa + 2] * Nb;
+        double wb[3];
+        BTv3(gb, rec + sMn, wb);
+        acc[12] += wb[0];
+        acc[13] += wb[1];
+        acc[14] += wb[2];
+        acc[15] += rec[sSn] * Nb + rec[sCg] * gb[0] + rec[sCg + 1] * gb[1] + rec[sCg + 2] * gb[2];
+    }
+}
+
 // One workgroup per node. Workgroups are dispatched round-robin over the 8 XCDs; with
 // xcdMap the node of block b is chosen so that each XCD processes a contiguous range of nodes
 // (neighbouring nodes share cells and node data, which then stay in that XCD's L2).
-template <bool WR, bool WK, bool FULLDMG>
+// PRE: the IP states come from kGdmHex8Ip; phase B then runs in two passes of 4 IPs so that only
+// 32 records are in LDS at once (9.3 KB per workgroup: 4 waves per SIMD at <= 128 VGPRs).
+template <bool WR, bool WK, bool FULLDMG, bool PRE>
 __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, const GdmHex8Tables T)
 {
     using namespace gdm8;
     constexpr int kSt = kStOf<FULLDMG>;
-    static_assert(8 * 8 * kSt * 8 >= kRed * 8 + kMaxAdj * kMaxInc, "LDS layout");
-    // [ip][r][kSt] records; after phase B: partial blocks, R terms, zero block and the slot map
-    __shared__ __align__(16) double state[8 * 8 * kSt];
+    constexpr int kRecIps = PRE ? 4 : 8;  // IPs whose records are in LDS at a time
+    constexpr int kState = 8 * kRecIps * kSt > kRed ? 8 * kRecIps * kSt : kRed;
+    // records [i][r][kSt]; after phase B: partial blocks, R sums, zero block and the slot map
+    __shared__ __align__(16) double state[kState];
 
     const int lane = threadIdx.x;
     const int r = lane >> 3, j = lane & 7;
@@ -348,7 +531,7 @@ __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, const GdmHex
     const int nA = L >> 2;
     const int re0 = A.rowPtr[(size_t)A.nU + p];
     NodeFetch f;
-    FetchNode(A, T, p, r, j, f);
+    FetchNode<PRE>(A, T, p, r, j, f);
     const bool valid = f.cell >= 0;  // r < number of cells incident to p
     const int la = f.la;
 
@@ -362,153 +545,140 @@ __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, const GdmHex
     }
     const double m3K = P.cK * E, m2G = P.cG * E;
 
-    // ---- phase A: lane (r, ip) ---------------------------------------------------------------------
-    double Ru0 = 0.0, Ru1 = 0.0, Ru2 = 0.0, Re = 0.0;
-    if (valid)
-    {
-        const int ip = j;
-        double Hu[3][3], he[3], eN;
-        GatherGradients(f.v, ip, Hu, he, eN);
-        const double inv[3][3] = {{f.g[0], f.g[1], f.g[2]}, {f.g[3], f.g[4], f.g[5]}, {f.g[6], f.g[7], f.g[8]}};
-        const double s = f.g[9];
-        GdmIp st;
-        GdmIpState<FULLDMG>(Hu, he, eN, inv, f.kh, m3K, m2G, k0, beta, P, st);
-        const double* sig = st.sig;
-        const double* dsw = st.dsw;
-        const double* ge = st.ge;
-        const double om = st.om, dom = st.dom, dk = st.dk, H = st.H, d2G = st.d2G, eq = st.eq;
-
-        double da[3], Na, ga[3];
-        ShapeAt(la, ip, da, Na);
-        GradOf(da, inv, ga);
-        if (WR)
-        {
-            double bt[3];
-            BTv3(ga, sig, bt);
-            Ru0 = s * bt[0];
-            Ru1 = s * bt[1];
-            Ru2 = s * bt[2];
-            Re = s * (Na * (eN - eq) + P.c * (ga[0] * ge[0] + ga[1] * ge[1] + ga[2] * ge[2]));
-        }
-        if (WK)
-        {
-            double m[6];
-            GdmMisesDerivative(st, P, m);
-            double* rp = state + (ip * 8 + r) * kSt;
-            double rec[kSt];
-#pragma unroll
-            for (int a = 0; a < 3; a++)
-#pragma unroll
-                for (int b = 0; b < 3; b++)
-                    rec[sInv + 3 * a + b] = inv[a][b];
-            const double ct = s * dom * dk;
-            double t6[6], va[3];
-#pragma unroll
-            for (int i = 0; i < 6; i++)
-            {
-                t6[i] = ct * dsw[i];
-                rec[sMn + i] = -Na * s * m[i];
-            }
-            BTv3(ga, t6, va);
-            const double c1 = FULLDMG ? s * (1.0 - om) : s * (m3K * (1.0 - om * H) - d2G) * kThird;
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-            {
-                rec[sHa + i] = c1 * ga[i];
-                rec[sVa + i] = va[i];
-                rec[sCg + i] = P.c * s * ga[i];
-                if (!FULLDMG)
-                    rec[sKa + i] = 0.5 * s * d2G * ga[i];
-            }
-            rec[sSn] = s * Na;
-            if (FULLDMG)
-                rec[25] = 0.0;
-            else
-                rec[28] = rec[29] = 0.0;
-#pragma unroll
-            for (int k = 0; k < kSt; k += 2)
-                *reinterpret_cast<double2*>(rp + k) = make_double2(rec[k], rec[k + 1]);
-        }
-    }
-    __syncthreads();
-
-    // ---- phase B: lane (r, b) accumulates block (a_r, b) over the 8 IPs ------------------------------
+    double Rv[4] = {0.0, 0.0, 0.0, 0.0};
     double acc[16];
 #pragma unroll
     for (int i = 0; i < 16; i++)
         acc[i] = 0.0;
-    if (WK && valid)
+    if constexpr (PRE)
     {
-        const int b = j;
-#pragma unroll 1
-        for (int ip = 0; ip < 8; ip++)
+        double rec[kSt];
+        if (valid)
+            PreRecord<FULLDMG, kSt>(f.sr, f.g, la, j, P, rec, Rv);
+        if (WK)
         {
-            double db[3], Nb;
-            ShapeAt(b, ip, db, Nb);
-            const double* st = state + (ip * 8 + r) * kSt;
-            constexpr int kRd = FULLDMG ? 26 : 28;
-            double rec[kRd];
-#pragma unroll
-            for (int k = 0; k < kRd; k += 2)
+            // pass 0: IPs 0..3 (records of lanes j < 4)
+            if (valid && j < 4)
+                StoreRecord<kSt>(state + (j * 8 + r) * kSt, rec);
+            __syncthreads();
+            if (valid)
+                PhaseB<FULLDMG, kSt>(state, r, j, 0, 4, acc);
+            __syncthreads();
+            // pass 1: IPs 4..7; lanes j >= 4 rebuild their records from the (cached) state records
+            if (valid && j >= 4)
             {
-                const double2 v = *reinterpret_cast<const double2*>(st + k);
-                rec[k] = v.x;
-                rec[k + 1] = v.y;
+                double g2[kGeo], sr2[kSr], Rd[4];
+                LoadIpRecords(T, f.cell, j, g2, sr2);
+                PreRecord<FULLDMG, kSt>(sr2, g2, la, j, P, rec, Rd);
+                StoreRecord<kSt>(state + ((j - 4) * 8 + r) * kSt, rec);
             }
-            const double inv[3][3] = {{rec[0], rec[1], rec[2]}, {rec[3], rec[4], rec[5]}, {rec[6], rec[7], rec[8]}};
-            double gb[3];
-            GradOf(db, inv, gb);
-            const double* ha = rec + sHa;
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int c = 0; c < 3; c++)
-                    acc[i * 4 + c] += ha[i] * gb[c];
-            if (!FULLDMG)
+            __syncthreads();
+            if (valid)
+                PhaseB<FULLDMG, kSt>(state, r, j, 4, 4, acc);
+        }
+    }
+    else
+    {
+        if (valid)
+        {
+            const int ip = j;
+            double Hu[3][3], he[3], eN;
+            GatherGradients(f.v, ip, Hu, he, eN);
+            const double inv[3][3] = {{f.g[0], f.g[1], f.g[2]}, {f.g[3], f.g[4], f.g[5]}, {f.g[6], f.g[7], f.g[8]}};
+            const double s = f.g[9];
+            GdmIp st;
+            GdmIpState<FULLDMG>(Hu, he, eN, inv, f.kh, m3K, m2G, k0, beta, P, st);
+            const double* sig = st.sig;
+            const double* dsw = st.dsw;
+            const double* ge = st.ge;
+            const double om = st.om, dom = st.dom, dk = st.dk, H = st.H, d2G = st.d2G, eq = st.eq;
+
+            double da[3], Na, ga[3];
+            ShapeAt(la, ip, da, Na);
+            GradOf(da, inv, ga);
+            if (WR)
             {
-                const double* ka = rec + sKa;
-                const double d = ka[0] * gb[0] + ka[1] * gb[1] + ka[2] * gb[2];
+                double bt[3];
+                BTv3(ga, sig, bt);
+                Rv[0] = s * bt[0];
+                Rv[1] = s * bt[1];
+                Rv[2] = s * bt[2];
+                Rv[3] = s * (Na * (eN - eq) + P.c * (ga[0] * ge[0] + ga[1] * ge[1] + ga[2] * ge[2]));
+            }
+            if (WK)
+            {
+                double m[6];
+                GdmMisesDerivative(st, P, m);
+                double* rp = state + (ip * 8 + r) * kSt;
+                double rec[kSt];
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+#pragma unroll
+                    for (int b = 0; b < 3; b++)
+                        rec[sInv + 3 * a + b] = inv[a][b];
+                const double ct = s * dom * dk;
+                double t6[6], va[3];
+#pragma unroll
+                for (int i = 0; i < 6; i++)
+                {
+                    t6[i] = ct * dsw[i];
+                    rec[sMn + i] = -Na * s * m[i];
+                }
+                BTv3(ga, t6, va);
+                const double c1 = FULLDMG ? s * (1.0 - om) : s * (m3K * (1.0 - om * H) - d2G) * kThird;
 #pragma unroll
                 for (int i = 0; i < 3; i++)
                 {
-#pragma unroll
-                    for (int c = 0; c < 3; c++)
-                        acc[i * 4 + c] += gb[i] * ka[c];
-                    acc[i * 5] += d;
+                    rec[sHa + i] = c1 * ga[i];
+                    rec[sVa + i] = va[i];
+                    rec[sCg + i] = P.c * s * ga[i];
+                    if (!FULLDMG)
+                        rec[sKa + i] = 0.5 * s * d2G * ga[i];
                 }
+                rec[sSn] = s * Na;
+                if (FULLDMG)
+                    rec[25] = 0.0;
+                else
+                    rec[28] = rec[29] = 0.0;
+#pragma unroll
+                for (int k = 0; k < kSt; k += 2)
+                    *reinterpret_cast<double2*>(rp + k) = make_double2(rec[k], rec[k + 1]);
             }
-            acc[3] += rec[sVa] * Nb;
-            acc[7] += rec[sVa + 1] * Nb;
-            acc[11] += rec[sVa + 2] * Nb;
-            double wb[3];
-            BTv3(gb, rec + sMn, wb);
-            acc[12] += wb[0];
-            acc[13] += wb[1];
-            acc[14] += wb[2];
-            acc[15] += rec[sSn] * Nb + rec[sCg] * gb[0] + rec[sCg + 1] * gb[1] + rec[sCg + 2] * gb[2];
         }
-        if (FULLDMG)
-        {
-            // uu = lambda' S + mu' S^T + mu' tr(S) I
-            const double lam = (m3K - m2G) * kThird, mu = 0.5 * m2G;
-            const double S[3][3] = {{acc[0], acc[1], acc[2]}, {acc[4], acc[5], acc[6]}, {acc[8], acc[9], acc[10]}};
-            const double tr = S[0][0] + S[1][1] + S[2][2];
+    }
+    if constexpr (!PRE)
+    {
+        __syncthreads();
+        if (WK && valid)
+            PhaseB<FULLDMG, kSt>(state, r, j, 0, 8, acc);
+    }
+    if (WK && valid && FULLDMG)
+    {
+        // uu = lambda' S + mu' S^T + mu' tr(S) I
+        const double lam = (m3K - m2G) * kThird, mu = 0.5 * m2G;
+        const double S[3][3] = {{acc[0], acc[1], acc[2]}, {acc[4], acc[5], acc[6]}, {acc[8], acc[9], acc[10]}};
+        const double tr = S[0][0] + S[1][1] + S[2][2];
 #pragma unroll
-            for (int i = 0; i < 3; i++)
+        for (int i = 0; i < 3; i++)
 #pragma unroll
-                for (int c = 0; c < 3; c++)
-                    acc[i * 4 + c] = lam * S[i][c] + mu * S[c][i] + (i == c ? mu * tr : 0.0);
-        }
+            for (int c = 0; c < 3; c++)
+                acc[i * 4 + c] = lam * S[i][c] + mu * S[c][i] + (i == c ? mu * tr : 0.0);
+    }
+    // R: per cell over its IPs (butterfly within the cell's 8 lanes), then over the cells in order
+    if (WR)
+    {
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            Rv[k] = GroupSum(Rv[k]);
     }
     __syncthreads();
 
-    // ---- partial blocks, R terms and the slot map to LDS (the records are dead now) -----------------
-    double* part = state;                   // [r][b][kPs]
-    double* rterm = state + 8 * 8 * kPs;    // [4][r][ip]
-    double* rcell = rterm + 4 * 64;         // [4][r]: R terms summed over the IPs of cell r
-    double* zero = rcell + 32;              // 8 zeros: the block of an absent (cell, slot) pair
+    // ---- partial blocks, R sums and the slot map to LDS (the records are dead now) ------------------
+    double* part = state;               // [r][b][kPs]
+    double* rcell = state + 8 * 8 * kPs;  // [4][r]: R terms summed over the IPs of cell r
+    double* zero = rcell + 32;          // 8 zeros: the block of an absent (cell, slot) pair
     // slotOf[q][r] = local index b of neighbour q in cell r, 0xFF if q is not a node of cell r
-    unsigned char(*slotOf)[kMaxInc] = reinterpret_cast<unsigned char(*)[kMaxInc]>(state + kRed);
+    unsigned char(*slotOf)[kMaxInc] = reinterpret_cast<unsigned char(*)[kMaxInc]>(zero + 8);
     if (WK)
     {
         reinterpret_cast<unsigned int*>(slotOf)[lane] = 0xFFFFFFFFu;  // 64 x 4 bytes = the whole map
@@ -518,26 +688,15 @@ __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, const GdmHex
         if (lane < 8)
             zero[lane] = 0.0;
     }
-    if (WR)
+    if (WR && j == 0)
     {
-        rterm[0 * 64 + lane] = Ru0;
-        rterm[1 * 64 + lane] = Ru1;
-        rterm[2 * 64 + lane] = Ru2;
-        rterm[3 * 64 + lane] = Re;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            rcell[k * 8 + r] = Rv[k];
     }
     __syncthreads();
     if (WK && valid)
         slotOf[f.slot][r] = (unsigned char)j;
-    // R: per cell over its IPs in order (Cell.h:108-117), then over the cells in order
-    if (WR && lane < 32)
-    {
-        const double* t = rterm + (lane >> 3) * 64 + (lane & 7) * 8;
-        double s = t[0];
-#pragma unroll
-        for (int ip = 1; ip < 8; ip++)
-            s += t[ip];
-        rcell[lane] = s;
-    }
     __syncthreads();
 
     // ---- reduce: lane (q, half) sums its 8 block entries over the cells in order and writes them ----
@@ -586,6 +745,92 @@ __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, const GdmHex
     }
 }
 
+// Pre-pass: the state of every (cell, IP) once, for the node kernel (which would otherwise evaluate
+// it in each of the cell's 8 node workgroups). Lane (c, ip) of 8-lane group c; node j of the cell is
+// gathered by lane (c, j) and shared by swizzle, as in the node kernel.
+template <bool FULLDMG>
+__global__ void __launch_bounds__(64) kGdmHex8Ip(const RowArgs A, const GdmHex8Tables T, double* __restrict__ out)
+{
+    using namespace gdm8;
+    const long long t = (long long)blockIdx.x * 64 + threadIdx.x;
+    const int j = (int)(t & 7);
+    const long long cl = t >> 3;
+    const bool valid = cl < A.nCells;  // uniform within each 8-lane group
+    const int cell = valid ? (int)cl : 0;
+    const LawParams& P = A.P;
+    double v[4] = {0.0, 0.0, 0.0, 0.0}, g[kGeo];
+    double kh = 0.0, E = P.E, k0 = P.kappa0, beta = P.beta;
+#pragma unroll
+    for (int k = 0; k < kGeo; k++)
+        g[k] = 0.0;
+    if (valid)
+    {
+        const int q = A.dconn[(size_t)cell * 8 + j];
+        v[0] = A.u[(size_t)q * 3];
+        v[1] = A.u[(size_t)q * 3 + 1];
+        v[2] = A.u[(size_t)q * 3 + 2];
+        v[3] = A.e[q];
+        const int hist = A.cellId ? A.cellId[cell] : cell;
+        kh = A.kappa[(size_t)hist + (size_t)j * A.kappaLd];
+        const double2* gr = reinterpret_cast<const double2*>(T.geo + ((size_t)cell * 8 + j) * kGeo);
+#pragma unroll
+        for (int k = 0; k < kGeo / 2; k++)
+        {
+            const double2 w = gr[k];
+            g[2 * k] = w.x;
+            g[2 * k + 1] = w.y;
+        }
+        if (A.thE)
+        {
+            E = A.thE[cell];
+            k0 = A.thFt[cell] / E;
+            beta = A.thFt[cell] / A.thGf[cell];
+        }
+    }
+    double Hu[3][3], he[3], eN;
+    GatherGradients(v, j, Hu, he, eN);
+    if (!valid)
+        return;
+    const double inv[3][3] = {{g[0], g[1], g[2]}, {g[3], g[4], g[5]}, {g[6], g[7], g[8]}};
+    const double s = g[9];
+    const double m3K = P.cK * E, m2G = P.cG * E;
+    GdmIp st;
+    GdmIpState<FULLDMG>(Hu, he, eN, inv, kh, m3K, m2G, k0, beta, P, st);
+    const double eV = st.I1 * kThird;
+    const double sV = (1.0 - st.om * st.H) * m3K * eV;
+    double am = P.K1, bm = 0.0;  // m = am [1,1,1,0,0,0] + bm e'' (A == 0: m = K1 [1,1,1,0,0,0])
+    if (st.Am != 0.0)
+    {
+        const double h = 0.5 / st.Am;
+        am = P.K1 + h * (2.0 * P.K1 * P.K1 * st.I1) - h * P.K2 * st.I1 * kThird;
+        bm = h * P.K2;
+    }
+    const double ct = s * st.dom * st.dk;
+    double rec[kSr];
+    rec[rEps] = st.eps[0];
+    rec[rEps + 1] = st.eps[1];
+    rec[rEps + 2] = st.eps[2];
+    rec[rEps + 3] = 0.5 * st.eps[3];
+    rec[rEps + 4] = 0.5 * st.eps[4];
+    rec[rEps + 5] = 0.5 * st.eps[5];
+    rec[rSig] = s * (sV - st.d2G * eV);
+    rec[rSig + 1] = s * st.d2G;
+    rec[rT] = -ct * (m3K * st.H - m2G) * eV;
+    rec[rT + 1] = -ct * m2G;
+    rec[rM] = s * am;
+    rec[rM + 1] = s * bm;
+    rec[rC1] = FULLDMG ? s * (1.0 - st.om) : s * (m3K * (1.0 - st.om * st.H) - st.d2G) * kThird;
+    rec[rC2] = FULLDMG ? 0.0 : 0.5 * s * st.d2G;
+    rec[rRe] = s * (st.eN - st.eq);
+    rec[rGe] = s * P.c * st.ge[0];
+    rec[rGe + 1] = s * P.c * st.ge[1];
+    rec[rGe + 2] = s * P.c * st.ge[2];
+    double2* o = reinterpret_cast<double2*>(out + ((size_t)cell * 8 + j) * kSr);
+#pragma unroll
+    for (int k = 0; k < kSr / 2; k++)
+        o[k] = make_double2(rec[2 * k], rec[2 * k + 1]);
+}
+
 // Geometry of every (cell, IP): J = sum_n X_n dN_n/dxi^T over the coordinate nodes in local
 // order (Jacobian.h:16-48), J^-1 and detJ (Jacobian.h:35-47), detJ * w (Cell.h:108-117).
 __global__ void __launch_bounds__(256) kGdmHex8Geometry(const double* x, const double* y, const double* z,
@@ -629,12 +874,15 @@ struct GdmHex8Plan
     int grid = 0;     // workgroups: one per node (rounded up to a multiple of 8 with the XCD mapping)
     int xcdMap = 1;   // contiguous node ranges per XCD (env NUTO_AMD_GDM_XCD_MAP=0 switches it off)
     DevBuf nodeCell, nodeHist, nodeLa, nodeSlot, geo;
+    DevBuf ipState;      // pre-pass state records (NUTO_AMD_GDM_PREPASS=0: recompute in the node kernel)
+    bool prepass = true;
+    int nCells = 0;
 
     void Release()
     {
         ready = false;
         nNodes = 0;
-        for (DevBuf* b : {&nodeCell, &nodeHist, &nodeLa, &nodeSlot, &geo})
+        for (DevBuf* b : {&nodeCell, &nodeHist, &nodeLa, &nodeSlot, &geo, &ipState})
             b->Release();
     }
 
@@ -695,6 +943,14 @@ struct GdmHex8Plan
                 return false;
             }
         }
+        const char* pre = std::getenv("NUTO_AMD_GDM_PREPASS");
+        prepass = !(pre && pre[0] == '0');
+        if (prepass && (err = ipState.Alloc((size_t)nCells * 8 * gdm8::kSr * sizeof(double))) != hipSuccess)
+        {
+            Release();
+            return false;
+        }
+        this->nCells = nCells;
         nNodes = n;
         const char* env = std::getenv("NUTO_AMD_GDM_XCD_MAP");
         xcdMap = env ? std::atoi(env) : 1;
@@ -706,20 +962,34 @@ struct GdmHex8Plan
     GdmHex8Tables Tables() const
     {
         return GdmHex8Tables{nodeCell.As<int>(), nodeHist.p ? nodeHist.As<int>() : nullptr,
-                             nodeLa.As<unsigned char>(), nodeSlot.As<unsigned char>(), geo.As<double>(), xcdMap, 0,
-                             nNodes};
+                             nodeLa.As<unsigned char>(), nodeSlot.As<unsigned char>(), geo.As<double>(),
+                             ipState.p ? ipState.As<double>() : nullptr, xcdMap, 0, nNodes};
+    }
+
+    template <bool FULLDMG, bool PRE>
+    void LaunchNodes(const RowArgs& a, const GdmHex8Tables& t, bool wr, bool wk, int grid_, hipStream_t stream)
+    {
+        if (wr && wk)
+            hipLaunchKernelGGL((kGdmHex8Node<true, true, FULLDMG, PRE>), dim3(grid_), dim3(64), 0, stream, a, t);
+        else if (wr)
+            hipLaunchKernelGGL((kGdmHex8Node<true, false, FULLDMG, PRE>), dim3(grid_), dim3(64), 0, stream, a, t);
+        else
+            hipLaunchKernelGGL((kGdmHex8Node<false, true, FULLDMG, PRE>), dim3(grid_), dim3(64), 0, stream, a, t);
     }
 
     template <bool FULLDMG>
     void LaunchT(const RowArgs& a, bool wr, bool wk, hipStream_t stream)
     {
         const GdmHex8Tables t = Tables();
-        if (wr && wk)
-            hipLaunchKernelGGL((kGdmHex8Node<true, true, FULLDMG>), dim3(grid), dim3(64), 0, stream, a, t);
-        else if (wr)
-            hipLaunchKernelGGL((kGdmHex8Node<true, false, FULLDMG>), dim3(grid), dim3(64), 0, stream, a, t);
+        if (prepass)
+        {
+            if (nCells > 0)
+                hipLaunchKernelGGL((kGdmHex8Ip<FULLDMG>), dim3((unsigned)(((long long)nCells * 8 + 63) / 64)), dim3(64),
+                                   0, stream, a, t, ipState.As<double>());
+            LaunchNodes<FULLDMG, true>(a, t, wr, wk, grid, stream);
+        }
         else
-            hipLaunchKernelGGL((kGdmHex8Node<false, true, FULLDMG>), dim3(grid), dim3(64), 0, stream, a, t);
+            LaunchNodes<FULLDMG, false>(a, t, wr, wk, grid, stream);
     }
 
     void Launch(const RowArgs& a, bool wr, bool wk, hipStream_t stream)
@@ -741,10 +1011,11 @@ struct GdmHex8Plan
         t.xcdMap = 0;
         t.pBegin = first;
         t.pEnd = end;
+        // per-mesh launches recompute the IP state in the node kernel (no pre-pass over the mesh's cells)
         if (a.P.damage_application == 0)
-            hipLaunchKernelGGL((kGdmHex8Node<true, true, true>), dim3(end - first), dim3(64), 0, stream, a, t);
+            LaunchNodes<true, false>(a, t, true, true, end - first, stream);
         else
-            hipLaunchKernelGGL((kGdmHex8Node<true, true, false>), dim3(end - first), dim3(64), 0, stream, a, t);
+            LaunchNodes<false, false>(a, t, true, true, end - first, stream);
     }
 };
 

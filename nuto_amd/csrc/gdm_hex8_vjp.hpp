@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(64) kGdmHex8Vjp(const RowArgs A, const GdmHex8
     if (p >= T.pEnd)
         return;
     NodeFetch f;
-    FetchNode(A, T, p, r, j, f);
+    FetchNode<false>(A, T, p, r, j, f);
     const bool valid = f.cell >= 0;
     const int la = f.la;
     double lv[4] = {0.0, 0.0, 0.0, 0.0};  // lambda of node j of cell r
