@@ -81,6 +81,7 @@ def lib():
         L.orc_problem_K.argtypes = [P, ip, ip, P, P]
         L.orc_problem_cell.argtypes = [P, ctypes.c_int, P, P, P, P, P]
         L.orc_problem_cell_dofs.argtypes = [P, ctypes.c_int, ip]
+        L.orc_set_mutation.argtypes = [ctypes.c_int]
         L.orc_mises.restype = ctypes.c_double
         L.orc_damage_exp.restype = ctypes.c_double
         L.orc_mock_assemble.restype = ctypes.c_longlong
@@ -321,3 +322,8 @@ def csr_to_dense(rp, ci, v, n, m=None):
     for r in range(n):
         A[r, ci[rp[r]:rp[r + 1]]] = v[rp[r]:rp[r + 1]]
     return A
+
+
+def set_mutation(m):
+    """Plant mistake m in the oracle (0: none); for tests that show the parity rule rejects it."""
+    lib().orc_set_mutation(int(m))

@@ -481,6 +481,14 @@ static Mat MulAbs(const Mat& a, const Mat& b)
 // Jacobian (Jacobian.h:16-48,100-103,131-146): J = X(dim x n) * dN(n x dim), inverse by the
 // adjugate, determinant by cofactor expansion along the first row.
 // ---------------------------------------------------------------------------------------------
+// Planted mistakes (tests only, off by default): tests/test_parity_sensitivity.py switches one on
+// and checks that the parity rule used against the device path rejects the result.
+//   1 drop the c B_e^T B_e term of K_ee        2 flip the sign of K_eu
+//   3 halve the engineering shear strains       4 kappa = eeq (history max dropped)
+//   5 flip the sign of the c B_e^T grad(eeq) term of R_e
+//   6 dN/dX with J^-T instead of J^-1 (transposed operand)
+static int g_mutation = 0;
+
 struct Jacobian
 {
     int d = 0;
@@ -1104,6 +1112,10 @@ static void IntegrateCell(const Problem& P, int c, const double* u, const double
         DShapeDim(m.coordType, xi, dNc);
         Jacobian jac = MakeJacobian(dim, nnC, Xc.data(), dNc);
         double detJ = jac.det;
+        if (g_mutation == 6)
+            for (int a = 0; a < dim; a++)
+                for (int b = a + 1; b < dim; b++)
+                    std::swap(jac.inv[a * dim + b], jac.inv[b * dim + a]);
 
         // per dof type: N, dN/dX
         std::vector<double> Nv[2];
@@ -1192,10 +1204,13 @@ static void IntegrateCell(const Problem& P, int c, const double* u, const double
             Ne.v = Nv[1];
             double eeq = Mul(Ne, Col(nodeVal[1])).v[0];
             double kh = kappa[(size_t)c + (size_t)ip * m.nCells];
-            double kap = std::max(kh, eeq);
+            double kap = g_mutation == 4 ? eeq : std::max(kh, eeq);
             double omega = dl.Damage(kap);
             Mat eeqGrad = Mul(Be, Col(nodeVal[1]));
             Mat strainM = Mul(Bd, Col(nodeVal[0]));
+            if (g_mutation == 3)
+                for (int i = 3; i < 6; i++)
+                    strainM.v[i] *= 0.5;
             const double* strain = strainM.v.data();
             const double g = 1., dgdw = 0.;
             Mat strainMag = MulAbs(Bd, Col(nodeVal[0]));
@@ -1205,7 +1220,7 @@ static void IntegrateCell(const Problem& P, int c, const double* u, const double
             {
                 Rip[0] = Mul(T(Bd), Col(led.Stress(strain, omega))).v;
                 Mat r1 = Scale(T(Ne), eeq - norm.Value(strain));
-                Mat r2 = Mul(T(Be), Scale(eeqGrad, P.p.c * g));
+                Mat r2 = Mul(T(Be), Scale(eeqGrad, (g_mutation == 5 ? -1.0 : 1.0) * P.p.c * g));
                 Rip[1] = Add(r1, r2).v;
                 Rmag[0] = MulAbs(T(Bd), Col(std::vector<double>(lm.sig, lm.sig + 6))).v;
                 Mat m1 = Scale(Abs(T(Ne)), std::abs(eeq) + lm.eeq);
@@ -1219,8 +1234,8 @@ static void IntegrateCell(const Problem& P, int c, const double* u, const double
                 Kip[0][0] = Mul(Mul(T(Bd), led.DstressDstrain(strain, omega)), Bd);
                 Mat dn(1, 6);
                 dn.v = norm.Derivative(strain);
-                Kip[1][0] = Mul(Mul(Scale(T(Ne), -1.), dn), Bd);
-                Mat ee = Add(Mul(T(Ne), Ne), Scale(Mul(T(Be), Be), P.p.c * g));
+                Kip[1][0] = Mul(Mul(Scale(T(Ne), g_mutation == 2 ? 1. : -1.), dn), Bd);
+                Mat ee = g_mutation == 1 ? Mul(T(Ne), Ne) : Add(Mul(T(Ne), Ne), Scale(Mul(T(Be), Be), P.p.c * g));
                 Mat ee3 = Mul(Scale(Scale(Mul(T(Be), Scale(eeqGrad, P.p.c)), dgdw), dwdk * dkde), Ne);
                 Kip[1][1] = Add(ee, ee3);
                 Mat dsw = Col(led.DstressDomega(strain));
@@ -1553,6 +1568,11 @@ void* orc_mesh_bricks(int nx, int ny, int nz)
 {
     return new Mesh(CreateBricks(nx, ny, nz));
 }
+void orc_set_mutation(int m)
+{
+    g_mutation = m;
+}
+
 void* orc_mesh_from_arrays(int coordType, int nNodes, const double* X, int nCells, const int* conn)
 {
     Mesh* m = new Mesh;
