@@ -392,11 +392,14 @@ __device__ inline void LoadIpRecords(const GdmHex8Tables& T, int cell, int ip, d
 // Terms of row node a (local index la) at IP ip of a cell from its pre-pass state record: the LDS
 // record for phase B (J^-1, s(1-w) g_a or c1 g_a, B_a^T t, -N_a s m, c s g_a, s N_a, c2 g_a) and the
 // R terms s B_a^T sigma, s (N_a (eeq - eps_eq) + c g_a . grad eeq).
-template <bool FULLDMG, int kSt>
+// WITHINV false: the record omits J^-1 (affine cells keep one J^-1 per cell) and the other terms
+// start at 0 instead of 9.
+template <bool FULLDMG, int kSt, bool WITHINV = true>
 __device__ inline void PreRecord(const double* sr, const double* g, int la, int ip, const LawParams& P, double rec[kSt],
                                  double Rv[4])
 {
     using namespace gdm8;
+    constexpr int o = WITHINV ? 0 : -sHa;  // index shift of the terms after J^-1
     const double inv[3][3] = {{g[0], g[1], g[2]}, {g[3], g[4], g[5]}, {g[6], g[7], g[8]}};
     const double s = g[9];
     double da[3], Na, ga[3];
@@ -410,28 +413,30 @@ __device__ inline void PreRecord(const double* sr, const double* g, int la, int 
     for (int i = 0; i < 3; i++)
         Rv[i] = sr[rSig] * ga[i] + sr[rSig + 1] * Eg[i];
     Rv[3] = Na * sr[rRe] + (ga[0] * sr[rGe] + ga[1] * sr[rGe + 1] + ga[2] * sr[rGe + 2]);
+    if constexpr (WITHINV)
+    {
 #pragma unroll
-    for (int a = 0; a < 3; a++)
+        for (int a = 0; a < 3; a++)
 #pragma unroll
-        for (int b = 0; b < 3; b++)
-            rec[sInv + 3 * a + b] = inv[a][b];
+            for (int b = 0; b < 3; b++)
+                rec[sInv + 3 * a + b] = inv[a][b];
+    }
 #pragma unroll
     for (int i = 0; i < 3; i++)
     {
-        rec[sHa + i] = sr[rC1] * ga[i];
-        rec[sVa + i] = sr[rT] * ga[i] + sr[rT + 1] * Eg[i];
-        rec[sCg + i] = P.c * s * ga[i];
+        rec[o + sHa + i] = sr[rC1] * ga[i];
+        rec[o + sVa + i] = sr[rT] * ga[i] + sr[rT + 1] * Eg[i];
+        rec[o + sCg + i] = P.c * s * ga[i];
         if (!FULLDMG)
-            rec[sKa + i] = sr[rC2] * ga[i];
+            rec[o + sKa + i] = sr[rC2] * ga[i];
     }
 #pragma unroll
     for (int i = 0; i < 6; i++)
-        rec[sMn + i] = -Na * ((i < 3 ? sr[rM] : 0.0) + sr[rM + 1] * sr[rEps + i]);
-    rec[sSn] = s * Na;
-    if (FULLDMG)
-        rec[25] = 0.0;
-    else
-        rec[28] = rec[29] = 0.0;
+        rec[o + sMn + i] = -Na * ((i < 3 ? sr[rM] : 0.0) + sr[rM + 1] * sr[rEps + i]);
+    rec[o + sSn] = s * Na;
+#pragma unroll
+    for (int k = o + (FULLDMG ? sSn + 1 : sKa + 3); k < kSt; k++)
+        rec[k] = 0.0;  // padding
 }
 
 template <int kSt>
@@ -444,10 +449,12 @@ __device__ inline void StoreRecord(double* dst, const double rec[kSt])
 
 // Phase B: lane (r, b) accumulates block (a_r, b) over IPs ip0 .. ip0 + n - 1 (records at
 // state[(i * 8 + r) * kSt], i = ip - ip0), in IP order.
-template <bool FULLDMG, int kSt>
-__device__ inline void PhaseB(const double* state, int r, int b, int ip0, int n, double acc[16])
+template <bool FULLDMG, int kSt, bool WITHINV = true>
+__device__ inline void PhaseB(const double* state, int r, int b, int ip0, int n, double acc[16],
+                              const double (*cellInv)[3] = nullptr)
 {
     using namespace gdm8;
+    constexpr int o = WITHINV ? 0 : -sHa;
 #pragma unroll 1
     for (int i = 0; i < n; i++)
     {
@@ -455,7 +462,7 @@ __device__ inline void PhaseB(const double* state, int r, int b, int ip0, int n,
         double db[3], Nb;
         ShapeAt(b, ip, db, Nb);
         const double* st = state + (i * 8 + r) * kSt;
-        constexpr int kRd = FULLDMG ? 26 : 28;
+        constexpr int kRd = WITHINV ? (FULLDMG ? 26 : 28) : (FULLDMG ? 16 : 20);
         double rec[kRd];
 #pragma unroll
         for (int k = 0; k < kRd; k += 2)
@@ -464,10 +471,15 @@ __device__ inline void PhaseB(const double* state, int r, int b, int ip0, int n,
             rec[k] = v.x;
             rec[k + 1] = v.y;
         }
-        const double inv[3][3] = {{rec[0], rec[1], rec[2]}, {rec[3], rec[4], rec[5]}, {rec[6], rec[7], rec[8]}};
         double gb[3];
-        GradOf(db, inv, gb);
-        const double* ha = rec + sHa;
+        if constexpr (WITHINV)
+        {
+            const double inv[3][3] = {{rec[0], rec[1], rec[2]}, {rec[3], rec[4], rec[5]}, {rec[6], rec[7], rec[8]}};
+            GradOf(db, inv, gb);
+        }
+        else
+            GradOf(db, cellInv, gb);
+        const double* ha = rec + o + sHa;
 #pragma unroll
         for (int ii = 0; ii < 3; ii++)
 #pragma unroll
@@ -475,7 +487,7 @@ __device__ inline void PhaseB(const double* state, int r, int b, int ip0, int n,
                 acc[ii * 4 + c] += ha[ii] * gb[c];
         if (!FULLDMG)
         {
-            const double* ka = rec + sKa;
+            const double* ka = rec + o + sKa;
             const double d = ka[0] * gb[0] + ka[1] * gb[1] + ka[2] * gb[2];
 #pragma unroll
             for (int ii = 0; ii < 3; ii++)
@@ -486,45 +498,47 @@ __device__ inline void PhaseB(const double* state, int r, int b, int ip0, int n,
                 acc[ii * 5] += d;
             }
         }
-        acc[3] += rec[sVa] * Nb;
-        acc[7] += rec[sVa + 1] * Nb;
-        acc[11] += rec[sVa + 2] * Nb;
+        acc[3] += rec[o + sVa] * Nb;
+        acc[7] += rec[o + sVa + 1] * Nb;
+        acc[11] += rec[o + sVa + 2] * Nb;
         double wb[3];
-        BTv3(gb, rec + sMn, wb);
+        BTv3(gb, rec + o + sMn, wb);
         acc[12] += wb[0];
         acc[13] += wb[1];
         acc[14] += wb[2];
-        acc[15] += rec[sSn] * Nb + rec[sCg] * gb[0] + rec[sCg + 1] * gb[1] + rec[sCg + 2] * gb[2];
+        acc[15] += rec[o + sSn] * Nb + rec[o + sCg] * gb[0] + rec[o + sCg + 1] * gb[1] + rec[o + sCg + 2] * gb[2];
     }
 }
 
 // One workgroup per node. Workgroups are dispatched round-robin over the 8 XCDs; with
 // xcdMap the node of block b is chosen so that each XCD processes a contiguous range of nodes
 // (neighbouring nodes share cells and node data, which then stay in that XCD's L2).
-// PRE: the IP states come from kGdmHex8Ip; phase B then runs in two passes of 4 IPs so that only
-// 32 records are in LDS at once (9.3 KB per workgroup: 4 waves per SIMD at <= 128 VGPRs).
-template <bool WR, bool WK, bool FULLDMG, bool PRE>
-__global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, const GdmHex8Tables T)
+// MODE 0: the node kernel evaluates the IP states itself (8 records per cell in LDS).
+// MODE 1: the IP states come from kGdmHex8Ip; phase B runs in two passes of 4 IPs so that only 32
+//         records are in LDS at once (9.3 KB per workgroup: 4 waves per SIMD at <= 128 VGPRs).
+// MODE 2: as 1 for meshes of affine cells (J^-1 equal at the 8 IPs of every cell, e.g. bricks):
+//         one J^-1 per cell, records without it (18 doubles), all 8 IPs in one pass (9.8 KB).
+template <bool FULLDMG, int MODE>
+struct NodeLds
+{
+    static constexpr int kSt = MODE == 2 ? (FULLDMG ? 18 : 22) : gdm8::kStOf<FULLDMG>;
+    static constexpr int kRecIps = MODE == 1 ? 4 : 8;                         // IPs of the records in LDS at a time
+    static constexpr int kRecs = 8 * kRecIps * kSt + (MODE == 2 ? 72 : 0);   // records (+ one J^-1 per cell)
+    static constexpr int kState = kRecs > gdm8::kRed ? kRecs : gdm8::kRed;   // doubles of LDS per workgroup
+};
+
+// The rows of node p (one 64-lane workgroup); `state` is the workgroup's LDS (NodeLds::kState doubles):
+// records [i][r][kSt]; after phase B: partial blocks, R sums, zero block and the slot map.
+template <bool WR, bool WK, bool FULLDMG, int MODE>
+__device__ inline void NodeBody(const RowArgs& A, const GdmHex8Tables& T, int p, double* state)
 {
     using namespace gdm8;
-    constexpr int kSt = kStOf<FULLDMG>;
-    constexpr int kRecIps = PRE ? 4 : 8;  // IPs whose records are in LDS at a time
-    constexpr int kState = 8 * kRecIps * kSt > kRed ? 8 * kRecIps * kSt : kRed;
-    // records [i][r][kSt]; after phase B: partial blocks, R sums, zero block and the slot map
-    __shared__ __align__(16) double state[kState];
+    constexpr bool PRE = MODE >= 1, AFF = MODE == 2;
+    constexpr int kSt = NodeLds<FULLDMG, MODE>::kSt;
 
     const int lane = threadIdx.x;
     const int r = lane >> 3, j = lane & 7;
     const LawParams& P = A.P;
-    int p = blockIdx.x;
-    if (T.xcdMap)
-    {
-        const int per = (int)((gridDim.x + 7) >> 3);
-        p = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
-    }
-    p += T.pBegin;
-    if (p >= T.pEnd)
-        return;
     // p's rows: 3 displacement rows (consecutive in the CSR) and the eeq row, each 4 nA long
     const int ru0 = A.rowPtr[(size_t)p * 3];
     const int L = A.rowPtr[(size_t)p * 3 + 1] - ru0;
@@ -550,7 +564,33 @@ __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, const GdmHex
 #pragma unroll
     for (int i = 0; i < 16; i++)
         acc[i] = 0.0;
-    if constexpr (PRE)
+    if constexpr (AFF)
+    {
+        double rec[kSt];
+        if (valid)
+            PreRecord<FULLDMG, kSt, false>(f.sr, f.g, la, j, P, rec, Rv);
+        if (WK)
+        {
+            double* cellInv = state + 8 * 8 * kSt;  // [r][9]
+            if (valid)
+                StoreRecord<kSt>(state + (j * 8 + r) * kSt, rec);
+            if (valid && j == 0)
+            {
+#pragma unroll
+                for (int k = 0; k < 9; k++)
+                    cellInv[r * 9 + k] = f.g[k];
+            }
+            __syncthreads();
+            if (valid)
+            {
+                const double inv[3][3] = {{cellInv[r * 9 + 0], cellInv[r * 9 + 1], cellInv[r * 9 + 2]},
+                                          {cellInv[r * 9 + 3], cellInv[r * 9 + 4], cellInv[r * 9 + 5]},
+                                          {cellInv[r * 9 + 6], cellInv[r * 9 + 7], cellInv[r * 9 + 8]}};
+                PhaseB<FULLDMG, kSt, false>(state, r, j, 0, 8, acc, inv);
+            }
+        }
+    }
+    else if constexpr (PRE)
     {
         double rec[kSt];
         if (valid)
@@ -745,6 +785,24 @@ __global__ void __launch_bounds__(64) kGdmHex8Node(const RowArgs A, const GdmHex
     }
 }
 
+// MODE >= 1 targets 4 waves per SIMD (<= 128 VGPRs; the LDS of modes 1 and 2 allows 16 workgroups per CU)
+template <bool WR, bool WK, bool FULLDMG, int MODE>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MODE >= 1 ? 4 : 1)))
+kGdmHex8Node(const RowArgs A, const GdmHex8Tables T)
+{
+    __shared__ __align__(16) double state[NodeLds<FULLDMG, MODE>::kState];
+    int p = blockIdx.x;
+    if (T.xcdMap)
+    {
+        const int per = (int)((gridDim.x + 7) >> 3);
+        p = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    }
+    p += T.pBegin;
+    if (p >= T.pEnd)
+        return;
+    NodeBody<WR, WK, FULLDMG, MODE>(A, T, p, state);
+}
+
 // Pre-pass: the state of every (cell, IP) once, for the node kernel (which would otherwise evaluate
 // it in each of the cell's 8 node workgroups). Lane (c, ip) of 8-lane group c; node j of the cell is
 // gathered by lane (c, j) and shared by swizzle, as in the node kernel.
@@ -867,6 +925,29 @@ __global__ void __launch_bounds__(256) kGdmHex8Geometry(const double* x, const d
     g[9] = detJ * tab[ip * kTab];
 }
 
+// Counts the cells that are not parallelepipeds (0 means every cell is affine and the node kernel
+// may keep one J^-1 per cell). For a parallelepiped J is the same at every point of the cell; the
+// J^-1 computed at the 8 IPs then differ only by rounding, so a cell counts as affine when they
+// agree to 1e-13 of its largest entry (IP 0's J^-1 is then used for all IPs: a relative change of
+// order 1e-16 in the results).
+__global__ void __launch_bounds__(256) kGdmHex8CountNonAffine(const double* geo, int nCells, int* count)
+{
+    using namespace gdm8;
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= nCells)
+        return;
+    const double* g0 = geo + (size_t)c * 8 * kGeo;
+    double scale = 0.0;
+    for (int k = 0; k < 9; k++)
+        scale = fmax(scale, fabs(g0[k]));
+    bool same = true;
+    for (int ip = 1; ip < 8; ip++)
+        for (int k = 0; k < 9; k++)
+            same = same && (fabs(g0[ip * kGeo + k] - g0[k]) <= 1e-13 * scale);
+    if (!same)
+        atomicAdd(count, 1);
+}
+
 struct GdmHex8Plan
 {
     bool ready = false;
@@ -876,6 +957,7 @@ struct GdmHex8Plan
     DevBuf nodeCell, nodeHist, nodeLa, nodeSlot, geo;
     DevBuf ipState;      // pre-pass state records (NUTO_AMD_GDM_PREPASS=0: recompute in the node kernel)
     bool prepass = true;
+    bool affine = false;  // every cell affine: MODE 2 of the node kernel
     int nCells = 0;
 
     void Release()
@@ -945,6 +1027,29 @@ struct GdmHex8Plan
         }
         const char* pre = std::getenv("NUTO_AMD_GDM_PREPASS");
         prepass = !(pre && pre[0] == '0');
+        const char* aff = std::getenv("NUTO_AMD_GDM_AFFINE");
+        affine = false;
+        if (prepass && nCells > 0 && !(aff && aff[0] == '0'))
+        {
+            DevBuf cnt;
+            int h = -1;
+            if ((err = cnt.Alloc(sizeof(int))) != hipSuccess ||
+                (err = hipMemsetAsync(cnt.p, 0, sizeof(int), stream)) != hipSuccess)
+            {
+                Release();
+                return false;
+            }
+            hipLaunchKernelGGL(kGdmHex8CountNonAffine, dim3((unsigned)((nCells + 255) / 256)), dim3(256), 0, stream,
+                               geo.As<double>(), nCells, cnt.As<int>());
+            if ((err = hipGetLastError()) != hipSuccess ||
+                (err = hipMemcpyAsync(&h, cnt.p, sizeof(int), hipMemcpyDeviceToHost, stream)) != hipSuccess ||
+                (err = hipStreamSynchronize(stream)) != hipSuccess)
+            {
+                Release();
+                return false;
+            }
+            affine = h == 0;
+        }
         if (prepass && (err = ipState.Alloc((size_t)nCells * 8 * gdm8::kSr * sizeof(double))) != hipSuccess)
         {
             Release();
@@ -966,15 +1071,15 @@ struct GdmHex8Plan
                              ipState.p ? ipState.As<double>() : nullptr, xcdMap, 0, nNodes};
     }
 
-    template <bool FULLDMG, bool PRE>
+    template <bool FULLDMG, int MODE>
     void LaunchNodes(const RowArgs& a, const GdmHex8Tables& t, bool wr, bool wk, int grid_, hipStream_t stream)
     {
         if (wr && wk)
-            hipLaunchKernelGGL((kGdmHex8Node<true, true, FULLDMG, PRE>), dim3(grid_), dim3(64), 0, stream, a, t);
+            hipLaunchKernelGGL((kGdmHex8Node<true, true, FULLDMG, MODE>), dim3(grid_), dim3(64), 0, stream, a, t);
         else if (wr)
-            hipLaunchKernelGGL((kGdmHex8Node<true, false, FULLDMG, PRE>), dim3(grid_), dim3(64), 0, stream, a, t);
+            hipLaunchKernelGGL((kGdmHex8Node<true, false, FULLDMG, MODE>), dim3(grid_), dim3(64), 0, stream, a, t);
         else
-            hipLaunchKernelGGL((kGdmHex8Node<false, true, FULLDMG, PRE>), dim3(grid_), dim3(64), 0, stream, a, t);
+            hipLaunchKernelGGL((kGdmHex8Node<false, true, FULLDMG, MODE>), dim3(grid_), dim3(64), 0, stream, a, t);
     }
 
     template <bool FULLDMG>
@@ -986,10 +1091,13 @@ struct GdmHex8Plan
             if (nCells > 0)
                 hipLaunchKernelGGL((kGdmHex8Ip<FULLDMG>), dim3((unsigned)(((long long)nCells * 8 + 63) / 64)), dim3(64),
                                    0, stream, a, t, ipState.As<double>());
-            LaunchNodes<FULLDMG, true>(a, t, wr, wk, grid, stream);
+            if (affine)
+                LaunchNodes<FULLDMG, 2>(a, t, wr, wk, grid, stream);
+            else
+                LaunchNodes<FULLDMG, 1>(a, t, wr, wk, grid, stream);
         }
         else
-            LaunchNodes<FULLDMG, false>(a, t, wr, wk, grid, stream);
+            LaunchNodes<FULLDMG, 0>(a, t, wr, wk, grid, stream);
     }
 
     void Launch(const RowArgs& a, bool wr, bool wk, hipStream_t stream)
@@ -1013,9 +1121,9 @@ struct GdmHex8Plan
         t.pEnd = end;
         // per-mesh launches recompute the IP state in the node kernel (no pre-pass over the mesh's cells)
         if (a.P.damage_application == 0)
-            LaunchNodes<true, false>(a, t, true, true, end - first, stream);
+            LaunchNodes<true, 0>(a, t, true, true, end - first, stream);
         else
-            LaunchNodes<false, false>(a, t, true, true, end - first, stream);
+            LaunchNodes<false, 0>(a, t, true, true, end - first, stream);
     }
 };
 
