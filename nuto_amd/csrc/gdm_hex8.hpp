@@ -51,7 +51,8 @@ namespace gdm8
 constexpr int kMaxInc = 8;   // incident cells per node
 constexpr int kMaxAdj = 32;  // neighbours per node
 // doubles per (ip, cell) record: 26 for FULL damage application, 30 with the UNILATERAL terms.
-// The 8 records of one IP start 52 (60) dwords apart: 8 disjoint 4-bank groups, 16-byte aligned.
+// Records are 13 (15) 16-byte chunks long: an odd count, so 16 consecutive records start on 16
+// distinct 4-bank groups.
 template <bool FULLDMG>
 constexpr int kStOf = FULLDMG ? 26 : 30;
 constexpr int kGeo = 10;     // doubles per (cell, IP) geometry record: J^-1 (row-major), detJ * w
@@ -283,7 +284,26 @@ struct GdmHex8Tables
     const double* ipState;          // [cell][ip][kSr] pre-pass state records (nullptr: recompute)
     int xcdMap;                     // 1: contiguous node ranges per XCD
     int pBegin, pEnd;               // nodes [pBegin, pEnd) of the launch
+    const int* order;               // traversal position -> node (nullptr: identity)
 };
+
+// Node of this workgroup, -1 past the end. Workgroups are dispatched round-robin over the 8 XCDs;
+// with xcdMap, XCD x takes the x-th contiguous eighth of the traversal, and `order` makes that
+// traversal follow space (nodes sorted along a Morton curve of their coordinates), so the nodes
+// one XCD works on at a time share their cells' records in its L2.
+__device__ inline int NodeOfBlock(const GdmHex8Tables& T)
+{
+    int p = blockIdx.x;
+    if (T.xcdMap)
+    {
+        const int per = (int)((gridDim.x + 7) >> 3);
+        p = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    }
+    p += T.pBegin;
+    if (p >= T.pEnd)
+        return -1;
+    return T.order ? T.order[p] : p;
+}
 
 // Inputs of lane (r, j) of node p.
 struct NodeFetch
@@ -297,7 +317,8 @@ struct NodeFetch
     double kh;  // history kappa of cell r at IP j
 };
 
-template <bool PRE>
+// GEO0: the geometry of IP 0 in every lane of the group (affine cells: one record per cell)
+template <bool PRE, bool GEO0 = false>
 __device__ inline void FetchNode(const RowArgs& A, const GdmHex8Tables& T, int p, int r, int j, NodeFetch& f)
 {
     const size_t pr = (size_t)p * 8 + r;
@@ -308,7 +329,8 @@ __device__ inline void FetchNode(const RowArgs& A, const GdmHex8Tables& T, int p
     f.q = 0;
     if (f.cell >= 0)
     {
-        const double2* gr = reinterpret_cast<const double2*>(T.geo + ((size_t)f.cell * 8 + j) * gdm8::kGeo);
+        const double2* gr =
+            reinterpret_cast<const double2*>(T.geo + ((size_t)f.cell * 8 + (GEO0 ? 0 : j)) * gdm8::kGeo);
 #pragma unroll
         for (int k = 0; k < gdm8::kGeo / 2; k++)
         {
@@ -448,7 +470,9 @@ __device__ inline void StoreRecord(double* dst, const double rec[kSt])
 }
 
 // Phase B: lane (r, b) accumulates block (a_r, b) over IPs ip0 .. ip0 + n - 1 (records at
-// state[(i * 8 + r) * kSt], i = ip - ip0), in IP order.
+// state[(r * n + i) * kSt], i = ip - ip0), in IP order. The records of one cell are adjacent, so
+// the record stores of consecutive lanes (r, i) hit consecutive records: with an odd number of
+// 16-byte chunks per record (kSt / 2 odd) any 16 of them fall on distinct 4-bank groups.
 template <bool FULLDMG, int kSt, bool WITHINV = true>
 __device__ inline void PhaseB(const double* state, int r, int b, int ip0, int n, double acc[16],
                               const double (*cellInv)[3] = nullptr)
@@ -461,7 +485,7 @@ __device__ inline void PhaseB(const double* state, int r, int b, int ip0, int n,
         const int ip = ip0 + i;
         double db[3], Nb;
         ShapeAt(b, ip, db, Nb);
-        const double* st = state + (i * 8 + r) * kSt;
+        const double* st = state + (r * n + i) * kSt;
         constexpr int kRd = WITHINV ? (FULLDMG ? 26 : 28) : (FULLDMG ? 16 : 20);
         double rec[kRd];
 #pragma unroll
@@ -516,14 +540,15 @@ __device__ inline void PhaseB(const double* state, int r, int b, int ip0, int n,
 // MODE 0: the node kernel evaluates the IP states itself (8 records per cell in LDS).
 // MODE 1: the IP states come from kGdmHex8Ip; phase B runs in two passes of 4 IPs so that only 32
 //         records are in LDS at once (9.3 KB per workgroup: 4 waves per SIMD at <= 128 VGPRs).
-// MODE 2: as 1 for meshes of affine cells (J^-1 equal at the 8 IPs of every cell, e.g. bricks):
-//         one J^-1 per cell, records without it (18 doubles), all 8 IPs in one pass (9.8 KB).
+// MODE 2: as 1 for meshes of affine cells (J^-1 and detJ w equal at the 8 IPs of every cell, e.g.
+//         bricks): IP 0's geometry for all IPs; records without J^-1 (18 doubles); all 8 IPs in one
+//         pass (9.3 KB).
 template <bool FULLDMG, int MODE>
 struct NodeLds
 {
     static constexpr int kSt = MODE == 2 ? (FULLDMG ? 18 : 22) : gdm8::kStOf<FULLDMG>;
     static constexpr int kRecIps = MODE == 1 ? 4 : 8;                         // IPs of the records in LDS at a time
-    static constexpr int kRecs = 8 * kRecIps * kSt + (MODE == 2 ? 72 : 0);   // records (+ one J^-1 per cell)
+    static constexpr int kRecs = 8 * kRecIps * kSt;                           // records
     static constexpr int kState = kRecs > gdm8::kRed ? kRecs : gdm8::kRed;   // doubles of LDS per workgroup
 };
 
@@ -545,7 +570,7 @@ __device__ inline void NodeBody(const RowArgs& A, const GdmHex8Tables& T, int p,
     const int nA = L >> 2;
     const int re0 = A.rowPtr[(size_t)A.nU + p];
     NodeFetch f;
-    FetchNode<PRE>(A, T, p, r, j, f);
+    FetchNode<PRE, AFF>(A, T, p, r, j, f);
     const bool valid = f.cell >= 0;  // r < number of cells incident to p
     const int la = f.la;
 
@@ -571,21 +596,13 @@ __device__ inline void NodeBody(const RowArgs& A, const GdmHex8Tables& T, int p,
             PreRecord<FULLDMG, kSt, false>(f.sr, f.g, la, j, P, rec, Rv);
         if (WK)
         {
-            double* cellInv = state + 8 * 8 * kSt;  // [r][9]
             if (valid)
-                StoreRecord<kSt>(state + (j * 8 + r) * kSt, rec);
-            if (valid && j == 0)
-            {
-#pragma unroll
-                for (int k = 0; k < 9; k++)
-                    cellInv[r * 9 + k] = f.g[k];
-            }
+                StoreRecord<kSt>(state + (r * 8 + j) * kSt, rec);
             __syncthreads();
             if (valid)
             {
-                const double inv[3][3] = {{cellInv[r * 9 + 0], cellInv[r * 9 + 1], cellInv[r * 9 + 2]},
-                                          {cellInv[r * 9 + 3], cellInv[r * 9 + 4], cellInv[r * 9 + 5]},
-                                          {cellInv[r * 9 + 6], cellInv[r * 9 + 7], cellInv[r * 9 + 8]}};
+                // lane (r, b) works on cell r, whose J^-1 the group holds
+                const double inv[3][3] = {{f.g[0], f.g[1], f.g[2]}, {f.g[3], f.g[4], f.g[5]}, {f.g[6], f.g[7], f.g[8]}};
                 PhaseB<FULLDMG, kSt, false>(state, r, j, 0, 8, acc, inv);
             }
         }
@@ -599,7 +616,7 @@ __device__ inline void NodeBody(const RowArgs& A, const GdmHex8Tables& T, int p,
         {
             // pass 0: IPs 0..3 (records of lanes j < 4)
             if (valid && j < 4)
-                StoreRecord<kSt>(state + (j * 8 + r) * kSt, rec);
+                StoreRecord<kSt>(state + (r * 4 + j) * kSt, rec);
             __syncthreads();
             if (valid)
                 PhaseB<FULLDMG, kSt>(state, r, j, 0, 4, acc);
@@ -610,7 +627,7 @@ __device__ inline void NodeBody(const RowArgs& A, const GdmHex8Tables& T, int p,
                 double g2[kGeo], sr2[kSr], Rd[4];
                 LoadIpRecords(T, f.cell, j, g2, sr2);
                 PreRecord<FULLDMG, kSt>(sr2, g2, la, j, P, rec, Rd);
-                StoreRecord<kSt>(state + ((j - 4) * 8 + r) * kSt, rec);
+                StoreRecord<kSt>(state + (r * 4 + j - 4) * kSt, rec);
             }
             __syncthreads();
             if (valid)
@@ -649,7 +666,7 @@ __device__ inline void NodeBody(const RowArgs& A, const GdmHex8Tables& T, int p,
             {
                 double m[6];
                 GdmMisesDerivative(st, P, m);
-                double* rp = state + (ip * 8 + r) * kSt;
+                double* rp = state + (r * 8 + ip) * kSt;
                 double rec[kSt];
 #pragma unroll
                 for (int a = 0; a < 3; a++)
@@ -791,14 +808,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MODE >=
 kGdmHex8Node(const RowArgs A, const GdmHex8Tables T)
 {
     __shared__ __align__(16) double state[NodeLds<FULLDMG, MODE>::kState];
-    int p = blockIdx.x;
-    if (T.xcdMap)
-    {
-        const int per = (int)((gridDim.x + 7) >> 3);
-        p = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
-    }
-    p += T.pBegin;
-    if (p >= T.pEnd)
+    const int p = NodeOfBlock(T);
+    if (p < 0)
         return;
     NodeBody<WR, WK, FULLDMG, MODE>(A, T, p, state);
 }
@@ -928,8 +939,8 @@ __global__ void __launch_bounds__(256) kGdmHex8Geometry(const double* x, const d
 // Counts the cells that are not parallelepipeds (0 means every cell is affine and the node kernel
 // may keep one J^-1 per cell). For a parallelepiped J is the same at every point of the cell; the
 // J^-1 computed at the 8 IPs then differ only by rounding, so a cell counts as affine when they
-// agree to 1e-13 of its largest entry (IP 0's J^-1 is then used for all IPs: a relative change of
-// order 1e-16 in the results).
+// agree to 1e-13 of its largest entry and detJ w to 1e-13 of IP 0's (equal weights, as in 2x2x2
+// Gauss). IP 0's geometry is then used for all IPs: a relative change of order 1e-16 in the results.
 __global__ void __launch_bounds__(256) kGdmHex8CountNonAffine(const double* geo, int nCells, int* count)
 {
     using namespace gdm8;
@@ -942,10 +953,77 @@ __global__ void __launch_bounds__(256) kGdmHex8CountNonAffine(const double* geo,
         scale = fmax(scale, fabs(g0[k]));
     bool same = true;
     for (int ip = 1; ip < 8; ip++)
+    {
         for (int k = 0; k < 9; k++)
             same = same && (fabs(g0[ip * kGeo + k] - g0[k]) <= 1e-13 * scale);
+        same = same && (fabs(g0[ip * kGeo + 9] - g0[9]) <= 1e-13 * fabs(g0[9]));
+    }
     if (!same)
         atomicAdd(count, 1);
+}
+
+// Nodes sorted along a Morton (Z-order) curve of their coordinates: each node takes the
+// coordinates of its first incident cell's coordinate node (hex8 dof nodes are the coordinate
+// nodes), quantised to 2^10 steps per axis over the bounding box; ties keep node order.
+inline hipError_t MortonOrder(const NodeGraph& g, int nCells, int nCoord, const double* x, const double* y,
+                              const double* z, const int* cconn, hipStream_t stream, std::vector<int>& order)
+{
+    std::vector<double> hx(nCoord), hy(nCoord), hz(nCoord);
+    std::vector<int> hc((size_t)nCells * 8);
+    hipError_t err;
+    if ((err = hipMemcpyAsync(hx.data(), x, nCoord * sizeof(double), hipMemcpyDeviceToHost, stream)) != hipSuccess ||
+        (err = hipMemcpyAsync(hy.data(), y, nCoord * sizeof(double), hipMemcpyDeviceToHost, stream)) != hipSuccess ||
+        (err = hipMemcpyAsync(hz.data(), z, nCoord * sizeof(double), hipMemcpyDeviceToHost, stream)) != hipSuccess ||
+        (err = hipMemcpyAsync(hc.data(), cconn, hc.size() * sizeof(int), hipMemcpyDeviceToHost, stream)) !=
+            hipSuccess ||
+        (err = hipStreamSynchronize(stream)) != hipSuccess)
+        return err;
+    const int n = g.nNodes;
+    std::vector<int> cn(n, -1);
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int p = 0; p < n; p++)
+    {
+        if (g.incPtr[p + 1] == g.incPtr[p])
+            continue;
+        const int k = g.incPtr[p];
+        cn[p] = hc[(size_t)g.incCell[k] * 8 + g.incLocal[k]];
+        const double v[3] = {hx[cn[p]], hy[cn[p]], hz[cn[p]]};
+        for (int d = 0; d < 3; d++)
+        {
+            lo[d] = std::min(lo[d], v[d]);
+            hi[d] = std::max(hi[d], v[d]);
+        }
+    }
+    auto spread = [](unsigned long long v) {  // 10 bits -> every third of 30
+        v &= 0x3ff;
+        v = (v | (v << 16)) & 0x30000ffULL;
+        v = (v | (v << 8)) & 0x300f00fULL;
+        v = (v | (v << 4)) & 0x30c30c3ULL;
+        v = (v | (v << 2)) & 0x9249249ULL;
+        return v;
+    };
+    std::vector<unsigned long long> key(n);
+    for (int p = 0; p < n; p++)
+    {
+        unsigned long long m = 0;
+        if (cn[p] >= 0)
+        {
+            const double v[3] = {hx[cn[p]], hy[cn[p]], hz[cn[p]]};
+            for (int d = 0; d < 3; d++)
+            {
+                const double w = hi[d] - lo[d];
+                const double t = w > 0.0 ? (v[d] - lo[d]) / w : 0.0;
+                const unsigned long long q = (unsigned long long)std::min(1023.0, std::max(0.0, t * 1024.0));
+                m |= spread(q) << d;
+            }
+        }
+        key[p] = (m << 32) | (unsigned long long)p;
+    }
+    std::sort(key.begin(), key.end());
+    order.resize(n);
+    for (int i = 0; i < n; i++)
+        order[i] = (int)(key[i] & 0xffffffffULL);
+    return hipSuccess;
 }
 
 struct GdmHex8Plan
@@ -955,6 +1033,7 @@ struct GdmHex8Plan
     int grid = 0;     // workgroups: one per node (rounded up to a multiple of 8 with the XCD mapping)
     int xcdMap = 1;   // contiguous node ranges per XCD (env NUTO_AMD_GDM_XCD_MAP=0 switches it off)
     DevBuf nodeCell, nodeHist, nodeLa, nodeSlot, geo;
+    DevBuf order;        // Morton traversal order (NUTO_AMD_GDM_ORDER=0: node order)
     DevBuf ipState;      // pre-pass state records (NUTO_AMD_GDM_PREPASS=0: recompute in the node kernel)
     bool prepass = true;
     bool affine = false;  // every cell affine: MODE 2 of the node kernel
@@ -964,7 +1043,7 @@ struct GdmHex8Plan
     {
         ready = false;
         nNodes = 0;
-        for (DevBuf* b : {&nodeCell, &nodeHist, &nodeLa, &nodeSlot, &geo, &ipState})
+        for (DevBuf* b : {&nodeCell, &nodeHist, &nodeLa, &nodeSlot, &geo, &ipState, &order})
             b->Release();
     }
 
@@ -977,7 +1056,7 @@ struct GdmHex8Plan
     // coordinates (x, y, z by coordinate node, cconn) and the IP table. Returns false (plan not
     // used) when the mesh exceeds the kernel's bounds; a HIP failure is returned through `err`.
     bool Prepare(int kind, int coordType, int dofType, int nip, const NodeGraph& g, int nCells, const int* cellIds,
-                 const double* x, const double* y, const double* z, const int* cconn, const double* tab,
+                 int nCoord, const double* x, const double* y, const double* z, const int* cconn, const double* tab,
                  hipStream_t stream, hipError_t& err)
     {
         Release();
@@ -1055,6 +1134,17 @@ struct GdmHex8Plan
             Release();
             return false;
         }
+        const char* ord = std::getenv("NUTO_AMD_GDM_ORDER");
+        if (n > 0 && nCells > 0 && !(ord && ord[0] == '0'))
+        {
+            std::vector<int> o;
+            if ((err = MortonOrder(g, nCells, nCoord, x, y, z, cconn, stream, o)) != hipSuccess ||
+                (err = order.Upload(o.data(), o.size() * sizeof(int), stream)) != hipSuccess)
+            {
+                Release();
+                return false;
+            }
+        }
         this->nCells = nCells;
         nNodes = n;
         const char* env = std::getenv("NUTO_AMD_GDM_XCD_MAP");
@@ -1068,7 +1158,8 @@ struct GdmHex8Plan
     {
         return GdmHex8Tables{nodeCell.As<int>(), nodeHist.p ? nodeHist.As<int>() : nullptr,
                              nodeLa.As<unsigned char>(), nodeSlot.As<unsigned char>(), geo.As<double>(),
-                             ipState.p ? ipState.As<double>() : nullptr, xcdMap, 0, nNodes};
+                             ipState.p ? ipState.As<double>() : nullptr, xcdMap, 0, nNodes,
+                             order.p ? order.As<int>() : nullptr};
     }
 
     template <bool FULLDMG, int MODE>
@@ -1119,6 +1210,7 @@ struct GdmHex8Plan
         t.xcdMap = 0;
         t.pBegin = first;
         t.pEnd = end;
+        t.order = nullptr;
         // per-mesh launches recompute the IP state in the node kernel (no pre-pass over the mesh's cells)
         if (a.P.damage_application == 0)
             LaunchNodes<true, 0>(a, t, true, true, end - first, stream);

@@ -39,14 +39,8 @@ __global__ void __launch_bounds__(64) kGdmHex8Vjp(const RowArgs A, const GdmHex8
     const int lane = threadIdx.x;
     const int r = lane >> 3, j = lane & 7;
     const LawParams& P = A.P;
-    int p = blockIdx.x;
-    if (T.xcdMap)
-    {
-        const int per = (int)((gridDim.x + 7) >> 3);
-        p = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
-    }
-    p += T.pBegin;
-    if (p >= T.pEnd)
+    const int p = NodeOfBlock(T);
+    if (p < 0)
         return;
     NodeFetch f;
     FetchNode<false>(A, T, p, r, j, f);
