@@ -233,8 +233,8 @@ def cpu_baseline(n, threads, budget_s=30.0):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=100, help="cells per direction (100 = the headline workload)")
     ap.add_argument("--cpu-sample", type=int, default=24, help="cells per direction of the CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0, help="OpenMP threads for the CPU baseline (0: auto)")

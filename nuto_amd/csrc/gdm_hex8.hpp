@@ -112,6 +112,31 @@ __device__ inline void ShapeAt(int n, int ip, double d[3], double& N)
 }
 
 // g = J^-T d  (dN/dX = dN/dxi J^-1, Jacobian.h:100-103)
+// Shape data of a hex8 node at the 2x2x2 Gauss point whose corner differs from the node's in the
+// axes of the set bits of t: f_k = 1 - g if bit k of t is set, else 1 + g (g = sqrt(1/3)), and
+// [|dN/dxi_0|, |dN/dxi_1|, |dN/dxi_2|, N] = [f1 f2, f0 f2, f0 f1, f0 f1 f2] / 8, with the products
+// taken in ShapeAt's order (so the values are bitwise ShapeAt's, up to the gradient signs).
+struct ShapeRelTable
+{
+    double v[8][4];
+};
+constexpr ShapeRelTable MakeShapeRel()
+{
+    ShapeRelTable t{};
+    for (int k = 0; k < 8; k++)
+    {
+        const double f0 = (k & 1) ? 1.0 - kGauss : 1.0 + kGauss;
+        const double f1 = (k & 2) ? 1.0 - kGauss : 1.0 + kGauss;
+        const double f2 = (k & 4) ? 1.0 - kGauss : 1.0 + kGauss;
+        t.v[k][0] = 0.125 * f1 * f2;
+        t.v[k][1] = 0.125 * f0 * f2;
+        t.v[k][2] = 0.125 * f0 * f1;
+        t.v[k][3] = 0.125 * f0 * f1 * f2;
+    }
+    return t;
+}
+__constant__ ShapeRelTable kShapeRel = MakeShapeRel();
+
 __device__ inline void GradOf(const double d[3], const double inv[3][3], double g[3])
 {
 #pragma unroll
@@ -369,23 +394,25 @@ __device__ inline void FetchNode(const RowArgs& A, const GdmHex8Tables& T, int p
     }
 }
 
-// value of lane i ^ k within each group of 8 lanes (ds_swizzle bit mode: and-mask 0x1F, xor-mask k)
-template <int k>
-__device__ inline double GroupXor(double v)
+// sum over the 8 lanes of a group, ((t0 + t1) + (t2 + t3)) + ((t4 + t5) + (t6 + t7)) in every lane
+// (each butterfly step adds the same two values in every lane, so all lanes hold the same bits)
+// v of lane i + dpp-selected lane (CTRL: a DPP control word; out-of-row reads give 0)
+template <int CTRL>
+__device__ inline double DppMov(double v)
 {
     const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_ds_swizzle((int)(b & 0xffffffffLL), 0x1F | (k << 10));
-    const int hi = __builtin_amdgcn_ds_swizzle((int)(b >> 32), 0x1F | (k << 10));
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, true);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// sum over the 8 lanes of a group, ((t0 + t1) + (t2 + t3)) + ((t4 + t5) + (t6 + t7)) in every lane
-// (each butterfly step adds the same two values in every lane, so all lanes hold the same bits)
-__device__ inline double GroupSum(double v)
+// sum over the 8 lanes of a group, ((t0 + t1) + (t2 + t3)) + ((t4 + t5) + (t6 + t7)), valid in
+// the group's lane 0 (DPP: no LDS instruction)
+__device__ inline double GroupSum0(double v)
 {
-    v += GroupXor<1>(v);
-    v += GroupXor<2>(v);
-    v += GroupXor<4>(v);
+    v += DppMov<0xB1>(v);   // quad_perm [1, 0, 3, 2]: lane i ^ 1
+    v += DppMov<0x4E>(v);   // quad_perm [2, 3, 0, 1]: lane i ^ 2
+    v += DppMov<0x104>(v);  // row_shl:4: lane i + 4
     return v;
 }
 
@@ -469,16 +496,52 @@ __device__ inline void StoreRecord(double* dst, const double rec[kSt])
         *reinterpret_cast<double2*>(dst + k) = make_double2(rec[k], rec[k + 1]);
 }
 
+// Adds one IP's contribution to block (a, b): h = the record's terms after J^-1 (c1 g_a, B_a^T t,
+// -N_a m, c g_a, N_a, [c2 g_a]), gb = g_b, Nb = N_b.
+template <bool FULLDMG>
+__device__ inline void AccumulateIp(const double* h, const double gb[3], double Nb, double acc[16])
+{
+    using namespace gdm8;
+    const double* ha = h;
+#pragma unroll
+    for (int ii = 0; ii < 3; ii++)
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            acc[ii * 4 + c] += ha[ii] * gb[c];
+    if (!FULLDMG)
+    {
+        const double* ka = h + (sKa - sHa);
+        const double d = ka[0] * gb[0] + ka[1] * gb[1] + ka[2] * gb[2];
+#pragma unroll
+        for (int ii = 0; ii < 3; ii++)
+        {
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                acc[ii * 4 + c] += gb[ii] * ka[c];
+            acc[ii * 5] += d;
+        }
+    }
+    const double* va = h + (sVa - sHa);
+    acc[3] += va[0] * Nb;
+    acc[7] += va[1] * Nb;
+    acc[11] += va[2] * Nb;
+    double wb[3];
+    BTv3(gb, h + (sMn - sHa), wb);
+    acc[12] += wb[0];
+    acc[13] += wb[1];
+    acc[14] += wb[2];
+    const double* cg = h + (sCg - sHa);
+    acc[15] += h[sSn - sHa] * Nb + cg[0] * gb[0] + cg[1] * gb[1] + cg[2] * gb[2];
+}
+
 // Phase B: lane (r, b) accumulates block (a_r, b) over IPs ip0 .. ip0 + n - 1 (records at
 // state[(r * n + i) * kSt], i = ip - ip0), in IP order. The records of one cell are adjacent, so
 // the record stores of consecutive lanes (r, i) hit consecutive records: with an odd number of
 // 16-byte chunks per record (kSt / 2 odd) any 16 of them fall on distinct 4-bank groups.
-template <bool FULLDMG, int kSt, bool WITHINV = true>
-__device__ inline void PhaseB(const double* state, int r, int b, int ip0, int n, double acc[16],
-                              const double (*cellInv)[3] = nullptr)
+template <bool FULLDMG, int kSt>
+__device__ inline void PhaseB(const double* state, int r, int b, int ip0, int n, double acc[16])
 {
     using namespace gdm8;
-    constexpr int o = WITHINV ? 0 : -sHa;
 #pragma unroll 1
     for (int i = 0; i < n; i++)
     {
@@ -486,7 +549,48 @@ __device__ inline void PhaseB(const double* state, int r, int b, int ip0, int n,
         double db[3], Nb;
         ShapeAt(b, ip, db, Nb);
         const double* st = state + (r * n + i) * kSt;
-        constexpr int kRd = WITHINV ? (FULLDMG ? 26 : 28) : (FULLDMG ? 16 : 20);
+        constexpr int kRd = FULLDMG ? 26 : 28;
+        double rec[kRd];
+#pragma unroll
+        for (int k = 0; k < kRd; k += 2)
+        {
+            const double2 v = *reinterpret_cast<const double2*>(st + k);
+            rec[k] = v.x;
+            rec[k + 1] = v.y;
+        }
+        const double inv[3][3] = {{rec[0], rec[1], rec[2]}, {rec[3], rec[4], rec[5]}, {rec[6], rec[7], rec[8]}};
+        double gb[3];
+        GradOf(db, inv, gb);
+        AccumulateIp<FULLDMG>(rec + sHa, gb, Nb, acc);
+    }
+}
+
+// Phase B of affine cells (MODE 2): lane (r, b) visits the IPs of cell r in the order
+// ip = t ^ m(b), m(b) = b's corner (bit k set: + side along axis k). Node b's shape data at that
+// IP depends on t only (kShapeRel, uniform over the wave); the signs of b's gradient are folded
+// into J^-1 once, so g_b is bitwise GradOf's. The 8 lanes of a cell read 8 different records:
+// 16 consecutive lanes read 16 consecutive records (distinct 4-bank groups).
+template <bool FULLDMG, int kSt>
+__device__ inline void PhaseBAffine(const double* state, int r, int b, const double inv[3][3], double acc[16])
+{
+    using namespace gdm8;
+    const int sx = ((b + 1) >> 1) & 1, sy = (b >> 1) & 1, sz = b >> 2;
+    const int m = sx | (sy << 1) | (sz << 2);
+    double sinv[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+    {
+        sinv[0][c] = sx ? inv[0][c] : -inv[0][c];
+        sinv[1][c] = sy ? inv[1][c] : -inv[1][c];
+        sinv[2][c] = sz ? inv[2][c] : -inv[2][c];
+    }
+    const double* cellRec = state + r * 8 * kSt;
+#pragma unroll 1
+    for (int t = 0; t < 8; t++)
+    {
+        const double d0 = kShapeRel.v[t][0], d1 = kShapeRel.v[t][1], d2 = kShapeRel.v[t][2], Nb = kShapeRel.v[t][3];
+        const double* st = cellRec + (t ^ m) * kSt;
+        constexpr int kRd = FULLDMG ? 16 : 20;
         double rec[kRd];
 #pragma unroll
         for (int k = 0; k < kRd; k += 2)
@@ -496,41 +600,10 @@ __device__ inline void PhaseB(const double* state, int r, int b, int ip0, int n,
             rec[k + 1] = v.y;
         }
         double gb[3];
-        if constexpr (WITHINV)
-        {
-            const double inv[3][3] = {{rec[0], rec[1], rec[2]}, {rec[3], rec[4], rec[5]}, {rec[6], rec[7], rec[8]}};
-            GradOf(db, inv, gb);
-        }
-        else
-            GradOf(db, cellInv, gb);
-        const double* ha = rec + o + sHa;
 #pragma unroll
-        for (int ii = 0; ii < 3; ii++)
-#pragma unroll
-            for (int c = 0; c < 3; c++)
-                acc[ii * 4 + c] += ha[ii] * gb[c];
-        if (!FULLDMG)
-        {
-            const double* ka = rec + o + sKa;
-            const double d = ka[0] * gb[0] + ka[1] * gb[1] + ka[2] * gb[2];
-#pragma unroll
-            for (int ii = 0; ii < 3; ii++)
-            {
-#pragma unroll
-                for (int c = 0; c < 3; c++)
-                    acc[ii * 4 + c] += gb[ii] * ka[c];
-                acc[ii * 5] += d;
-            }
-        }
-        acc[3] += rec[o + sVa] * Nb;
-        acc[7] += rec[o + sVa + 1] * Nb;
-        acc[11] += rec[o + sVa + 2] * Nb;
-        double wb[3];
-        BTv3(gb, rec + o + sMn, wb);
-        acc[12] += wb[0];
-        acc[13] += wb[1];
-        acc[14] += wb[2];
-        acc[15] += rec[o + sSn] * Nb + rec[o + sCg] * gb[0] + rec[o + sCg + 1] * gb[1] + rec[o + sCg + 2] * gb[2];
+        for (int c = 0; c < 3; c++)
+            gb[c] = d0 * sinv[0][c] + d1 * sinv[1][c] + d2 * sinv[2][c];
+        AccumulateIp<FULLDMG>(rec, gb, Nb, acc);
     }
 }
 
@@ -603,7 +676,7 @@ __device__ inline void NodeBody(const RowArgs& A, const GdmHex8Tables& T, int p,
             {
                 // lane (r, b) works on cell r, whose J^-1 the group holds
                 const double inv[3][3] = {{f.g[0], f.g[1], f.g[2]}, {f.g[3], f.g[4], f.g[5]}, {f.g[6], f.g[7], f.g[8]}};
-                PhaseB<FULLDMG, kSt, false>(state, r, j, 0, 8, acc, inv);
+                PhaseBAffine<FULLDMG, kSt>(state, r, j, inv, acc);
             }
         }
     }
@@ -726,7 +799,7 @@ __device__ inline void NodeBody(const RowArgs& A, const GdmHex8Tables& T, int p,
     {
 #pragma unroll
         for (int k = 0; k < 4; k++)
-            Rv[k] = GroupSum(Rv[k]);
+            Rv[k] = GroupSum0(Rv[k]);  // stored by lane j == 0
     }
     __syncthreads();
 
