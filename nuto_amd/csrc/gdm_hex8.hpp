@@ -307,6 +307,7 @@ struct GdmHex8Tables
     const unsigned char* nodeSlot;  // [node][8][8]: neighbour slot (column block) of local node j
     const double* geo;              // [cell][ip][kGeo] geometry records
     const double* ipState;          // [cell][ip][kSr] pre-pass state records (nullptr: recompute)
+    const double* geoCell;          // [cell][kGeo] IP 0's geometry record (affine meshes, else nullptr)
     int xcdMap;                     // 1: contiguous node ranges per XCD
     int pBegin, pEnd;               // nodes [pBegin, pEnd) of the launch
     const int* order;               // traversal position -> node (nullptr: identity)
@@ -342,7 +343,7 @@ struct NodeFetch
     double kh;  // history kappa of cell r at IP j
 };
 
-// GEO0: the geometry of IP 0 in every lane of the group (affine cells: one record per cell)
+// GEO0: the cell's geometry record (IP 0's) in every lane of the group (affine cells)
 template <bool PRE, bool GEO0 = false>
 __device__ inline void FetchNode(const RowArgs& A, const GdmHex8Tables& T, int p, int r, int j, NodeFetch& f)
 {
@@ -354,8 +355,8 @@ __device__ inline void FetchNode(const RowArgs& A, const GdmHex8Tables& T, int p
     f.q = 0;
     if (f.cell >= 0)
     {
-        const double2* gr =
-            reinterpret_cast<const double2*>(T.geo + ((size_t)f.cell * 8 + (GEO0 ? 0 : j)) * gdm8::kGeo);
+        const double2* gr = reinterpret_cast<const double2*>(
+            GEO0 ? T.geoCell + (size_t)f.cell * gdm8::kGeo : T.geo + ((size_t)f.cell * 8 + j) * gdm8::kGeo);
 #pragma unroll
         for (int k = 0; k < gdm8::kGeo / 2; k++)
         {
@@ -394,8 +395,6 @@ __device__ inline void FetchNode(const RowArgs& A, const GdmHex8Tables& T, int p
     }
 }
 
-// sum over the 8 lanes of a group, ((t0 + t1) + (t2 + t3)) + ((t4 + t5) + (t6 + t7)) in every lane
-// (each butterfly step adds the same two values in every lane, so all lanes hold the same bits)
 // v of lane i + dpp-selected lane (CTRL: a DPP control word; out-of-row reads give 0)
 template <int CTRL>
 __device__ inline double DppMov(double v)
@@ -792,7 +791,12 @@ __device__ inline void NodeBody(const RowArgs& A, const GdmHex8Tables& T, int p,
         for (int i = 0; i < 3; i++)
 #pragma unroll
             for (int c = 0; c < 3; c++)
-                acc[i * 4 + c] = lam * S[i][c] + mu * S[c][i] + (i == c ? mu * tr : 0.0);
+            {
+                if (i == c)
+                    acc[i * 4 + c] = lam * S[i][c] + mu * S[c][i] + mu * tr;
+                else
+                    acc[i * 4 + c] = lam * S[i][c] + mu * S[c][i];
+            }
     }
     // R: per cell over its IPs (butterfly within the cell's 8 lanes), then over the cells in order
     if (WR)
@@ -838,7 +842,7 @@ __device__ inline void NodeBody(const RowArgs& A, const GdmHex8Tables& T, int p,
         if (q < nA)
         {
             const unsigned long long row = *reinterpret_cast<const unsigned long long*>(&slotOf[q][0]);
-            double sum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            double sum[8];
 #pragma unroll
             for (int rr = 0; rr < 8; rr++)
             {
@@ -846,7 +850,7 @@ __device__ inline void NodeBody(const RowArgs& A, const GdmHex8Tables& T, int p,
                 const double* src = b == 0xFFu ? zero : part + (rr * 8 + (int)b) * kPs + h * 8;
 #pragma unroll
                 for (int k = 0; k < 8; k++)
-                    sum[k] += src[k];
+                    sum[k] = rr == 0 ? src[k] : sum[k] + src[k];
             }
             // block entry (i, c): i = row component (0..2 displacement rows, 3 eeq row),
             // c = column component (0..2 displacement columns, 3 eeq column)
@@ -890,21 +894,38 @@ kGdmHex8Node(const RowArgs A, const GdmHex8Tables T)
 // Pre-pass: the state of every (cell, IP) once, for the node kernel (which would otherwise evaluate
 // it in each of the cell's 8 node workgroups). Lane (c, ip) of 8-lane group c; node j of the cell is
 // gathered by lane (c, j) and shared by swizzle, as in the node kernel.
-template <bool FULLDMG>
+// AFF: affine cells, IP 0's geometry (T.geoCell) at every IP.
+// Workgroup w handles cells 8w .. 8w + 7 (lane (r, j): cell 8w + r, IP j). Their geometry and their
+// records are contiguous blocks in memory; both pass through LDS so that every global load and
+// store of them is 16 contiguous bytes per lane across the wave (the address and data paths, not
+// the arithmetic, bound this kernel).
+template <bool FULLDMG, bool AFF>
 __global__ void __launch_bounds__(64) kGdmHex8Ip(const RowArgs A, const GdmHex8Tables T, double* __restrict__ out)
 {
     using namespace gdm8;
-    const long long t = (long long)blockIdx.x * 64 + threadIdx.x;
-    const int j = (int)(t & 7);
-    const long long cl = t >> 3;
-    const bool valid = cl < A.nCells;  // uniform within each 8-lane group
-    const int cell = valid ? (int)cl : 0;
+    __shared__ __align__(16) double stage[64 * kSr];  // the block's geometry, then its records
+    const int lane = threadIdx.x;
+    const int j = lane & 7, r = lane >> 3;
+    const long long cell0 = (long long)blockIdx.x * 8;
+    const int nc = (int)(A.nCells - cell0 < 8 ? A.nCells - cell0 : 8);  // cells of this block, >= 1
+    const bool valid = r < nc;  // uniform within each 8-lane group
+    const int cell = valid ? (int)(cell0 + r) : 0;
     const LawParams& P = A.P;
+    {
+        constexpr int kPerCell = AFF ? kGeo / 2 : 8 * kGeo / 2;  // 16-byte chunks of geometry per cell
+        const double2* src =
+            reinterpret_cast<const double2*>(AFF ? T.geoCell + cell0 * kGeo : T.geo + cell0 * 8 * kGeo);
+        double2* dst = reinterpret_cast<double2*>(stage);
+#pragma unroll
+        for (int i = 0; i < (8 * kPerCell + 63) / 64; i++)
+        {
+            const int c = lane + 64 * i;
+            if (c < nc * kPerCell)
+                dst[c] = src[c];
+        }
+    }
     double v[4] = {0.0, 0.0, 0.0, 0.0}, g[kGeo];
     double kh = 0.0, E = P.E, k0 = P.kappa0, beta = P.beta;
-#pragma unroll
-    for (int k = 0; k < kGeo; k++)
-        g[k] = 0.0;
     if (valid)
     {
         const int q = A.dconn[(size_t)cell * 8 + j];
@@ -914,14 +935,6 @@ __global__ void __launch_bounds__(64) kGdmHex8Ip(const RowArgs A, const GdmHex8T
         v[3] = A.e[q];
         const int hist = A.cellId ? A.cellId[cell] : cell;
         kh = A.kappa[(size_t)hist + (size_t)j * A.kappaLd];
-        const double2* gr = reinterpret_cast<const double2*>(T.geo + ((size_t)cell * 8 + j) * kGeo);
-#pragma unroll
-        for (int k = 0; k < kGeo / 2; k++)
-        {
-            const double2 w = gr[k];
-            g[2 * k] = w.x;
-            g[2 * k + 1] = w.y;
-        }
         if (A.thE)
         {
             E = A.thE[cell];
@@ -929,48 +942,72 @@ __global__ void __launch_bounds__(64) kGdmHex8Ip(const RowArgs A, const GdmHex8T
             beta = A.thFt[cell] / A.thGf[cell];
         }
     }
+    __syncthreads();
+    {
+        const double2* gs = reinterpret_cast<const double2*>(stage + (AFF ? r : r * 8 + j) * kGeo);
+#pragma unroll
+        for (int k = 0; k < kGeo / 2; k++)
+        {
+            const double2 w = gs[k];
+            g[2 * k] = w.x;
+            g[2 * k + 1] = w.y;
+        }
+    }
     double Hu[3][3], he[3], eN;
     GatherGradients(v, j, Hu, he, eN);
-    if (!valid)
-        return;
-    const double inv[3][3] = {{g[0], g[1], g[2]}, {g[3], g[4], g[5]}, {g[6], g[7], g[8]}};
-    const double s = g[9];
-    const double m3K = P.cK * E, m2G = P.cG * E;
-    GdmIp st;
-    GdmIpState<FULLDMG>(Hu, he, eN, inv, kh, m3K, m2G, k0, beta, P, st);
-    const double eV = st.I1 * kThird;
-    const double sV = (1.0 - st.om * st.H) * m3K * eV;
-    double am = P.K1, bm = 0.0;  // m = am [1,1,1,0,0,0] + bm e'' (A == 0: m = K1 [1,1,1,0,0,0])
-    if (st.Am != 0.0)
+    __syncthreads();  // the records overwrite the geometry
+    if (valid)
     {
-        const double h = 0.5 / st.Am;
-        am = P.K1 + h * (2.0 * P.K1 * P.K1 * st.I1) - h * P.K2 * st.I1 * kThird;
-        bm = h * P.K2;
+        const double inv[3][3] = {{g[0], g[1], g[2]}, {g[3], g[4], g[5]}, {g[6], g[7], g[8]}};
+        const double s = g[9];
+        const double m3K = P.cK * E, m2G = P.cG * E;
+        GdmIp st;
+        GdmIpState<FULLDMG>(Hu, he, eN, inv, kh, m3K, m2G, k0, beta, P, st);
+        const double eV = st.I1 * kThird;
+        const double sV = (1.0 - st.om * st.H) * m3K * eV;
+        double am = P.K1, bm = 0.0;  // m = am [1,1,1,0,0,0] + bm e'' (A == 0: m = K1 [1,1,1,0,0,0])
+        if (st.Am != 0.0)
+        {
+            const double h = 0.5 / st.Am;
+            am = P.K1 + h * (2.0 * P.K1 * P.K1 * st.I1) - h * P.K2 * st.I1 * kThird;
+            bm = h * P.K2;
+        }
+        const double ct = s * st.dom * st.dk;
+        double rec[kSr];
+        rec[rEps] = st.eps[0];
+        rec[rEps + 1] = st.eps[1];
+        rec[rEps + 2] = st.eps[2];
+        rec[rEps + 3] = 0.5 * st.eps[3];
+        rec[rEps + 4] = 0.5 * st.eps[4];
+        rec[rEps + 5] = 0.5 * st.eps[5];
+        rec[rSig] = s * (sV - st.d2G * eV);
+        rec[rSig + 1] = s * st.d2G;
+        rec[rT] = -ct * (m3K * st.H - m2G) * eV;
+        rec[rT + 1] = -ct * m2G;
+        rec[rM] = s * am;
+        rec[rM + 1] = s * bm;
+        rec[rC1] = FULLDMG ? s * (1.0 - st.om) : s * (m3K * (1.0 - st.om * st.H) - st.d2G) * kThird;
+        rec[rC2] = FULLDMG ? 0.0 : 0.5 * s * st.d2G;
+        rec[rRe] = s * (st.eN - st.eq);
+        rec[rGe] = s * P.c * st.ge[0];
+        rec[rGe + 1] = s * P.c * st.ge[1];
+        rec[rGe + 2] = s * P.c * st.ge[2];
+        // record (r, j) at row r * 8 + j: 9 chunks (odd), consecutive lanes on distinct bank groups
+        StoreRecord<kSr>(stage + lane * kSr, rec);
     }
-    const double ct = s * st.dom * st.dk;
-    double rec[kSr];
-    rec[rEps] = st.eps[0];
-    rec[rEps + 1] = st.eps[1];
-    rec[rEps + 2] = st.eps[2];
-    rec[rEps + 3] = 0.5 * st.eps[3];
-    rec[rEps + 4] = 0.5 * st.eps[4];
-    rec[rEps + 5] = 0.5 * st.eps[5];
-    rec[rSig] = s * (sV - st.d2G * eV);
-    rec[rSig + 1] = s * st.d2G;
-    rec[rT] = -ct * (m3K * st.H - m2G) * eV;
-    rec[rT + 1] = -ct * m2G;
-    rec[rM] = s * am;
-    rec[rM + 1] = s * bm;
-    rec[rC1] = FULLDMG ? s * (1.0 - st.om) : s * (m3K * (1.0 - st.om * st.H) - st.d2G) * kThird;
-    rec[rC2] = FULLDMG ? 0.0 : 0.5 * s * st.d2G;
-    rec[rRe] = s * (st.eN - st.eq);
-    rec[rGe] = s * P.c * st.ge[0];
-    rec[rGe + 1] = s * P.c * st.ge[1];
-    rec[rGe + 2] = s * P.c * st.ge[2];
-    double2* o = reinterpret_cast<double2*>(out + ((size_t)cell * 8 + j) * kSr);
+    __syncthreads();
+    {
+        constexpr int kPerCell = 8 * kSr / 2;  // 16-byte chunks of records per cell
+        const double2* src = reinterpret_cast<const double2*>(stage);
+        double2* dst = reinterpret_cast<double2*>(out + cell0 * 8 * kSr);
 #pragma unroll
-    for (int k = 0; k < kSr / 2; k++)
-        o[k] = make_double2(rec[2 * k], rec[2 * k + 1]);
+        for (int i = 0; i < (8 * kPerCell + 63) / 64; i++)
+        {
+            const int c = lane + 64 * i;
+            if (c < nc * kPerCell)
+                dst[c] = src[c];
+        }
+    }
 }
 
 // Geometry of every (cell, IP): J = sum_n X_n dN_n/dxi^T over the coordinate nodes in local
@@ -1108,6 +1145,7 @@ struct GdmHex8Plan
     DevBuf nodeCell, nodeHist, nodeLa, nodeSlot, geo;
     DevBuf order;        // Morton traversal order (NUTO_AMD_GDM_ORDER=0: node order)
     DevBuf ipState;      // pre-pass state records (NUTO_AMD_GDM_PREPASS=0: recompute in the node kernel)
+    DevBuf geoCell;      // affine meshes: IP 0's geometry record per cell
     bool prepass = true;
     bool affine = false;  // every cell affine: MODE 2 of the node kernel
     int nCells = 0;
@@ -1116,7 +1154,7 @@ struct GdmHex8Plan
     {
         ready = false;
         nNodes = 0;
-        for (DevBuf* b : {&nodeCell, &nodeHist, &nodeLa, &nodeSlot, &geo, &ipState, &order})
+        for (DevBuf* b : {&nodeCell, &nodeHist, &nodeLa, &nodeSlot, &geo, &ipState, &order, &geoCell})
             b->Release();
     }
 
@@ -1201,6 +1239,15 @@ struct GdmHex8Plan
                 return false;
             }
             affine = h == 0;
+            if (affine && ((err = geoCell.Alloc((size_t)nCells * gdm8::kGeo * sizeof(double))) != hipSuccess ||
+                           (err = hipMemcpy2DAsync(geoCell.p, gdm8::kGeo * sizeof(double), geo.p,
+                                                   8 * gdm8::kGeo * sizeof(double), gdm8::kGeo * sizeof(double),
+                                                   nCells, hipMemcpyDeviceToDevice, stream)) != hipSuccess ||
+                           (err = hipStreamSynchronize(stream)) != hipSuccess))
+            {
+                Release();
+                return false;
+            }
         }
         if (prepass && (err = ipState.Alloc((size_t)nCells * 8 * gdm8::kSr * sizeof(double))) != hipSuccess)
         {
@@ -1231,8 +1278,8 @@ struct GdmHex8Plan
     {
         return GdmHex8Tables{nodeCell.As<int>(), nodeHist.p ? nodeHist.As<int>() : nullptr,
                              nodeLa.As<unsigned char>(), nodeSlot.As<unsigned char>(), geo.As<double>(),
-                             ipState.p ? ipState.As<double>() : nullptr, xcdMap, 0, nNodes,
-                             order.p ? order.As<int>() : nullptr};
+                             ipState.p ? ipState.As<double>() : nullptr, geoCell.p ? geoCell.As<double>() : nullptr,
+                             xcdMap, 0, nNodes, order.p ? order.As<int>() : nullptr};
     }
 
     template <bool FULLDMG, int MODE>
@@ -1252,13 +1299,21 @@ struct GdmHex8Plan
         const GdmHex8Tables t = Tables();
         if (prepass)
         {
-            if (nCells > 0)
-                hipLaunchKernelGGL((kGdmHex8Ip<FULLDMG>), dim3((unsigned)(((long long)nCells * 8 + 63) / 64)), dim3(64),
-                                   0, stream, a, t, ipState.As<double>());
+            const dim3 ipGrid((unsigned)(((long long)nCells * 8 + 63) / 64));
             if (affine)
+            {
+                if (nCells > 0)
+                    hipLaunchKernelGGL((kGdmHex8Ip<FULLDMG, true>), ipGrid, dim3(64), 0, stream, a, t,
+                                       ipState.As<double>());
                 LaunchNodes<FULLDMG, 2>(a, t, wr, wk, grid, stream);
+            }
             else
+            {
+                if (nCells > 0)
+                    hipLaunchKernelGGL((kGdmHex8Ip<FULLDMG, false>), ipGrid, dim3(64), 0, stream, a, t,
+                                       ipState.As<double>());
                 LaunchNodes<FULLDMG, 1>(a, t, wr, wk, grid, stream);
+            }
         }
         else
             LaunchNodes<FULLDMG, 0>(a, t, wr, wk, grid, stream);
