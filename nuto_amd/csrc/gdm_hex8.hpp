@@ -298,37 +298,40 @@ __device__ inline void GatherGradients(const double v[4], int ip, double Hu[3][3
     }
 }
 
-// Per-node tables of the node kernel (built once per pattern, see GdmHex8Plan::Prepare).
+// Per-node tables of the node kernel (built once per pattern, see GdmHex8Plan::Prepare), stored in
+// traversal order: position s holds the node order[s] (nodes sorted along a Morton curve).
 struct GdmHex8Tables
 {
-    const int* nodeCell;            // [node][8]: r-th incident cell (group order), -1 past the last
-    const int* nodeHist;            // [node][8]: history column Cell::Id of that cell (nullptr: the cell index)
-    const unsigned char* nodeLa;    // [node][8]: local index of the node in that cell
-    const unsigned char* nodeSlot;  // [node][8][8]: neighbour slot (column block) of local node j
+    const int4* nodeHead;           // [pos]: node p, first CSR entry of its displacement rows, their
+                                    //        length 4 nA, first CSR entry of its eeq row
+    const int* nodeCell;            // [pos][8]: r-th incident cell (group order), -1 past the last
+    const int* nodeHist;            // [pos][8]: history column Cell::Id of that cell (nullptr: the cell index)
+    const unsigned char* nodeLa;    // [pos][8]: local index of the node in that cell
+    const unsigned char* nodeSlot;  // [pos][8][8]: neighbour slot (column block) of local node j
     const double* geo;              // [cell][ip][kGeo] geometry records
     const double* ipState;          // [cell][ip][kSr] pre-pass state records (nullptr: recompute)
     const double* geoCell;          // [cell][kGeo] IP 0's geometry record (affine meshes, else nullptr)
     int xcdMap;                     // 1: contiguous node ranges per XCD
-    int pBegin, pEnd;               // nodes [pBegin, pEnd) of the launch
-    const int* order;               // traversal position -> node (nullptr: identity)
+    int pBegin, pEnd;               // launch indices [pBegin, pEnd)
+    const int* order;               // launch index -> position (nullptr: the index is the position)
 };
 
-// Node of this workgroup, -1 past the end. Workgroups are dispatched round-robin over the 8 XCDs;
-// with xcdMap, XCD x takes the x-th contiguous eighth of the traversal, and `order` makes that
-// traversal follow space (nodes sorted along a Morton curve of their coordinates), so the nodes
-// one XCD works on at a time share their cells' records in its L2.
+// Table position of this workgroup, -1 past the end. Workgroups are dispatched round-robin over the
+// 8 XCDs; with xcdMap, XCD x takes the x-th contiguous eighth of the traversal, which follows space
+// (Morton order), so the nodes one XCD works on at a time share their cells' records in its L2.
+// Launches over a node range (one mesh of a batch) index by node and map it through `order`.
 __device__ inline int NodeOfBlock(const GdmHex8Tables& T)
 {
-    int p = blockIdx.x;
+    int i = blockIdx.x;
     if (T.xcdMap)
     {
         const int per = (int)((gridDim.x + 7) >> 3);
-        p = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+        i = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
     }
-    p += T.pBegin;
-    if (p >= T.pEnd)
+    i += T.pBegin;
+    if (i >= T.pEnd)
         return -1;
-    return T.order ? T.order[p] : p;
+    return T.order ? T.order[i] : i;
 }
 
 // Inputs of lane (r, j) of node p.
@@ -345,9 +348,9 @@ struct NodeFetch
 
 // GEO0: the cell's geometry record (IP 0's) in every lane of the group (affine cells)
 template <bool PRE, bool GEO0 = false>
-__device__ inline void FetchNode(const RowArgs& A, const GdmHex8Tables& T, int p, int r, int j, NodeFetch& f)
+__device__ inline void FetchNode(const RowArgs& A, const GdmHex8Tables& T, int pos, int r, int j, NodeFetch& f)
 {
-    const size_t pr = (size_t)p * 8 + r;
+    const size_t pr = (size_t)pos * 8 + r;
     f.cell = T.nodeCell[pr];
     f.la = T.nodeLa[pr];
     f.slot = T.nodeSlot[pr * 8 + j];
@@ -524,13 +527,13 @@ __device__ inline void AccumulateIp(const double* h, const double gb[3], double 
     acc[3] += va[0] * Nb;
     acc[7] += va[1] * Nb;
     acc[11] += va[2] * Nb;
-    double wb[3];
-    BTv3(gb, h + (sMn - sHa), wb);
-    acc[12] += wb[0];
-    acc[13] += wb[1];
-    acc[14] += wb[2];
+    // B_b^T (-N_a m) and the eeq-eeq entry, each term fused into the accumulator
+    const double* mn = h + (sMn - sHa);
+    acc[12] = fma(gb[1], mn[5], fma(gb[2], mn[4], fma(gb[0], mn[0], acc[12])));
+    acc[13] = fma(gb[0], mn[5], fma(gb[2], mn[3], fma(gb[1], mn[1], acc[13])));
+    acc[14] = fma(gb[0], mn[4], fma(gb[1], mn[3], fma(gb[2], mn[2], acc[14])));
     const double* cg = h + (sCg - sHa);
-    acc[15] += h[sSn - sHa] * Nb + cg[0] * gb[0] + cg[1] * gb[1] + cg[2] * gb[2];
+    acc[15] = fma(cg[2], gb[2], fma(cg[1], gb[1], fma(cg[0], gb[0], fma(h[sSn - sHa], Nb, acc[15]))));
 }
 
 // Phase B: lane (r, b) accumulates block (a_r, b) over IPs ip0 .. ip0 + n - 1 (records at
@@ -615,6 +618,25 @@ __device__ inline void PhaseBAffine(const double* state, int r, int b, const dou
 // MODE 2: as 1 for meshes of affine cells (J^-1 and detJ w equal at the 8 IPs of every cell, e.g.
 //         bricks): IP 0's geometry for all IPs; records without J^-1 (18 doubles); all 8 IPs in one
 //         pass (9.3 KB).
+// K is written once and not read again here: streaming (non-temporal) stores keep the records,
+// which are re-read, in the caches.
+typedef double D2v __attribute__((ext_vector_type(2)));
+
+// Stores n consecutive doubles from LDS to dst, the 64 lanes writing consecutive 16-byte pairs (a
+// leading double when dst is not 16-byte aligned, a trailing one when the rest is odd).
+__device__ inline void StoreRows(double* dst, const double* src, int n, int lane)
+{
+    const int head = (reinterpret_cast<unsigned long long>(dst) & 15) ? 1 : 0;
+    if (head && lane == 0)
+        __builtin_nontemporal_store(src[0], dst);
+    const int np = (n - head) >> 1;
+    for (int k = lane; k < np; k += 64)
+        __builtin_nontemporal_store(D2v{src[head + 2 * k], src[head + 2 * k + 1]},
+                                    reinterpret_cast<D2v*>(dst + head + 2 * k));
+    if (((n - head) & 1) && lane == 0)
+        __builtin_nontemporal_store(src[n - 1], dst + n - 1);
+}
+
 template <bool FULLDMG, int MODE>
 struct NodeLds
 {
@@ -627,7 +649,7 @@ struct NodeLds
 // The rows of node p (one 64-lane workgroup); `state` is the workgroup's LDS (NodeLds::kState doubles):
 // records [i][r][kSt]; after phase B: partial blocks, R sums, zero block and the slot map.
 template <bool WR, bool WK, bool FULLDMG, int MODE>
-__device__ inline void NodeBody(const RowArgs& A, const GdmHex8Tables& T, int p, double* state)
+__device__ inline void NodeBody(const RowArgs& A, const GdmHex8Tables& T, int pos, double* state)
 {
     using namespace gdm8;
     constexpr bool PRE = MODE >= 1, AFF = MODE == 2;
@@ -636,13 +658,13 @@ __device__ inline void NodeBody(const RowArgs& A, const GdmHex8Tables& T, int p,
     const int lane = threadIdx.x;
     const int r = lane >> 3, j = lane & 7;
     const LawParams& P = A.P;
-    // p's rows: 3 displacement rows (consecutive in the CSR) and the eeq row, each 4 nA long
-    const int ru0 = A.rowPtr[(size_t)p * 3];
-    const int L = A.rowPtr[(size_t)p * 3 + 1] - ru0;
+    // node p at table position pos; its rows: 3 displacement rows (consecutive in the CSR) and the
+    // eeq row, each 4 nA long
+    const int4 head = T.nodeHead[pos];
+    const int p = head.x, ru0 = head.y, L = head.z, re0 = head.w;
     const int nA = L >> 2;
-    const int re0 = A.rowPtr[(size_t)A.nU + p];
     NodeFetch f;
-    FetchNode<PRE, AFF>(A, T, p, r, j, f);
+    FetchNode<PRE, AFF>(A, T, pos, r, j, f);
     const bool valid = f.cell >= 0;  // r < number of cells incident to p
     const int la = f.la;
 
@@ -839,10 +861,10 @@ __device__ inline void NodeBody(const RowArgs& A, const GdmHex8Tables& T, int p,
     if (WK)
     {
         const int q = lane >> 1, h = lane & 1;
+        double sum[8];
         if (q < nA)
         {
             const unsigned long long row = *reinterpret_cast<const unsigned long long*>(&slotOf[q][0]);
-            double sum[8];
 #pragma unroll
             for (int rr = 0; rr < 8; rr++)
             {
@@ -852,18 +874,25 @@ __device__ inline void NodeBody(const RowArgs& A, const GdmHex8Tables& T, int p,
                 for (int k = 0; k < 8; k++)
                     sum[k] = rr == 0 ? src[k] : sum[k] + src[k];
             }
-            // block entry (i, c): i = row component (0..2 displacement rows, 3 eeq row),
-            // c = column component (0..2 displacement columns, 3 eeq column)
+        }
+        __syncthreads();  // the rows below overwrite the partial blocks
+        // p's rows in CSR order in LDS: [3 L] displacement rows, then [L] the eeq row. Block entry
+        // (i, c): i = row component (0..2 displacement rows, 3 eeq row), c = column component
+        // (0..2 displacement columns, 3 eeq column)
+        double* rows = state;
+        if (q < nA)
+        {
 #pragma unroll
             for (int k = 0; k < 8; k++)
             {
                 const int e = h * 8 + k;
                 const int i = e >> 2, c = e & 3;
-                const int col = c < 3 ? 3 * q + c : 3 * nA + q;
-                const size_t at = (i < 3 ? (size_t)ru0 + (size_t)i * L : (size_t)re0) + col;
-                A.K[at] = sum[k];
+                rows[i * L + (c < 3 ? 3 * q + c : 3 * nA + q)] = sum[k];
             }
         }
+        __syncthreads();
+        StoreRows(A.K + ru0, rows, 3 * L, lane);
+        StoreRows(A.K + re0, rows + 3 * L, L, lane);
     }
     if (WR && lane < 4)
     {
@@ -885,10 +914,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MODE >=
 kGdmHex8Node(const RowArgs A, const GdmHex8Tables T)
 {
     __shared__ __align__(16) double state[NodeLds<FULLDMG, MODE>::kState];
-    const int p = NodeOfBlock(T);
-    if (p < 0)
+    const int pos = NodeOfBlock(T);
+    if (pos < 0)
         return;
-    NodeBody<WR, WK, FULLDMG, MODE>(A, T, p, state);
+    NodeBody<WR, WK, FULLDMG, MODE>(A, T, pos, state);
 }
 
 // Pre-pass: the state of every (cell, IP) once, for the node kernel (which would otherwise evaluate
@@ -1142,8 +1171,8 @@ struct GdmHex8Plan
     int nNodes = 0;
     int grid = 0;     // workgroups: one per node (rounded up to a multiple of 8 with the XCD mapping)
     int xcdMap = 1;   // contiguous node ranges per XCD (env NUTO_AMD_GDM_XCD_MAP=0 switches it off)
-    DevBuf nodeCell, nodeHist, nodeLa, nodeSlot, geo;
-    DevBuf order;        // Morton traversal order (NUTO_AMD_GDM_ORDER=0: node order)
+    DevBuf nodeHead, nodeCell, nodeHist, nodeLa, nodeSlot, geo;
+    DevBuf posOf;        // node -> table position (Morton order; NUTO_AMD_GDM_ORDER=0: node order, empty)
     DevBuf ipState;      // pre-pass state records (NUTO_AMD_GDM_PREPASS=0: recompute in the node kernel)
     DevBuf geoCell;      // affine meshes: IP 0's geometry record per cell
     bool prepass = true;
@@ -1154,7 +1183,7 @@ struct GdmHex8Plan
     {
         ready = false;
         nNodes = 0;
-        for (DevBuf* b : {&nodeCell, &nodeHist, &nodeLa, &nodeSlot, &geo, &ipState, &order, &geoCell})
+        for (DevBuf* b : {&nodeHead, &nodeCell, &nodeHist, &nodeLa, &nodeSlot, &geo, &ipState, &posOf, &geoCell})
             b->Release();
     }
 
@@ -1168,25 +1197,53 @@ struct GdmHex8Plan
     // used) when the mesh exceeds the kernel's bounds; a HIP failure is returned through `err`.
     bool Prepare(int kind, int coordType, int dofType, int nip, const NodeGraph& g, int nCells, const int* cellIds,
                  int nCoord, const double* x, const double* y, const double* z, const int* cconn, const double* tab,
-                 hipStream_t stream, hipError_t& err)
+                 const int* rowPtr, int nRows, hipStream_t stream, hipError_t& err)
     {
         Release();
         err = hipSuccess;
         if (kind != kGradientDamage || coordType != kHex8 || dofType != kHex8 || nip != 8)
             return false;
         const int n = g.nNodes;
+        if (nRows != 4 * n)  // 3 displacement rows and 1 eeq row per node
+            return false;
         for (int p = 0; p < n; p++)
         {
             if (g.incPtr[p + 1] - g.incPtr[p] > gdm8::kMaxInc || g.adjPtr[p + 1] - g.adjPtr[p] > gdm8::kMaxAdj)
                 return false;
         }
-        std::vector<int> cell((size_t)n * 8, -1), hist(cellIds ? (size_t)n * 8 : 0, 0);
-        std::vector<unsigned char> la((size_t)n * 8, 0), slot((size_t)n * 64, 0);
-        for (int p = 0; p < n; p++)
+        // traversal order: position s holds node ord[s]
+        std::vector<int> ord(n);
+        bool reordered = false;
+        const char* ordEnv = std::getenv("NUTO_AMD_GDM_ORDER");
+        if (n > 0 && nCells > 0 && !(ordEnv && ordEnv[0] == '0'))
         {
+            if ((err = MortonOrder(g, nCells, nCoord, x, y, z, cconn, stream, ord)) != hipSuccess)
+                return false;
+            reordered = true;
+        }
+        else
+        {
+            for (int s = 0; s < n; s++)
+                ord[s] = s;
+        }
+        // CSR rows: displacement rows 3p .. 3p + 2, eeq row 3n + p
+        std::vector<int> rp((size_t)4 * n + 1);
+        if ((err = hipMemcpyAsync(rp.data(), rowPtr, rp.size() * sizeof(int), hipMemcpyDeviceToHost, stream)) !=
+                hipSuccess ||
+            (err = hipStreamSynchronize(stream)) != hipSuccess)
+            return false;
+        std::vector<int4> head(n);
+        std::vector<int> cell((size_t)n * 8, -1), hist(cellIds ? (size_t)n * 8 : 0, 0), pos(reordered ? n : 0);
+        std::vector<unsigned char> la((size_t)n * 8, 0), slot((size_t)n * 64, 0);
+        for (int s = 0; s < n; s++)
+        {
+            const int p = ord[s];
+            head[s] = make_int4(p, rp[(size_t)3 * p], rp[(size_t)3 * p + 1] - rp[(size_t)3 * p], rp[(size_t)3 * n + p]);
+            if (reordered)
+                pos[p] = s;
             for (int k = g.incPtr[p], r = 0; k < g.incPtr[p + 1]; k++, r++)
             {
-                const size_t pr = (size_t)p * 8 + r;
+                const size_t pr = (size_t)s * 8 + r;
                 cell[pr] = g.incCell[k];
                 la[pr] = g.incLocal[k];
                 if (cellIds)
@@ -1195,7 +1252,9 @@ struct GdmHex8Plan
                     slot[pr * 8 + j] = (unsigned char)(g.slot[(size_t)k * 8 + j] & 0x7fff);
             }
         }
-        if ((err = nodeCell.Upload(cell.data(), cell.size() * sizeof(int), stream)) != hipSuccess ||
+        if ((err = nodeHead.Upload(head.data(), head.size() * sizeof(int4), stream)) != hipSuccess ||
+            (reordered && (err = posOf.Upload(pos.data(), pos.size() * sizeof(int), stream)) != hipSuccess) ||
+            (err = nodeCell.Upload(cell.data(), cell.size() * sizeof(int), stream)) != hipSuccess ||
             (err = nodeLa.Upload(la.data(), la.size(), stream)) != hipSuccess ||
             (err = nodeSlot.Upload(slot.data(), slot.size(), stream)) != hipSuccess ||
             (cellIds && (err = nodeHist.Upload(hist.data(), hist.size() * sizeof(int), stream)) != hipSuccess) ||
@@ -1254,17 +1313,6 @@ struct GdmHex8Plan
             Release();
             return false;
         }
-        const char* ord = std::getenv("NUTO_AMD_GDM_ORDER");
-        if (n > 0 && nCells > 0 && !(ord && ord[0] == '0'))
-        {
-            std::vector<int> o;
-            if ((err = MortonOrder(g, nCells, nCoord, x, y, z, cconn, stream, o)) != hipSuccess ||
-                (err = order.Upload(o.data(), o.size() * sizeof(int), stream)) != hipSuccess)
-            {
-                Release();
-                return false;
-            }
-        }
         this->nCells = nCells;
         nNodes = n;
         const char* env = std::getenv("NUTO_AMD_GDM_XCD_MAP");
@@ -1276,10 +1324,10 @@ struct GdmHex8Plan
 
     GdmHex8Tables Tables() const
     {
-        return GdmHex8Tables{nodeCell.As<int>(), nodeHist.p ? nodeHist.As<int>() : nullptr,
+        return GdmHex8Tables{nodeHead.As<int4>(), nodeCell.As<int>(), nodeHist.p ? nodeHist.As<int>() : nullptr,
                              nodeLa.As<unsigned char>(), nodeSlot.As<unsigned char>(), geo.As<double>(),
                              ipState.p ? ipState.As<double>() : nullptr, geoCell.p ? geoCell.As<double>() : nullptr,
-                             xcdMap, 0, nNodes, order.p ? order.As<int>() : nullptr};
+                             xcdMap, 0, nNodes, nullptr};
     }
 
     template <bool FULLDMG, int MODE>
@@ -1338,7 +1386,7 @@ struct GdmHex8Plan
         t.xcdMap = 0;
         t.pBegin = first;
         t.pEnd = end;
-        t.order = nullptr;
+        t.order = posOf.p ? posOf.As<int>() : nullptr;  // launch index = node
         // per-mesh launches recompute the IP state in the node kernel (no pre-pass over the mesh's cells)
         if (a.P.damage_application == 0)
             LaunchNodes<true, 0>(a, t, true, true, end - first, stream);

@@ -39,11 +39,12 @@ __global__ void __launch_bounds__(64) kGdmHex8Vjp(const RowArgs A, const GdmHex8
     const int lane = threadIdx.x;
     const int r = lane >> 3, j = lane & 7;
     const LawParams& P = A.P;
-    const int p = NodeOfBlock(T);
-    if (p < 0)
+    const int pos = NodeOfBlock(T);
+    if (pos < 0)
         return;
+    const int p = T.nodeHead[pos].x;
     NodeFetch f;
-    FetchNode<false>(A, T, p, r, j, f);
+    FetchNode<false>(A, T, pos, r, j, f);
     const bool valid = f.cell >= 0;
     const int la = f.la;
     double lv[4] = {0.0, 0.0, 0.0, 0.0};  // lambda of node j of cell r

@@ -346,8 +346,8 @@ void PrepareGdm(nuto_ctx* ctx)
     hipError_t err = hipSuccess;
     ctx->gdm.Prepare(ctx->params.kind, ctx->coordType, ctx->f[0].type, ctx->nip, ctx->graph, ctx->nCells,
                      ctx->hasCellIds ? ctx->hcellId.data() : nullptr, ctx->nCoordNodes, ctx->x.As<double>(),
-                     ctx->y.As<double>(),
-                     ctx->z.As<double>(), ctx->cconn.As<int>(), ctx->tab.As<double>(), ctx->stream, err);
+                     ctx->y.As<double>(), ctx->z.As<double>(), ctx->cconn.As<int>(), ctx->tab.As<double>(),
+                     ctx->rowPtr.As<int>(), ctx->nRows, ctx->stream, err);
     Check(err, "hex8 node tables");
     ctx->gdmTried = true;
 }
