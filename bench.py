@@ -6,10 +6,12 @@ integrand (GradientDamage<3>, 3 displacement + 1 nonlocal-strain dof per node, 2
 the structured 100 x 100 x 100 mesh of a 100 mm cube: 1,000,000 cells, 4,121,204 dofs,
 436,334,416 nonzeros. E = 30000, nu = 0.2, k = 10, kappa0 = 1e-4, beta = 100, alpha = 0.99,
 c = 4 mm^2; u ~ U[-5e-3, 5e-3] mm, eeq ~ U[0, 5e-4], kappa ~ U[0, 3e-4] per IP (seeded).
-Inputs and outputs are resident in HBM; the timed region is K fused R+K launches.
+Inputs and outputs are resident in HBM; the timed region is K steps, each the pre-pass over the
+(cell, IP) states and the node kernel that writes R and K.
 
 Prints one JSON line (rank 0). `value` is cells/s; `roofline` prices the assembly kernel's
-algorithmic bytes per launch against 8 TB/s HBM; `cpu_baseline` times the CPU restatement of
+algorithmic bytes per launch against 8 TB/s HBM; `fp64` prices the measured FP64 flops of the step
+against 78.6 TFLOP/s; `cpu_baseline` times the CPU restatement of
 NuTo's algorithm (the oracle, OpenMP, triplets + setFromTriplets) on a bounded sample.
 """
 import argparse
@@ -36,14 +38,19 @@ def algorithmic_bytes(nnz, n_rows, n_nodes, n_cells, nip, nn_cell):
     return (8 * nnz + 8 * n_rows + 8 * (3 + 3 + 1) * n_nodes + 8 * n_cells * nip + 4 * n_cells * nn_cell)
 
 
-def measured_traffic(n):
-    """Bytes per launch from the committed counter summary (None when absent or for another size)."""
+def measured_counter(n, key):
+    """A per-step figure from the committed counter summary (None when absent or for another size)."""
     if n != 100 or not os.path.exists(TRAFFIC_FILE):
         return None
     for line in open(TRAFFIC_FILE):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":")[1])
     return None
+
+
+def measured_traffic(n):
+    """HBM bytes per step (FETCH_SIZE + WRITE_SIZE of both kernels, calibrated on a copy kernel)."""
+    return measured_counter(n, "traffic_bytes_per_launch")
 
 
 def copy_calibration(mib):
@@ -330,6 +337,12 @@ def main():
                      "kernel_ms": kernel_ms, "algorithmic_bytes": bytes_alg},
         "setup_s": setup_s,
     }
+    flops = measured_counter(n, "fp64_flops_per_launch")
+    if flops is not None:
+        # FP64 VALU flops per step (rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes, both kernels)
+        tflops = flops / (kernel_ms * 1e-3) / 1e12
+        rec["fp64"] = {"flops_per_step": flops, "achieved_tflops": tflops, "peak_tflops": FP64_PEAK_TFLOPS,
+                       "frac": tflops / FP64_PEAK_TFLOPS}
     if not args.no_adjoint:
         del R, K
         torch.cuda.empty_cache()
