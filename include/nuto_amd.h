@@ -138,6 +138,32 @@ int nuto_build_vector_matrix(nuto_ctx* ctx, const double* u, const double* e, co
  * kappa(cell, ip) = max(kappa(cell, ip), eeq(ip)), in place. */
 int nuto_update_history(nuto_ctx* ctx, const double* u, const double* e, double* kappa);
 
+/* SimpleAssembler::BuildDiagonallyLumpedMatrix (SimpleAssembler.cpp:67-98) with the integrand
+ * DynamicMomentumBalance::Hessian2 = N^T N rho (DynamicMomentumBalance.h:22-30) on dof type slot 0
+ * (the displacements, dim components): per cell the consistent mass integrated with the context's
+ * rule (Cell.h:105-119), its diagonal scaled by (sum of all entries / sum of the diagonal) /
+ * number of components (DofType::GetNum), summed over the cells into M (device, length
+ * n_dofs of slot 0, every component of a node holding the same value). Needs the pattern. */
+int nuto_build_lumped_mass(nuto_ctx* ctx, double rho, double* M);
+
+/* Fields at the integration points, Cell::Eval (Cell.h:80-93) of the context's integrand, for
+ * every cell (group order) and IP (rule order): out[(cell * n_ip + ip) * n_comp + comp], device.
+ *   NUTO_IP_STRAIN     B u, engineering Voigt [xx,yy,xy] / [xx,yy,zz,yz,zx,xy]
+ *                      (Nabla::Strain, DifferentialOperators.h:24-86); n_comp 3 (2D) or 6 (3D)
+ *   NUTO_IP_STRESS     the law's stress: LinearElastic (LinearElastic.h:76-95), LinearElasticDamage
+ *                      at omega(kappa) (LinearElasticDamage.h:65-72); n_comp as the strain
+ *   NUTO_IP_KAPPA      GradientDamage::Kappa = max(history, eeq) (GradientDamage.h:126-129), local
+ *                      damage max(history, eps_eq) (LocalIsotropicDamage.h:52-56); 1
+ *   NUTO_IP_DAMAGE     DamageLawExponential::Damage(kappa) (DamageLawExponential.h:39-45); 1
+ *   NUTO_IP_EQ_STRAIN  ModifiedMisesStrainNorm::Value(strain) (ModifiedMisesStrainNorm.h:70-84); 1
+ * The damage fields need a damage integrand. Inputs as nuto_build_vector. */
+#define NUTO_IP_STRAIN 0
+#define NUTO_IP_STRESS 1
+#define NUTO_IP_KAPPA 2
+#define NUTO_IP_DAMAGE 3
+#define NUTO_IP_EQ_STRAIN 4
+int nuto_eval_ip(nuto_ctx* ctx, const double* u, const double* e, const double* kappa, int field, double* out);
+
 /* Adjoint of BuildVector for the gradient-damage integrand (no counterpart in the reference, which
  * has no derivatives w.r.t. material parameters; its K^T lambda is the transpose of BuildMatrix,
  * SimpleAssembler.cpp:100-156 with GradientDamage::Hessian0, GradientDamage.h:92-120).

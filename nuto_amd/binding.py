@@ -18,6 +18,8 @@ DIM = {QUAD4: 2, HEX8: 3, TET4: 3, TET10: 3, HEX20: 3}
 MOMENTUM_LINEAR_ELASTIC, GRADIENT_DAMAGE, MOMENTUM_LOCAL_DAMAGE = 0, 1, 2
 PLANE_STRESS, PLANE_STRAIN = 0, 1
 FULL, UNILATERAL = 0, 1
+# fields of Context.eval_ip (nuto_eval_ip)
+IP_STRAIN, IP_STRESS, IP_KAPPA, IP_DAMAGE, IP_EQ_STRAIN = 0, 1, 2, 3, 4
 
 _ERRORS = {1: "argument", 2: "state", 3: "device", 4: "memory", 5: "unsupported"}
 
@@ -119,6 +121,8 @@ def lib():
         for f in ("nuto_build_vector", "nuto_build_matrix"):
             getattr(L, f).argtypes = [vp, vp, vp, vp, vp]
         L.nuto_update_history.argtypes = [vp, vp, vp, vp]
+        L.nuto_build_lumped_mass.argtypes = [vp, ctypes.c_double, vp]
+        L.nuto_eval_ip.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp]
         L.nuto_build_vector_matrix.argtypes = [vp, vp, vp, vp, vp, vp]
         L.nuto_device_alloc.argtypes = [vp, ctypes.POINTER(vp), ctypes.c_ulonglong]
         L.nuto_device_free.argtypes = [vp, vp]
@@ -228,18 +232,23 @@ class Context:
         conn = np.ascontiguousarray(conn, np.int32)
         cid = None if cell_ids is None else np.ascontiguousarray(cell_ids, np.int32)
         self.n_cells = conn.shape[0]
+        self.dim = coords.shape[1]
         self._chk(lib().nuto_set_cells(self.h, coord_type, conn.shape[0], coords.shape[0], _np_ptr(coords),
                                        _np_ptr(conn), _np_ptr(cid)))
 
     def set_dof_type(self, slot, dof_type, ncomp, n_dofs, cell_dofs_):
         cd = np.ascontiguousarray(cell_dofs_, np.int32)
         self._chk(lib().nuto_set_dof_type(self.h, slot, dof_type, ncomp, n_dofs, _np_ptr(cd)))
+        if slot == 0:
+            self.n_dofs0, self.ncomp0 = n_dofs, ncomp
 
     def set_integrand(self, params):
         self._chk(lib().nuto_set_integrand(self.h, ctypes.byref(params)))
 
     def set_rule(self, kind, n):
         self._chk(lib().nuto_set_rule(self.h, kind, n))
+        # IntegrationTypeTensorProduct: n^dim points; IntegrationTypeTetrahedron order 1: 1, order 2: 4
+        self.n_ip = n ** self.dim if kind == 0 else {1: 1, 2: 4}[n]
 
     def set_history_ld(self, ld):
         self._chk(lib().nuto_set_history_ld(self.h, ld))
@@ -309,6 +318,21 @@ class Context:
     def update_history(self, u, e, kappa):
         self._chk(lib().nuto_update_history(self.h, _dev_ptr(u), _dev_ptr(e), _dev_ptr(kappa)))
         return kappa
+
+    def build_lumped_mass(self, rho, out=None):
+        """BuildDiagonallyLumpedMatrix with DynamicMomentumBalance::Hessian2 (N^T N rho): one value
+        per dof of slot 0."""
+        M = self._new(self.n_dofs0) if out is None else out
+        self._chk(lib().nuto_build_lumped_mass(self.h, float(rho), _dev_ptr(M)))
+        return M
+
+    def eval_ip(self, field, u, e=None, kappa=None, out=None):
+        """Cell::Eval of one field (IP_STRAIN, IP_STRESS, IP_KAPPA, IP_DAMAGE, IP_EQ_STRAIN) at every
+        (cell, IP): a (n_cells, n_ip, n_comp) tensor."""
+        nc = (3 if self.dim == 2 else 6) if field in (IP_STRAIN, IP_STRESS) else 1
+        o = self._new(self.n_cells * self.n_ip * nc) if out is None else out
+        self._chk(lib().nuto_eval_ip(self.h, _dev_ptr(u), _dev_ptr(e), _dev_ptr(kappa), field, _dev_ptr(o)))
+        return o.view(self.n_cells, self.n_ip, nc)
 
 
 class Batch:

@@ -14,6 +14,7 @@
 #include "gdm_hex8.hpp"
 #include "gdm_hex8_vjp.hpp"
 #include "batch.hpp"
+#include "cell_fields.hpp"
 #include "mesh.hpp"
 
 using namespace nuto_amd;
@@ -284,9 +285,41 @@ void LaunchHistoryT(nuto_ctx* ctx, const RowArgs& a, double* kappa)
     }
 }
 
+template <int DIM, int NNC, int NND, int NIP>
+void LaunchLumpedT(nuto_ctx* ctx, const RowArgs& a, double rho, double* M)
+{
+    if (a.nNodes == 0)
+        return;
+    hipLaunchKernelGGL((kLumpedMass<DIM, NNC, NND, NIP>), dim3((a.nNodes + 63) / 64), dim3(64), 0, ctx->stream, a, rho,
+                       M);
+    Check(hipGetLastError(), "lumped mass launch");
+    SyncCheck(ctx, "lumped mass kernel");
+}
+
+template <int KIND, int DIM, int NNC, int NND, int NIP>
+void LaunchEvalT(nuto_ctx* ctx, const RowArgs& a, int field, double* out)
+{
+    const long long n = (long long)a.nCells * NIP;
+    if (n == 0)
+        return;
+    hipLaunchKernelGGL((kEvalIp<KIND, DIM, NNC, NND, NIP>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       ctx->stream, a, field, out);
+    Check(hipGetLastError(), "ip field launch");
+    SyncCheck(ctx, "ip field kernel");
+}
+
+// Arguments of the per-cell outputs (modes 2 and 3 of Dispatch)
+struct CellOutput
+{
+    double rho = 0.0;       // mode 2: density
+    int field = 0;          // mode 3: IpField
+    double* out = nullptr;  // mode 2: lumped mass per dof of slot 0; mode 3: field values
+};
+
 // Dispatch over the instantiated (integrand, coordinate element, dof element, rule) combinations.
-// mode 0: assembly, mode 1: history update
-void Dispatch(nuto_ctx* ctx, const RowArgs& a, bool wr, bool wk, int mode, double* kappaOut)
+// mode 0: assembly, 1: history update, 2: lumped mass, 3: IP field
+void Dispatch(nuto_ctx* ctx, const RowArgs& a, bool wr, bool wk, int mode, double* kappaOut,
+              const CellOutput* co = nullptr)
 {
     const int kind = ctx->params.kind, ct = ctx->coordType, dt = ctx->f[0].type, nip = ctx->nip;
 #define NUTO_CASE(KIND, DIM, CT, NNC, DT, NND, NIP, BH)                       \
@@ -294,8 +327,12 @@ void Dispatch(nuto_ctx* ctx, const RowArgs& a, bool wr, bool wk, int mode, doubl
     {                                                                          \
         if (mode == 0)                                                         \
             LaunchRowsT<KIND, DIM, NNC, NND, NIP, BH>(ctx, a, wr, wk);         \
-        else                                                                   \
+        else if (mode == 1)                                                    \
             LaunchHistoryT<KIND, DIM, NNC, NND, NIP>(ctx, a, kappaOut);        \
+        else if (mode == 2)                                                    \
+            LaunchLumpedT<DIM, NNC, NND, NIP>(ctx, a, co->rho, co->out);       \
+        else                                                                   \
+            LaunchEvalT<KIND, DIM, NNC, NND, NIP>(ctx, a, co->field, co->out); \
         return;                                                                \
     }
     NUTO_CASE(kLinearElastic, 2, kQuad4, 4, kQuad4, 4, 4, 4)
@@ -758,6 +795,44 @@ int nuto_update_history(nuto_ctx* ctx, const double* u, const double* e, double*
             BuildTables(ctx);
         RowArgs a = MakeRowArgs(ctx, u, e, kappa, nullptr, nullptr);
         Dispatch(ctx, a, false, false, 1, kappa);
+    });
+}
+
+int nuto_build_lumped_mass(nuto_ctx* ctx, double rho, double* M)
+{
+    return Guard(ctx, [&] {
+        Require(ctx->cellsSet && ctx->f[0].set, NUTO_ERR_STATE, "cells and dof type slot 0 must be set");
+        Require(ctx->integrandSet && ctx->ruleSet, NUTO_ERR_STATE, "integrand and rule must be set");
+        Require(ctx->patternBuilt, NUTO_ERR_STATE, "pattern not built (call nuto_build_pattern)");
+        Require(ctx->f[0].ncomp == ctx->dim, NUTO_ERR_UNSUPPORTED,
+                "the lumped mass is defined for the displacement dof type (dim components) in slot 0");
+        Require(M != nullptr, NUTO_ERR_ARGUMENT, "M is NULL");
+        if (!ctx->tabValid)
+            BuildTables(ctx);
+        RowArgs a = MakeRowArgs(ctx, nullptr, nullptr, nullptr, nullptr, nullptr);
+        CellOutput co;
+        co.rho = rho;
+        co.out = M;
+        Dispatch(ctx, a, false, false, 2, nullptr, &co);
+    });
+}
+
+int nuto_eval_ip(nuto_ctx* ctx, const double* u, const double* e, const double* kappa, int field, double* out)
+{
+    return Guard(ctx, [&] {
+        Require(field >= NUTO_IP_STRAIN && field <= NUTO_IP_EQ_STRAIN, NUTO_ERR_ARGUMENT, "unknown field");
+        Require(out != nullptr, NUTO_ERR_ARGUMENT, "out is NULL");
+        const bool damageField = field == NUTO_IP_KAPPA || field == NUTO_IP_DAMAGE || field == NUTO_IP_EQ_STRAIN;
+        Require(!(damageField && ctx->params.kind == kLinearElastic), NUTO_ERR_UNSUPPORTED,
+                "kappa, damage and the equivalent strain need a damage integrand");
+        CheckReady(ctx, u, e, kappa);
+        if (!ctx->tabValid)
+            BuildTables(ctx);
+        RowArgs a = MakeRowArgs(ctx, u, e, kappa, nullptr, nullptr);
+        CellOutput co;
+        co.field = field;
+        co.out = out;
+        Dispatch(ctx, a, false, false, 3, nullptr, &co);
     });
 }
 

@@ -92,6 +92,7 @@ void SimpleAssembler::Setup(const CellGroup& cells, const std::vector<DofType>& 
     }
     Check(nuto_set_integrand(mCtx, &p), "SimpleAssembler::SetIntegrand");
     const bool tet = cells.coordType == NUTO_TET4;
+    mTet = tet;
     Check(nuto_set_rule(mCtx, tet ? 1 : 0, tet ? 1 : 2), "SimpleAssembler::SetRule");
     Check(nuto_build_pattern(mCtx, &mNnz, &mRows), "SimpleAssembler::BuildPattern");
 }
@@ -190,6 +191,73 @@ DofMatrixSparse<double> SimpleAssembler::BuildMatrix(const CellGroup& cells, con
     DofMatrixSparse<double> K;
     Run({f.disp}, state, nullptr, false, true, nullptr, &K);
     return K;
+}
+
+DofVector<double> SimpleAssembler::BuildDiagonallyLumpedMatrix(const CellGroup& cells,
+                                                               const DynamicMomentumBalance& f)
+{
+    const int dim = cells.coordType == NUTO_QUAD4 ? 2 : 3;
+    const nuto_params p{NUTO_MOMENTUM_LINEAR_ELASTIC, dim, f.E, f.nu, f.planeState, 0, 0, 0, 0, 0, 0};
+    Setup(cells, {f.disp}, p, dim);
+    const int n = mDofInfo.Total(f.disp);
+    DeviceArray m(mCtx, n * sizeof(double));
+    Check(nuto_build_lumped_mass(mCtx, f.rho, m.d()), "SimpleAssembler::BuildDiagonallyLumpedMatrix");
+    Check(nuto_synchronize(mCtx), "SimpleAssembler::BuildDiagonallyLumpedMatrix");
+    DofVector<double> M;
+    M[f.disp].resize(n);
+    Check(nuto_copy_to_host(mCtx, M[f.disp].data(), m.p, n * sizeof(double)),
+          "SimpleAssembler::BuildDiagonallyLumpedMatrix");
+    return M;
+}
+
+std::vector<double> SimpleAssembler::RunEval(const std::vector<DofType>& dofs, const DofVector<double>& state,
+                                             const std::vector<double>* kappas, int nCells, int dim, IpField field)
+{
+    std::vector<int> n;
+    for (const DofType& d : dofs)
+        n.push_back(mDofInfo.Total(d));
+    DeviceArray u(mCtx, n[0] * sizeof(double));
+    Check(nuto_copy_to_device(mCtx, u.p, state[dofs[0]].data(), n[0] * sizeof(double)), "Eval");
+    std::unique_ptr<DeviceArray> e, k;
+    if (dofs.size() > 1)
+    {
+        e.reset(new DeviceArray(mCtx, n[1] * sizeof(double)));
+        Check(nuto_copy_to_device(mCtx, e->p, state[dofs[1]].data(), n[1] * sizeof(double)), "Eval");
+    }
+    if (kappas)
+    {
+        k.reset(new DeviceArray(mCtx, kappas->size() * sizeof(double)));
+        Check(nuto_copy_to_device(mCtx, k->p, kappas->data(), kappas->size() * sizeof(double)), "Eval");
+    }
+    // IPs of the rule Setup chose: tetrahedron order 1 (1 point) or 2x2(x2) Gauss
+    const int nip = mTet ? 1 : (dim == 2 ? 4 : 8);
+    const int nc = (field == IpField::Strain || field == IpField::Stress) ? (dim == 2 ? 3 : 6) : 1;
+    const size_t count = (size_t)nCells * nip * nc;
+    DeviceArray o(mCtx, count * sizeof(double));
+    Check(nuto_eval_ip(mCtx, u.d(), e ? e->d() : nullptr, k ? k->d() : nullptr, (int)field, o.d()),
+          "SimpleAssembler::Eval");
+    Check(nuto_synchronize(mCtx), "SimpleAssembler::Eval");
+    std::vector<double> out(count);
+    Check(nuto_copy_to_host(mCtx, out.data(), o.p, count * sizeof(double)), "SimpleAssembler::Eval");
+    return out;
+}
+
+std::vector<double> SimpleAssembler::Eval(const CellGroup& cells, const GradientDamage& f,
+                                          const DofVector<double>& state, IpField field)
+{
+    const nuto_params p{NUTO_GRADIENT_DAMAGE, 3, f.E, f.nu, 0, f.k, f.kappa0, f.beta, f.alpha, f.c,
+                        f.damageApplication};
+    Setup(cells, {f.disp, f.eeq}, p, 3);
+    return RunEval({f.disp, f.eeq}, state, &f.kappas, cells.NumCells(), 3, field);
+}
+
+std::vector<double> SimpleAssembler::Eval(const CellGroup& cells, const MomentumBalance& f,
+                                          const DofVector<double>& state, IpField field)
+{
+    const int dim = cells.coordType == NUTO_QUAD4 ? 2 : 3;
+    const nuto_params p{NUTO_MOMENTUM_LINEAR_ELASTIC, dim, f.E, f.nu, f.planeState, 0, 0, 0, 0, 0, 0};
+    Setup(cells, {f.disp}, p, dim);
+    return RunEval({f.disp}, state, nullptr, cells.NumCells(), dim, field);
 }
 
 }  // namespace host

@@ -124,6 +124,23 @@ struct MomentumBalance
     int planeState = 0;  // 2D: 0 PLANE_STRESS, 1 PLANE_STRAIN
 };
 
+// DynamicMomentumBalance<D> (DynamicMomentumBalance.h:10-37): MomentumBalance whose Hessian2 is the
+// consistent mass N^T N rho
+struct DynamicMomentumBalance : MomentumBalance
+{
+    double rho = 1.0;
+};
+
+// fields of SimpleAssembler::Eval (Cell::Eval with the integrand's quantities, Cell.h:80-93)
+enum class IpField
+{
+    Strain = NUTO_IP_STRAIN,      // Nabla::Strain of the displacements (engineering Voigt)
+    Stress = NUTO_IP_STRESS,      // the law's stress
+    Kappa = NUTO_IP_KAPPA,        // GradientDamage::Kappa (GradientDamage.h:126-129)
+    Damage = NUTO_IP_DAMAGE,      // DamageLawExponential::Damage(kappa)
+    EqStrain = NUTO_IP_EQ_STRAIN  // ModifiedMisesStrainNorm::Value(strain)
+};
+
 class SimpleAssembler
 {
 public:
@@ -142,16 +159,29 @@ public:
     DofMatrixSparse<double> BuildMatrix(const CellGroup& cells, const MomentumBalance& f,
                                         const DofVector<double>& state);
 
+    // SimpleAssembler::BuildDiagonallyLumpedMatrix (SimpleAssembler.cpp:67-98) with
+    // DynamicMomentumBalance::Hessian2 (DynamicMomentumBalance.h:22-30): the displacement dofs only
+    DofVector<double> BuildDiagonallyLumpedMatrix(const CellGroup& cells, const DynamicMomentumBalance& f);
+
+    // Cell::Eval (Cell.h:80-93) of one field for every cell of the group: values [cell][ip][comp]
+    std::vector<double> Eval(const CellGroup& cells, const GradientDamage& f, const DofVector<double>& state,
+                             IpField field);
+    std::vector<double> Eval(const CellGroup& cells, const MomentumBalance& f, const DofVector<double>& state,
+                             IpField field);
+
 private:
     void Check(int status, const char* caller) const;
     void Setup(const CellGroup& cells, const std::vector<DofType>& dofs, const nuto_params& p, int dim);
     void Run(const std::vector<DofType>& dofs, const DofVector<double>& state, const std::vector<double>* kappas,
              bool wantR, bool wantK, DofVector<double>* R, DofMatrixSparse<double>* K);
+    std::vector<double> RunEval(const std::vector<DofType>& dofs, const DofVector<double>& state,
+                                const std::vector<double>* kappas, int nCells, int dim, IpField field);
 
     DofInfo mDofInfo;
     nuto_ctx* mCtx = nullptr;
     long long mNnz = 0;
     int mRows = 0;
+    bool mTet = false;  // the rule of the last Setup: tetrahedron order 1 (else 2x2(x2) Gauss)
 };
 
 }  // namespace host

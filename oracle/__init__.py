@@ -82,6 +82,8 @@ def lib():
         L.orc_problem_cell.argtypes = [P, ctypes.c_int, P, P, P, P, P]
         L.orc_problem_cell_dofs.argtypes = [P, ctypes.c_int, ip]
         L.orc_set_mutation.argtypes = [ctypes.c_int]
+        L.orc_problem_lumped_mass.argtypes = [P, ctypes.c_double, dp, P]
+        L.orc_problem_eval_ip.argtypes = [P, P, P, P, ctypes.c_int, P]
         L.orc_mises.restype = ctypes.c_double
         L.orc_damage_exp.restype = ctypes.c_double
         L.orc_mock_assemble.restype = ctypes.c_longlong
@@ -292,6 +294,25 @@ class Problem:
                                _ptr(None if kappa is None else np.ascontiguousarray(kappa, np.float64)), _ptr(Re),
                                _ptr(Ke))
         return Re, Ke
+
+    def lumped_mass(self, rho):
+        """BuildDiagonallyLumpedMatrix with Hessian2 = N^T N rho on dof type 0: (M, magnitude scale)."""
+        M = np.zeros(self.ndof[0])
+        Mb = np.zeros(self.ndof[0])
+        lib().orc_problem_lumped_mass(self.h, float(rho), M, _ptr(Mb))
+        return M, Mb
+
+    def eval_ip(self, field, u, e=None, kappa=None):
+        """Cell::Eval of one field at every (cell, IP): (n_cells, n_ip, n_comp)."""
+        nc = lib().orc_problem_eval_ip(self.h, None, None, None, field, None)
+        if nc == 0:
+            raise ValueError("field not defined for this integrand")
+        out = np.zeros(self.ncells * self.nip * nc)
+        lib().orc_problem_eval_ip(self.h, _ptr(np.ascontiguousarray(u, np.float64)),
+                                  _ptr(None if e is None else np.ascontiguousarray(e, np.float64)),
+                                  _ptr(None if kappa is None else np.ascontiguousarray(kappa, np.float64)), field,
+                                  _ptr(out))
+        return out.reshape(self.ncells, self.nip, nc)
 
     def cell_dofs(self, c):
         out = np.zeros(64, dtype=np.int32)

@@ -1452,6 +1452,161 @@ static void Assemble(Problem& P, const double* u, const double* e, const double*
         }
 }
 
+// ---------------------------------------------------------------------------------------------
+// lumped mass and fields at the integration points
+// ---------------------------------------------------------------------------------------------
+// SimpleAssembler::BuildDiagonallyLumpedMatrix (SimpleAssembler.cpp:67-98) with the integrand
+// DynamicMomentumBalance::Hessian2 = N^T N rho (DynamicMomentumBalance.h:22-30) on dof type 0, one
+// thread in cell order: per cell the consistent mass (Cell.h:105-119), localDiagonalDof *=
+// (fullSum / diagonalSum) / dof.GetNum(), added at the cell's dof numbers. Mb: the same with
+// |detJ| (the magnitude scale of each entry).
+static void LumpedMass(const Problem& P, double rho, std::vector<double>& M, std::vector<double>& Mb)
+{
+    const Mesh& m = P.mesh;
+    const int dim = m.dim, nnC = NumNodes(m.coordType);
+    const DofLayer& L = m.dofs[P.layer[0]];
+    const int nnD = NumNodes(L.type), nc = L.ncomp, n = nc * nnD;
+    M.assign(P.ndof[0], 0.0);
+    Mb.assign(P.ndof[0], 0.0);
+    std::vector<double> Xc((size_t)nnC * dim);
+    for (int c = 0; c < m.nCells; c++)
+    {
+        for (int k = 0; k < nnC; k++)
+            for (int d = 0; d < dim; d++)
+                Xc[k * dim + d] = m.X[(size_t)m.conn[(size_t)c * nnC + k] * dim + d];
+        Mat Me, Mm;
+        for (int ip = 0; ip < P.rule.Num(); ip++)
+        {
+            const double* xi = P.rule.xi[ip].data();
+            const double w = P.rule.w[ip];
+            double dNc[60];
+            DShapeDim(m.coordType, xi, dNc);
+            Jacobian jac = MakeJacobian(dim, nnC, Xc.data(), dNc);
+            std::vector<double> N(nnD);
+            Shape(L.type, xi, N.data());
+            Mat Nm = NMatrix(N, nc);
+            Mat f = Scale(Mul(T(Nm), Nm), rho);
+            Mat fm = Scale(Abs(f), std::abs(jac.det) * w);
+            for (double& x : f.v)
+                x = (x * jac.det) * w;
+            if (ip == 0)
+            {
+                Me = f;
+                Mm = fm;
+            }
+            else
+            {
+                Me = Add(Me, f);
+                Mm = Add(Mm, fm);
+            }
+        }
+        double diagSum = 0.0, fullSum = 0.0;
+        for (int i = 0; i < n; i++)
+            diagSum += Me(i, i);
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i < n; i++)
+                fullSum += Me(i, j);
+        const double scale = (fullSum / diagSum) / nc;
+        std::vector<int> dofs = CellDofs(P, c, 0);
+        for (int i = 0; i < n; i++)
+        {
+            M[dofs[i]] += Me(i, i) * scale;
+            Mb[dofs[i]] += Mm(i, i) * std::abs(scale);
+        }
+    }
+}
+
+// Cell::Eval (Cell.h:80-93) of one field of the problem's integrand at every (cell, IP), written
+// to out[(c * nip + ip) * ncomp + comp]:
+//   0 strain  Nabla::Strain (DifferentialOperators.h:24-86) applied to u
+//   1 stress  LinearElastic (LinearElastic.h:76-95) or LinearElasticDamage::Stress at omega(kappa)
+//             (LinearElasticDamage.h:65-72)
+//   2 kappa   GradientDamage::Kappa = max(history, eeq) (GradientDamage.h:126-129); local damage:
+//             EvolutionImplicit, max(history, eps_eq(strain))
+//   3 damage  DamageLawExponential::Damage(kappa) (DamageLawExponential.h:39-45)
+//   4 eps_eq  ModifiedMisesStrainNorm::Value(strain) (ModifiedMisesStrainNorm.h:70-84)
+// Returns the number of components (0: field not defined for the integrand).
+static int EvalIp(const Problem& P, const double* u, const double* e, const double* kappa, int field,
+                  std::vector<double>& out)
+{
+    const Mesh& m = P.mesh;
+    const int dim = m.dim, nnC = NumNodes(m.coordType), nip = P.rule.Num();
+    const int nv = dim == 2 ? 3 : 6;
+    const int ncomp = field <= 1 ? nv : 1;
+    if (field > 1 && P.p.kind == 0)
+        return 0;
+    out.assign((size_t)m.nCells * nip * ncomp, 0.0);
+    std::vector<double> Xc((size_t)nnC * dim);
+    for (int c = 0; c < m.nCells; c++)
+    {
+        for (int k = 0; k < nnC; k++)
+            for (int d = 0; d < dim; d++)
+                Xc[k * dim + d] = m.X[(size_t)m.conn[(size_t)c * nnC + k] * dim + d];
+        std::vector<double> nodeVal[2];
+        const double* src[2] = {u, e};
+        int nnD[2] = {0, 0};
+        for (int t = 0; t < P.ntypes; t++)
+        {
+            const DofLayer& L = m.dofs[P.layer[t]];
+            nnD[t] = NumNodes(L.type);
+            for (int a = 0; a < nnD[t]; a++)
+            {
+                int node = L.conn[(size_t)c * nnD[t] + a];
+                for (int comp = 0; comp < L.ncomp; comp++)
+                    nodeVal[t].push_back(src[t][P.numbers[t][(size_t)node * L.ncomp + comp]]);
+            }
+        }
+        CellLaw law = LawFor(P, c);
+        for (int ip = 0; ip < nip; ip++)
+        {
+            const double* xi = P.rule.xi[ip].data();
+            double dNc[60];
+            DShapeDim(m.coordType, xi, dNc);
+            Jacobian jac = MakeJacobian(dim, nnC, Xc.data(), dNc);
+            const int ty = m.dofs[P.layer[0]].type;
+            double dN[60];
+            DShapeDim(ty, xi, dN);
+            Mat B = StrainB(GlobalDerivatives(jac, dN, nnD[0]));
+            Mat strainM = Mul(B, Col(nodeVal[0]));
+            const double* strain = strainM.v.data();
+            std::vector<double> val;
+            if (field == 0)
+                val = strainM.v;
+            else if (P.p.kind == 0)
+                val = Mul(ElasticC(dim, law.E, P.p.nu, P.p.plane_state), strainM).v;
+            else
+            {
+                ModMises3 norm(P.p.nu, P.p.k);
+                const double eq = norm.Value(strain);
+                double drive = eq;
+                if (P.p.kind == 1)
+                {
+                    std::vector<double> Ne(nnD[1]);
+                    Shape(m.dofs[P.layer[1]].type, xi, Ne.data());
+                    Mat Nr(1, nnD[1]);
+                    Nr.v = Ne;
+                    drive = Mul(Nr, Col(nodeVal[1])).v[0];
+                }
+                const double kh = kappa[(size_t)c + (size_t)ip * m.nCells];
+                const double kap = std::max(kh, drive);
+                DamageExp dl{law.kappa0, law.beta, P.p.alpha};
+                const double omega = dl.Damage(kap);
+                if (field == 1)
+                    val = ElasticDamage3(law.E, P.p.nu, P.p.damage_application).Stress(strain, omega);
+                else if (field == 2)
+                    val = {kap};
+                else if (field == 3)
+                    val = {omega};
+                else
+                    val = {eq};
+            }
+            for (int k = 0; k < ncomp; k++)
+                out[((size_t)c * nip + ip) * ncomp + k] = val[k];
+        }
+    }
+    return ncomp;
+}
+
 }  // namespace orc
 
 // =============================================================================================
@@ -1662,6 +1817,27 @@ void* orc_problem(void* mp, const Params* p, int ruleKind, int ruleN, int layer0
         P->ndof[t] = (int)P->numbers[t].size();
     }
     return P;
+}
+// lumped mass of dof type 0 (length ndof[0]) and its magnitude scale
+void orc_problem_lumped_mass(void* Pp, double rho, double* M, double* Mb)
+{
+    std::vector<double> m, mb;
+    LumpedMass(*(Problem*)Pp, rho, m, mb);
+    std::copy(m.begin(), m.end(), M);
+    if (Mb)
+        std::copy(mb.begin(), mb.end(), Mb);
+}
+// field values at every (cell, IP); returns the number of components (0: not defined); with out
+// NULL only the count
+int orc_problem_eval_ip(void* Pp, const double* u, const double* e, const double* kappa, int field, double* out)
+{
+    const Problem& P = *(Problem*)Pp;
+    if (!out)
+        return field <= 1 ? (P.mesh.dim == 2 ? 3 : 6) : (P.p.kind == 0 ? 0 : 1);
+    std::vector<double> o;
+    const int nc = EvalIp(P, u, e, kappa, field, o);
+    std::copy(o.begin(), o.end(), out);
+    return nc;
 }
 void orc_problem_free(void* P)
 {

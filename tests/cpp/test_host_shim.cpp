@@ -20,6 +20,8 @@ long long orc_problem_assemble(void* Pp, const double* u, const double* e, const
                                int nthreads);
 void orc_problem_R(void* Pp, double* R, double* Rb);
 void orc_problem_K(void* Pp, int* rp, int* ci, double* v, double* b);
+void orc_problem_lumped_mass(void* Pp, double rho, double* M, double* Mb);
+int orc_problem_eval_ip(void* Pp, const double* u, const double* e, const double* kappa, int field, double* out);
 }
 
 using namespace nuto_amd::host;
@@ -188,12 +190,59 @@ static void MomentumBalanceCase()
     orc_mesh_free(om);
 }
 
+// BuildDiagonallyLumpedMatrix with DynamicMomentumBalance and Eval of the stress, against the oracle
+static void LumpedMassAndEvalCase()
+{
+    Mesh m = MakeMesh(NUTO_HEX8, 3, 2, 2);
+    DofType disp("Displacements", 3);
+    DofInfo info;
+    info.numIndependent[disp.Id()] = 3 * m.nDofNodes;
+    CellGroup cells;
+    cells.coordType = NUTO_HEX8;
+    cells.coords = m.X;
+    cells.conn = m.C;
+    cells.dofNumbering[disp.Id()] = {NUTO_HEX8, CellDofs(m.ranks, 3)};
+    DynamicMomentumBalance f{{disp, 30000.0, 0.2, 0}, 2.4e-9};
+    SimpleAssembler asmbl(info);
+    DofVector<double> M = asmbl.BuildDiagonallyLumpedMatrix(cells, f);
+    void* om = orc_mesh_from_arrays(NUTO_HEX8, m.nNodes, m.X.data(), m.nCells, m.C.data());
+    const int Ld = orc_mesh_add_dof(om, NUTO_HEX8, 3);
+    nuto_params p{NUTO_MOMENTUM_LINEAR_ELASTIC, 3, 30000.0, 0.2, 0, 0, 0, 0, 0, 0, 0};
+    void* P = orc_problem(om, &p, 0, 2, Ld, -1);
+    const int n = 3 * m.nDofNodes;
+    std::vector<double> ref(n), bound(n);
+    orc_problem_lumped_mass(P, f.rho, ref.data(), bound.data());
+    for (int i = 0; i < n; i++)
+        EXPECT(std::fabs(M[disp][i] - ref[i]) <= 1e-12 * bound[i], "lumped mass");
+    unsigned long long seed = 4242;
+    DofVector<double> state;
+    for (int i = 0; i < n; i++)
+        state[disp].push_back(Uniform(seed, -1e-3, 1e-3));
+    std::vector<double> sig = asmbl.Eval(cells, f, state, IpField::Stress);
+    std::vector<double> sref((size_t)m.nCells * 8 * 6);
+    EXPECT(orc_problem_eval_ip(P, state[disp].data(), nullptr, nullptr, NUTO_IP_STRESS, sref.data()) == 6,
+           "eval components");
+    double scale = 0.0;
+    for (double v : sref)
+        scale = std::fmax(scale, std::fabs(v));
+    EXPECT(sig.size() == sref.size(), "eval size");
+    for (size_t i = 0; i < sref.size() && i < sig.size(); i++)
+        if (!(std::fabs(sig[i] - sref[i]) <= 1e-12 * (std::fabs(sref[i]) + scale)))
+        {
+            EXPECT(false, "eval stress");
+            break;
+        }
+    orc_problem_free(P);
+    orc_mesh_free(om);
+}
+
 int main()
 {
     try
     {
         GradientDamageCase();
         MomentumBalanceCase();
+        LumpedMassAndEvalCase();
     }
     catch (const std::exception& e)
     {
