@@ -164,6 +164,50 @@ int nuto_build_lumped_mass(nuto_ctx* ctx, double rho, double* M);
 #define NUTO_IP_EQ_STRAIN 4
 int nuto_eval_ip(nuto_ctx* ctx, const double* u, const double* e, const double* kappa, int field, double* out);
 
+/* ---- constraints and the constrained linear solve ------------------------------------------------
+ * Constraints::Add (Constraints.cpp:7-21) for dof type slot `slot`: n_eq equations (Equation.h,
+ * Term.h), equation i reading  1 * x[dep[i]] + sum_t coef[t] * x[term_dof[t]] = rhs_i(time)  over
+ * t in [term_ptr[i], term_ptr[i + 1]), dof numbers of that slot's DofType. Replaces the slot's
+ * equations (n_eq = 0 removes them; setting the slot's dof type also does). The checks of
+ * Constraints::TermChecker (Constraints.h:112-129) and GetJKNumbering (Constraints.cpp:62-71)
+ * apply: every dependent dof is numbered and in range, appears in no other equation, and no term
+ * is a dependent dof. Host arrays. */
+int nuto_set_constraints(nuto_ctx* ctx, int slot, int n_eq, const int* dep, const int* term_ptr, const int* term_dof,
+                         const double* coef);
+
+#define NUTO_SOLVER_CG 0       /* EigenConjugateGradient (EigenSparseSolve.cpp:44), symmetric K */
+#define NUTO_SOLVER_BICGSTAB 1 /* EigenBiCGSTAB (EigenSparseSolve.cpp:48), general K */
+typedef struct nuto_solver_options
+{
+    int method;           /* NUTO_SOLVER_CG or NUTO_SOLVER_BICGSTAB; diagonal preconditioner, x0 = 0 */
+    double tolerance;     /* stop at |C^T f - A u| <= tolerance |C^T f|; <= 0: machine epsilon (Eigen's) */
+    int max_iterations;   /* <= 0: 2 x the independent dofs (Eigen's default) */
+} nuto_solver_options;
+typedef struct nuto_solve_info
+{
+    int iterations;            /* Eigen's count: the iterations before the converging one */
+    double relative_residual;  /* |C^T f - A u| / |C^T f| at the end */
+    int independent_dofs;      /* size of the reduced system */
+} nuto_solve_info;
+
+/* ConstrainedSystemSolver::Solve -> NuTo::Solve (Solve.cpp:8-32): with C the unit constraint
+ * matrix of all slots (Constraints::BuildUnitConstraintMatrix, Constraints.cpp:89-137, block
+ * diagonal), x = C u with (C^T K C) u = C^T f. K: values in the context's pattern (as
+ * nuto_build_matrix writes them), f and x: length n_rows; device arrays. A = C^T K C is applied
+ * as three sparse products, never formed; K stays on the device. options NULL: BiCGSTAB,
+ * machine-epsilon tolerance, 2 n iterations. info may be NULL. */
+int nuto_solve(nuto_ctx* ctx, const double* K, const double* f, double* x, const nuto_solver_options* options,
+               nuto_solve_info* info);
+
+/* ConstrainedSystemSolver::SolveTrialState -> NuTo::SolveTrialState (Solve.cpp:34-78): with
+ * delta = rhs(newTime) - rhs(oldTime) at the dependent dofs (GetSparseGlobalRhs, zero elsewhere),
+ * x = C (C^T K C)^-1 C^T (f + K delta) - delta (the negative increment). rhs_old / rhs_new: the
+ * equations' right-hand sides at the two times, slot 0's equations then slot 1's (device arrays;
+ * rhs_old NULL means zeros). */
+int nuto_solve_trial_state(nuto_ctx* ctx, const double* K, const double* f, const double* rhs_old,
+                           const double* rhs_new, double* x, const nuto_solver_options* options,
+                           nuto_solve_info* info);
+
 /* Adjoint of BuildVector for the gradient-damage integrand (no counterpart in the reference, which
  * has no derivatives w.r.t. material parameters; its K^T lambda is the transpose of BuildMatrix,
  * SimpleAssembler.cpp:100-156 with GradientDamage::Hessian0, GradientDamage.h:92-120).

@@ -18,6 +18,8 @@ DIM = {QUAD4: 2, HEX8: 3, TET4: 3, TET10: 3, HEX20: 3}
 MOMENTUM_LINEAR_ELASTIC, GRADIENT_DAMAGE, MOMENTUM_LOCAL_DAMAGE = 0, 1, 2
 PLANE_STRESS, PLANE_STRAIN = 0, 1
 FULL, UNILATERAL = 0, 1
+# methods of Context.solve (nuto_solve): EigenConjugateGradient, EigenBiCGSTAB
+SOLVER_CG, SOLVER_BICGSTAB = 0, 1
 # fields of Context.eval_ip (nuto_eval_ip)
 IP_STRAIN, IP_STRESS, IP_KAPPA, IP_DAMAGE, IP_EQ_STRAIN = 0, 1, 2, 3, 4
 
@@ -28,6 +30,15 @@ class NutoError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"nuto_amd error {code} ({_ERRORS.get(code, '?')}): {msg}")
         self.code = code
+
+
+class SolverOptions(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int), ("tolerance", ctypes.c_double), ("max_iterations", ctypes.c_int)]
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("relative_residual", ctypes.c_double),
+                ("independent_dofs", ctypes.c_int)]
 
 
 class Params(ctypes.Structure):
@@ -123,6 +134,10 @@ def lib():
         L.nuto_update_history.argtypes = [vp, vp, vp, vp]
         L.nuto_build_lumped_mass.argtypes = [vp, ctypes.c_double, vp]
         L.nuto_eval_ip.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp]
+        L.nuto_set_constraints.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]
+        L.nuto_solve.argtypes = [vp, vp, vp, vp, ctypes.POINTER(SolverOptions), ctypes.POINTER(SolveInfo)]
+        L.nuto_solve_trial_state.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.POINTER(SolverOptions),
+                                              ctypes.POINTER(SolveInfo)]
         L.nuto_build_vector_matrix.argtypes = [vp, vp, vp, vp, vp, vp]
         L.nuto_device_alloc.argtypes = [vp, ctypes.POINTER(vp), ctypes.c_ulonglong]
         L.nuto_device_free.argtypes = [vp, vp]
@@ -325,6 +340,46 @@ class Context:
         M = self._new(self.n_dofs0) if out is None else out
         self._chk(lib().nuto_build_lumped_mass(self.h, float(rho), _dev_ptr(M)))
         return M
+
+    def set_constraints(self, slot, equations):
+        """Constraints::Add for one dof type slot: equations = [(dependent dof, [(dof, coef), ...]), ...],
+        each reading 1 x[dependent] + sum coef x[dof] = rhs."""
+        dep = np.array([q[0] for q in equations], np.int32)
+        tp = np.zeros(len(equations) + 1, np.int32)
+        td, tc = [], []
+        for i, (_, terms) in enumerate(equations):
+            for d, c in terms:
+                td.append(d)
+                tc.append(c)
+            tp[i + 1] = len(td)
+        td = np.array(td, np.int32)
+        tc = np.array(tc, np.float64)
+        self._chk(lib().nuto_set_constraints(self.h, slot, len(dep), _np_ptr(dep), _np_ptr(tp), _np_ptr(td),
+                                             _np_ptr(tc)))
+
+    def _solve_opts(self, method, tol, max_iter):
+        return SolverOptions(method, tol, max_iter)
+
+    def solve(self, K, f, method=SOLVER_BICGSTAB, tol=0.0, max_iter=0, out=None):
+        """NuTo::Solve: x = C (C^T K C)^-1 C^T f on the device. Returns (x, info dict)."""
+        x = self._new(self.n_rows) if out is None else out
+        info = SolveInfo()
+        opt = self._solve_opts(method, tol, max_iter)
+        self._chk(lib().nuto_solve(self.h, _dev_ptr(K), _dev_ptr(f), _dev_ptr(x), ctypes.byref(opt),
+                                   ctypes.byref(info)))
+        return x, {"iterations": info.iterations, "relative_residual": info.relative_residual,
+                   "independent_dofs": info.independent_dofs}
+
+    def solve_trial_state(self, K, f, rhs_old, rhs_new, method=SOLVER_BICGSTAB, tol=0.0, max_iter=0, out=None):
+        """NuTo::SolveTrialState: the negative increment C (C^T K C)^-1 C^T (f + K delta) - delta."""
+        x = self._new(self.n_rows) if out is None else out
+        info = SolveInfo()
+        opt = self._solve_opts(method, tol, max_iter)
+        self._chk(lib().nuto_solve_trial_state(self.h, _dev_ptr(K), _dev_ptr(f), _dev_ptr(rhs_old),
+                                               _dev_ptr(rhs_new), _dev_ptr(x), ctypes.byref(opt),
+                                               ctypes.byref(info)))
+        return x, {"iterations": info.iterations, "relative_residual": info.relative_residual,
+                   "independent_dofs": info.independent_dofs}
 
     def eval_ip(self, field, u, e=None, kappa=None, out=None):
         """Cell::Eval of one field (IP_STRAIN, IP_STRESS, IP_KAPPA, IP_DAMAGE, IP_EQ_STRAIN) at every

@@ -15,6 +15,7 @@
 #include "gdm_hex8_vjp.hpp"
 #include "batch.hpp"
 #include "cell_fields.hpp"
+#include "solve.hpp"
 #include "mesh.hpp"
 
 using namespace nuto_amd;
@@ -71,6 +72,16 @@ struct nuto_ctx
     // hex8 gradient-damage node kernel (prepared lazily from the node graph)
     GdmHex8Plan gdm;
     bool gdmTried = false;
+    // constraint equations per dof type slot (Constraints::Add) and the unit constraint matrix
+    // built from them for the monolithic system (C and C^T, CSR, device)
+    struct Equations
+    {
+        std::vector<int> dep, termPtr{0}, termDof;
+        std::vector<double> coef;
+    } cons[2];
+    bool consBuilt = false;
+    int nInd = 0, nEq = 0;
+    DevBuf cRow, cCol, cVal, ctRow, ctCol, ctVal, consDep;
 };
 
 namespace
@@ -155,6 +166,7 @@ void InvalidatePattern(nuto_ctx* ctx)
     ctx->patternBuilt = false;
     ctx->gdm.Release();
     ctx->gdmTried = false;
+    ctx->consBuilt = false;
 }
 
 // IP tables: per IP w, dNc[nnC][dim], N[nnD], dNd[nnD][dim] (shape tables of the coordinate
@@ -609,6 +621,7 @@ int nuto_set_dof_type(nuto_ctx* ctx, int slot, int dof_type, int ncomp, int n_do
                 "dof count is not a multiple of the component count");
         InvalidatePattern(ctx);
         ctx->f[slot] = std::move(fd);
+        ctx->cons[slot] = nuto_ctx::Equations();  // the equations refer to the previous numbering
         ctx->tabValid = false;
         if (slot == 0)
             Upload(ctx, ctx->dconn, ctx->f[0].ranks.data(), ctx->f[0].ranks.size(), "dof connectivity");
@@ -834,6 +847,227 @@ int nuto_eval_ip(nuto_ctx* ctx, const double* u, const double* e, const double* 
         co.out = out;
         Dispatch(ctx, a, false, false, 3, nullptr, &co);
     });
+}
+
+int nuto_set_constraints(nuto_ctx* ctx, int slot, int n_eq, const int* dep, const int* term_ptr, const int* term_dof,
+                         const double* coef)
+{
+    return Guard(ctx, [&] {
+        Require(slot == 0 || slot == 1, NUTO_ERR_ARGUMENT, "slot must be 0 or 1");
+        Require(ctx->f[slot].set, NUTO_ERR_STATE, "set the dof type of this slot first");
+        Require(n_eq >= 0 && (n_eq == 0 || (dep && term_ptr)), NUTO_ERR_ARGUMENT, "bad constraint arrays");
+        const int nd = ctx->f[slot].nDofs;
+        nuto_ctx::Equations q;
+        // Constraints::TermChecker (Constraints.h:112-129): a dependent dof appears in no other
+        // equation, and no independent term is a dependent dof of another equation
+        std::vector<char> isDep(nd, 0), isInd(nd, 0);
+        for (int i = 0; i < n_eq; i++)
+        {
+            const int d = dep[i];
+            // Constraints::GetJKNumbering (Constraints.cpp:62-71)
+            Require(d >= 0, NUTO_ERR_ARGUMENT, "There is no dof numbering for a node in equation " + std::to_string(i));
+            Require(d < nd, NUTO_ERR_ARGUMENT,
+                    "The provided dof number of the dependent term exceeds the total number of dofs in equation " +
+                        std::to_string(i));
+            Require(!isDep[d] && !isInd[d], NUTO_ERR_ARGUMENT,
+                    "equation " + std::to_string(i) + ": its dependent dof is already part of another equation");
+            Require(term_ptr[i + 1] >= term_ptr[i] && term_ptr[0] == 0, NUTO_ERR_ARGUMENT, "bad term_ptr");
+            for (int t = term_ptr[i]; t < term_ptr[i + 1]; t++)
+            {
+                const int g = term_dof[t];
+                Require(g >= 0 && g < nd, NUTO_ERR_ARGUMENT,
+                        "There is no dof numbering for a node in equation " + std::to_string(i));
+                Require(!isDep[g] && g != d, NUTO_ERR_ARGUMENT,
+                        "equation " + std::to_string(i) + ": a term is the dependent dof of another equation");
+            }
+            isDep[d] = 1;
+            for (int t = term_ptr[i]; t < term_ptr[i + 1]; t++)
+                isInd[term_dof[t]] = 1;
+            q.dep.push_back(d);
+            for (int t = term_ptr[i]; t < term_ptr[i + 1]; t++)
+            {
+                q.termDof.push_back(term_dof[t]);
+                q.coef.push_back(coef[t]);
+            }
+            q.termPtr.push_back((int)q.termDof.size());
+        }
+        ctx->cons[slot] = std::move(q);
+        ctx->consBuilt = false;
+    });
+}
+
+}  // extern "C"
+
+namespace
+{
+
+// The unit constraint matrix of the monolithic system: block diagonal over the slots, block s =
+// Constraints::BuildUnitConstraintMatrix(dof_s, n_s) (Constraints.cpp:89-137): the JK numbering
+// puts the independent dofs first (ascending), then the dependent ones (equation order); column i
+// < nInd_s holds a 1 at independent dof jk(i), and the row of each dependent dof holds -coef at the
+// independent column of each term with |coef| > 1e-18 (setFromTriplets: duplicates summed in order).
+void BuildConstraints(nuto_ctx* ctx)
+{
+    if (ctx->consBuilt)
+        return;
+    const int nSlots = ctx->f[1].set ? 2 : 1;
+    int n[2] = {ctx->f[0].nDofs, nSlots > 1 ? ctx->f[1].nDofs : 0};
+    Require(n[0] + n[1] == ctx->nRows, NUTO_ERR_STATE, "dof counts do not match the pattern");
+    std::vector<std::vector<std::pair<int, double>>> rows(ctx->nRows);
+    std::vector<int> depAll;
+    int rowOff = 0, colOff = 0;
+    for (int s = 0; s < nSlots; s++)
+    {
+        const nuto_ctx::Equations& q = ctx->cons[s];
+        std::vector<char> isDep(n[s], 0);
+        for (int d : q.dep)
+            isDep[d] = 1;
+        std::vector<int> rev(n[s], -1);
+        int nIndS = 0;
+        for (int g = 0; g < n[s]; g++)
+            if (!isDep[g])
+            {
+                rev[g] = nIndS;
+                rows[rowOff + g].push_back({colOff + nIndS, 1.0});
+                nIndS++;
+            }
+        for (size_t e = 0; e < q.dep.size(); e++)
+        {
+            depAll.push_back(rowOff + q.dep[e]);
+            auto& row = rows[rowOff + q.dep[e]];
+            for (int t = q.termPtr[e]; t < q.termPtr[e + 1]; t++)
+            {
+                if (!(std::fabs(q.coef[t]) > 1.e-18))
+                    continue;
+                const int c = colOff + rev[q.termDof[t]];
+                bool merged = false;
+                for (auto& kv : row)
+                    if (kv.first == c)
+                    {
+                        kv.second += -q.coef[t];
+                        merged = true;
+                    }
+                if (!merged)
+                    row.push_back({c, -q.coef[t]});
+            }
+        }
+        rowOff += n[s];
+        colOff += nIndS;
+    }
+    const int nInd = colOff;
+    std::vector<int> cr(ctx->nRows + 1, 0), cc;
+    std::vector<double> cv;
+    std::vector<std::vector<std::pair<int, double>>> cols(nInd);
+    for (int r = 0; r < ctx->nRows; r++)
+    {
+        std::sort(rows[r].begin(), rows[r].end());
+        for (auto& kv : rows[r])
+        {
+            cc.push_back(kv.first);
+            cv.push_back(kv.second);
+            cols[kv.first].push_back({r, kv.second});
+        }
+        cr[r + 1] = (int)cc.size();
+    }
+    std::vector<int> tr(nInd + 1, 0), tc;
+    std::vector<double> tv;
+    for (int j = 0; j < nInd; j++)
+    {
+        for (auto& kv : cols[j])
+        {
+            tc.push_back(kv.first);
+            tv.push_back(kv.second);
+        }
+        tr[j + 1] = (int)tc.size();
+    }
+    Upload(ctx, ctx->cRow, cr.data(), cr.size(), "constraints");
+    Upload(ctx, ctx->cCol, cc.data(), cc.size(), "constraints");
+    Upload(ctx, ctx->cVal, cv.data(), cv.size(), "constraints");
+    Upload(ctx, ctx->ctRow, tr.data(), tr.size(), "constraints");
+    Upload(ctx, ctx->ctCol, tc.data(), tc.size(), "constraints");
+    Upload(ctx, ctx->ctVal, tv.data(), tv.size(), "constraints");
+    Upload(ctx, ctx->consDep, depAll.data(), depAll.size(), "constraints");
+    ctx->nInd = nInd;
+    ctx->nEq = (int)depAll.size();
+    ctx->consBuilt = true;
+}
+
+// ConstrainedSystemSolver::Solve / SolveTrialState (Solve.cpp:8-78) with delta = NULL / given
+void SolveConstrained(nuto_ctx* ctx, const double* K, const double* f, const double* rhsOld, const double* rhsNew,
+                      bool trial, double* x, const nuto_solver_options* opt, nuto_solve_info* info)
+{
+    Require(ctx->patternBuilt, NUTO_ERR_STATE, "pattern not built (call nuto_build_pattern)");
+    Require(K && f && x, NUTO_ERR_ARGUMENT, "K, f and x must be device arrays");
+    const int method = opt ? opt->method : NUTO_SOLVER_BICGSTAB;
+    Require(method == NUTO_SOLVER_CG || method == NUTO_SOLVER_BICGSTAB, NUTO_ERR_ARGUMENT, "unknown solver");
+    BuildConstraints(ctx);
+    Require(!trial || ctx->nEq == 0 || rhsNew, NUTO_ERR_ARGUMENT, "rhs_new is NULL");
+    const long long n = ctx->nRows;
+    const int m = ctx->nInd;
+    Csr Kc{ctx->nRows, ctx->rowPtr.As<int>(), ctx->col.As<int>(), K};
+    Csr Cc{ctx->nRows, ctx->cRow.As<int>(), ctx->cCol.As<int>(), ctx->cVal.As<double>()};
+    Csr Ct{m, ctx->ctRow.As<int>(), ctx->ctCol.As<int>(), ctx->ctVal.As<double>()};
+    ReducedSolver rs(ctx->stream, Kc, Cc, Ct, n, m);
+    Check(rs.Alloc(), "solver buffers");
+    DevBuf invd, b, u, g, delta;
+    Check(invd.Alloc((size_t)m * 8), "solver buffers");
+    Check(b.Alloc((size_t)m * 8), "solver buffers");
+    Check(u.Alloc((size_t)m * 8), "solver buffers");
+    if (m > 0)
+        hipLaunchKernelGGL(kDiagCtKC, dim3((m + 255) / 256), dim3(256), 0, ctx->stream, m, Ct.row, Ct.col, Ct.val,
+                           Kc.row, Kc.col, Kc.val, invd.As<double>());
+    const double* rhs = f;
+    if (trial)
+    {
+        // g = f + K delta, delta = rhs(new) - rhs(old) at the dependent dofs (GetSparseGlobalRhs)
+        Check(delta.Alloc((size_t)n * 8), "solver buffers");
+        Check(g.Alloc((size_t)n * 8), "solver buffers");
+        rs.Axpby(n, 0.0, nullptr, 0.0, delta.As<double>());
+        if (ctx->nEq > 0)
+            hipLaunchKernelGGL(kScatterDelta, dim3((ctx->nEq + 255) / 256), dim3(256), 0, ctx->stream, ctx->nEq,
+                               ctx->consDep.As<int>(), rhsOld, rhsNew, delta.As<double>());
+        rs.SpmvK(delta.As<double>(), g.As<double>());
+        rs.Axpby(n, 1.0, f, 1.0, g.As<double>());
+        rhs = g.As<double>();
+    }
+    rs.Spmv(Ct, rhs, b.As<double>());
+    Check(hipGetLastError(), "solver setup");
+    const double tol = opt && opt->tolerance > 0.0 ? opt->tolerance : 2.220446049250313e-16;
+    const int maxIt = opt && opt->max_iterations > 0 ? opt->max_iterations : 2 * m;
+    int iters = 0;
+    double rel = 0.0;
+    if (method == NUTO_SOLVER_CG)
+        Check(rs.Cg(b.As<double>(), invd.As<double>(), u.As<double>(), tol, maxIt, iters, rel), "conjugate gradients");
+    else
+        Check(rs.Bicgstab(b.As<double>(), invd.As<double>(), u.As<double>(), tol, maxIt, iters, rel), "BiCGSTAB");
+    rs.Spmv(Cc, u.As<double>(), x);  // x = C u
+    if (trial)
+        rs.Axpby(n, -1.0, delta.As<double>(), 1.0, x);  // the negative increment: C u - delta
+    Check(hipGetLastError(), "solver result");
+    Check(hipStreamSynchronize(ctx->stream), "solve");
+    if (info)
+    {
+        info->iterations = iters;
+        info->relative_residual = rel;
+        info->independent_dofs = m;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int nuto_solve(nuto_ctx* ctx, const double* K, const double* f, double* x, const nuto_solver_options* options,
+               nuto_solve_info* info)
+{
+    return Guard(ctx, [&] { SolveConstrained(ctx, K, f, nullptr, nullptr, false, x, options, info); });
+}
+
+int nuto_solve_trial_state(nuto_ctx* ctx, const double* K, const double* f, const double* rhs_old,
+                           const double* rhs_new, double* x, const nuto_solver_options* options,
+                           nuto_solve_info* info)
+{
+    return Guard(ctx, [&] { SolveConstrained(ctx, K, f, rhs_old, rhs_new, true, x, options, info); });
 }
 
 int nuto_vjp(nuto_ctx* ctx, const double* u, const double* e, const double* kappa, const double* lambda,

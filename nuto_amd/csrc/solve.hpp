@@ -1,0 +1,371 @@
+// Constrained linear solve on the device: ConstrainedSystemSolver::Solve / SolveTrialState
+// (Solve.cpp:8-78) with the unit constraint matrix C of Constraints::BuildUnitConstraintMatrix
+// (Constraints.cpp:89-137).
+//
+// The reduced system (C^T K C) u = C^T f is never formed: the operator is applied as three CSR
+// products (C, K, C^T) with K resident in HBM, and solved with Jacobi-preconditioned conjugate
+// gradients or BiCGSTAB (the iterative solvers NuTo selects by name, EigenSparseSolve.cpp:44-49,
+// whose default preconditioner is the diagonal). Every reduction runs in a fixed order (per-lane
+// strided sums, then fixed trees), so a solve is bit-reproducible.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "devbuf.hpp"
+
+namespace nuto_amd
+{
+
+// y = A x for a CSR matrix, LPR lanes per row: lane l sums the entries l, l + LPR, ... of its row
+// in order, then the LPR partial sums are added by an xor butterfly (a fixed order).
+template <int LPR>
+__global__ void __launch_bounds__(256) kCsrSpmv(int nRows, const int* __restrict__ rowPtr, const int* __restrict__ col,
+                                                const double* __restrict__ val, const double* __restrict__ x,
+                                                double* __restrict__ y)
+{
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long r = t / LPR;
+    const int l = (int)(t % LPR);
+    double s = 0.0;
+    if (r < nRows)
+        for (int k = rowPtr[r] + l; k < rowPtr[r + 1]; k += LPR)
+            s += val[k] * x[col[k]];
+#pragma unroll
+    for (int m = LPR / 2; m >= 1; m >>= 1)
+        s += __shfl_xor(s, m, LPR);
+    if (r < nRows && l == 0)
+        y[r] = s;
+}
+
+// 1 / diag(C^T K C)_j, diag_j = sum over the entries (a, c_a), (b, c_b) of column j of C of
+// c_a c_b K(a, b) (column j of C = row j of C^T; K's columns are ascending in every row: binary
+// search)
+__global__ void __launch_bounds__(256) kDiagCtKC(int nInd, const int* __restrict__ ctRow, const int* __restrict__ ctCol,
+                                                 const double* __restrict__ ctVal, const int* __restrict__ kRow,
+                                                 const int* __restrict__ kCol, const double* __restrict__ kVal,
+                                                 double* __restrict__ d)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= nInd)
+        return;
+    double s = 0.0;
+    for (int p = ctRow[j]; p < ctRow[j + 1]; p++)
+    {
+        const int a = ctCol[p];
+        for (int q = ctRow[j]; q < ctRow[j + 1]; q++)
+        {
+            const int b = ctCol[q];
+            int lo = kRow[a], hi = kRow[a + 1];
+            while (lo < hi)
+            {
+                const int mid = (lo + hi) >> 1;
+                if (kCol[mid] < b)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            if (lo < kRow[a + 1] && kCol[lo] == b)
+                s += ctVal[p] * ctVal[q] * kVal[lo];
+        }
+    }
+    d[j] = s != 0.0 ? 1.0 / s : 1.0;  // the inverse diagonal (1 where the diagonal is zero)
+}
+
+constexpr int kDotBlocks = 1024;
+
+// partial[b] = sum over i = b * 256 + tid + k * gridDim * 256 of a[i] b[i] (k increasing per lane,
+// then a fixed tree over the block's lanes)
+__global__ void __launch_bounds__(256) kDotPartial(long long n, const double* __restrict__ a,
+                                                   const double* __restrict__ b, double* __restrict__ partial)
+{
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        s += a[i] * b[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int m = 128; m > 0; m >>= 1)
+    {
+        if ((int)threadIdx.x < m)
+            sh[threadIdx.x] += sh[threadIdx.x + m];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        partial[blockIdx.x] = sh[0];
+}
+
+// out[slot] = sum of the np partials (one block, same fixed tree)
+__global__ void __launch_bounds__(256) kDotFinal(int np, const double* __restrict__ partial, double* __restrict__ out)
+{
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += 256)
+        s += partial[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int m = 128; m > 0; m >>= 1)
+    {
+        if ((int)threadIdx.x < m)
+            sh[threadIdx.x] += sh[threadIdx.x + m];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        out[0] = sh[0];
+}
+
+// y = a x + b y (x, y of length n); a zero factor leaves its operand unread (y = 0 with a = b = 0)
+__global__ void __launch_bounds__(256) kAxpby(long long n, double a, const double* __restrict__ x, double b,
+                                              double* __restrict__ y)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n)
+        y[i] = (a == 0.0 ? 0.0 : a * x[i]) + (b == 0.0 ? 0.0 : b * y[i]);
+}
+
+// z = invd x (the diagonal preconditioner)
+__global__ void __launch_bounds__(256) kJacobi(long long n, const double* __restrict__ invd,
+                                               const double* __restrict__ x, double* __restrict__ z)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n)
+        z[i] = invd[i] * x[i];
+}
+
+// BiCGSTAB's p = r + beta (p - w v)
+__global__ void __launch_bounds__(256) kBicgP(long long n, const double* __restrict__ r, double beta, double w,
+                                              const double* __restrict__ v, double* __restrict__ p)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n)
+        p[i] = r[i] + beta * (p[i] - w * v[i]);
+}
+
+// delta[dep[e]] = rhsNew[e] - rhsOld[e] (dependent dofs are distinct)
+__global__ void __launch_bounds__(256) kScatterDelta(int nEq, const int* __restrict__ dep,
+                                                     const double* __restrict__ rhsOld,
+                                                     const double* __restrict__ rhsNew, double* __restrict__ delta)
+{
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e < nEq)
+        delta[dep[e]] = rhsNew[e] - (rhsOld ? rhsOld[e] : 0.0);
+}
+
+// Device CSR view
+struct Csr
+{
+    int nRows = 0;
+    const int* row = nullptr;
+    const int* col = nullptr;
+    const double* val = nullptr;
+};
+
+// Krylov solver on the operator C^T K C. Buffers are allocated per solve; the stream orders every
+// launch, and the host reads the scalars it branches on (one small copy per reduction).
+class ReducedSolver
+{
+public:
+    ReducedSolver(hipStream_t s, Csr K, Csr C, Csr Ct, long long nFull, int nInd)
+        : mS(s), mK(K), mC(C), mCt(Ct), mN(nFull), mM(nInd)
+    {
+    }
+
+    // y (nInd) = C^T K C x (nInd)
+    hipError_t Apply(const double* x, double* y)
+    {
+        Spmv(mC, x, t1.As<double>());
+        SpmvK(t1.As<double>(), t2.As<double>());
+        Spmv(mCt, t2.As<double>(), y);
+        return hipGetLastError();
+    }
+
+    hipError_t Alloc()
+    {
+        hipError_t e;
+        if ((e = t1.Alloc(mN * sizeof(double))) != hipSuccess || (e = t2.Alloc(mN * sizeof(double))) != hipSuccess ||
+            (e = part.Alloc(kDotBlocks * sizeof(double))) != hipSuccess ||
+            (e = scal.Alloc(8 * sizeof(double))) != hipSuccess)
+            return e;
+        return hipHostMalloc(&mHost, 8 * sizeof(double), hipHostMallocDefault);
+    }
+    ~ReducedSolver()
+    {
+        if (mHost)
+            (void)hipHostFree(mHost);
+    }
+
+    void Spmv(const Csr& A, const double* x, double* y)
+    {
+        if (A.nRows == 0)
+            return;
+        const long long threads = (long long)A.nRows * 4;
+        hipLaunchKernelGGL(kCsrSpmv<4>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, mS, A.nRows, A.row,
+                           A.col, A.val, x, y);
+    }
+    void SpmvK(const double* x, double* y)
+    {
+        if (mK.nRows == 0)
+            return;
+        const long long threads = (long long)mK.nRows * 16;
+        hipLaunchKernelGGL(kCsrSpmv<16>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, mS, mK.nRows, mK.row,
+                           mK.col, mK.val, x, y);
+    }
+    double Dot(long long n, const double* a, const double* b, hipError_t& err)
+    {
+        const int g = (int)std::min<long long>(kDotBlocks, (n + 255) / 256 > 0 ? (n + 255) / 256 : 1);
+        hipLaunchKernelGGL(kDotPartial, dim3(g), dim3(256), 0, mS, n, a, b, part.As<double>());
+        hipLaunchKernelGGL(kDotFinal, dim3(1), dim3(256), 0, mS, g, part.As<const double>(), scal.As<double>());
+        if ((err = hipGetLastError()) != hipSuccess ||
+            (err = hipMemcpyAsync(mHost, scal.p, sizeof(double), hipMemcpyDeviceToHost, mS)) != hipSuccess ||
+            (err = hipStreamSynchronize(mS)) != hipSuccess)
+            return 0.0;
+        return mHost[0];
+    }
+    void Axpby(long long n, double a, const double* x, double b, double* y)
+    {
+        if (n > 0)
+            hipLaunchKernelGGL(kAxpby, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, mS, n, a, x, b, y);
+    }
+    void Precond(long long n, const double* d, const double* x, double* z)
+    {
+        if (n > 0)
+            hipLaunchKernelGGL(kJacobi, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, mS, n, d, x, z);
+    }
+
+    // Preconditioned conjugate gradients from x = 0 on A = C^T K C: stops when |r|^2 < tol^2 |b|^2
+    // or after maxIt iterations; `iters` counts the iterations before the converging one.
+    hipError_t Cg(const double* b, const double* invd, double* x, double tol, int maxIt, int& iters, double& relRes)
+    {
+        const long long m = mM;
+        DevBuf r, z, p, q;
+        hipError_t err;
+        if ((err = r.Alloc(m * 8)) != hipSuccess || (err = z.Alloc(m * 8)) != hipSuccess ||
+            (err = p.Alloc(m * 8)) != hipSuccess || (err = q.Alloc(m * 8)) != hipSuccess)
+            return err;
+        double *R = r.As<double>(), *Z = z.As<double>(), *P = p.As<double>(), *Q = q.As<double>();
+        iters = 0;
+        relRes = 0.0;
+        Axpby(m, 0.0, nullptr, 0.0, x);
+        const double bb = Dot(m, b, b, err);
+        if (err != hipSuccess)
+            return err;
+        if (bb == 0.0)
+            return hipSuccess;
+        const double thr = std::max(tol * tol * bb, 1e-300);
+        if ((err = hipMemcpyAsync(R, b, m * 8, hipMemcpyDeviceToDevice, mS)) != hipSuccess)
+            return err;
+        double rr = bb;
+        if (rr < thr)
+            return hipSuccess;
+        Precond(m, invd, R, P);
+        double absNew = Dot(m, R, P, err);
+        int i = 0;
+        while (i < maxIt && err == hipSuccess)
+        {
+            if ((err = Apply(P, Q)) != hipSuccess)
+                break;
+            const double alpha = absNew / Dot(m, P, Q, err);
+            Axpby(m, alpha, P, 1.0, x);
+            Axpby(m, -alpha, Q, 1.0, R);
+            rr = Dot(m, R, R, err);
+            if (rr < thr)
+                break;
+            Precond(m, invd, R, Z);
+            const double absOld = absNew;
+            absNew = Dot(m, R, Z, err);
+            Axpby(m, 1.0, Z, absNew / absOld, P);
+            i++;
+        }
+        iters = i;
+        relRes = std::sqrt(rr / bb);
+        return err;
+    }
+
+    // Preconditioned BiCGSTAB from x = 0 on A = C^T K C (with a restart when rho becomes tiny)
+    hipError_t Bicgstab(const double* b, const double* invd, double* x, double tol, int maxIt, int& iters,
+                        double& relRes)
+    {
+        const long long m = mM;
+        DevBuf bufs[8];
+        hipError_t err;
+        for (DevBuf& d : bufs)
+            if ((err = d.Alloc(m * 8)) != hipSuccess)
+                return err;
+        double *R = bufs[0].As<double>(), *R0 = bufs[1].As<double>(), *P = bufs[2].As<double>(),
+               *V = bufs[3].As<double>(), *Y = bufs[4].As<double>(), *Z = bufs[5].As<double>(),
+               *S = bufs[6].As<double>(), *T = bufs[7].As<double>();
+        iters = 0;
+        relRes = 0.0;
+        Axpby(m, 0.0, nullptr, 0.0, x);
+        const double bb = Dot(m, b, b, err);
+        if (err != hipSuccess)
+            return err;
+        if (bb == 0.0)
+            return hipSuccess;
+        if ((err = hipMemcpyAsync(R, b, m * 8, hipMemcpyDeviceToDevice, mS)) != hipSuccess ||
+            (err = hipMemcpyAsync(R0, b, m * 8, hipMemcpyDeviceToDevice, mS)) != hipSuccess)
+            return err;
+        Axpby(m, 0.0, nullptr, 0.0, P);
+        Axpby(m, 0.0, nullptr, 0.0, V);
+        double r0sq = bb, rho = 1.0, alpha = 1.0, w = 1.0;
+        const double tol2 = tol * tol * bb, eps2 = 2.220446049250313e-16 * 2.220446049250313e-16;
+        double rr = bb;
+        int i = 0, restarts = 0;
+        while (rr > tol2 && i < maxIt && err == hipSuccess)
+        {
+            const double rhoOld = rho;
+            rho = Dot(m, R0, R, err);
+            if (std::fabs(rho) < eps2 * r0sq)
+            {
+                // restart with the current residual r = b - A x
+                if ((err = Apply(x, T)) != hipSuccess)
+                    break;
+                if ((err = hipMemcpyAsync(R, b, m * 8, hipMemcpyDeviceToDevice, mS)) != hipSuccess)
+                    break;
+                Axpby(m, -1.0, T, 1.0, R);
+                if ((err = hipMemcpyAsync(R0, R, m * 8, hipMemcpyDeviceToDevice, mS)) != hipSuccess)
+                    break;
+                rho = r0sq = Dot(m, R, R, err);
+                if (restarts++ == 0)
+                    i = 0;
+            }
+            const double beta = (rho / rhoOld) * (alpha / w);
+            hipLaunchKernelGGL(kBicgP, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, mS, m, R, beta, w, V, P);
+            Precond(m, invd, P, Y);
+            if ((err = Apply(Y, V)) != hipSuccess)
+                break;
+            alpha = rho / Dot(m, R0, V, err);
+            if ((err = hipMemcpyAsync(S, R, m * 8, hipMemcpyDeviceToDevice, mS)) != hipSuccess)
+                break;
+            Axpby(m, -alpha, V, 1.0, S);
+            Precond(m, invd, S, Z);
+            if ((err = Apply(Z, T)) != hipSuccess)
+                break;
+            const double tt = Dot(m, T, T, err);
+            w = tt > 0.0 ? Dot(m, T, S, err) / tt : 0.0;
+            Axpby(m, alpha, Y, 1.0, x);
+            Axpby(m, w, Z, 1.0, x);
+            if ((err = hipMemcpyAsync(R, S, m * 8, hipMemcpyDeviceToDevice, mS)) != hipSuccess)
+                break;
+            Axpby(m, -w, T, 1.0, R);
+            rr = Dot(m, R, R, err);
+            ++i;
+        }
+        iters = i;
+        relRes = std::sqrt(rr / bb);
+        return err;
+    }
+
+    DevBuf t1, t2, part, scal;
+
+private:
+    hipStream_t mS;
+    Csr mK, mC, mCt;
+    long long mN;
+    int mM;
+    double* mHost = nullptr;
+};
+
+}  // namespace nuto_amd

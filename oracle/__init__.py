@@ -83,6 +83,8 @@ def lib():
         L.orc_problem_cell_dofs.argtypes = [P, ctypes.c_int, ip]
         L.orc_set_mutation.argtypes = [ctypes.c_int]
         L.orc_problem_lumped_mass.argtypes = [P, ctypes.c_double, dp, P]
+        L.orc_unit_constraint_matrix.argtypes = [ctypes.c_int, ctypes.c_int, ip, ip, ip, dp, P]
+        L.orc_solve_constrained.argtypes = [ctypes.c_int, ip, ip, ip, ip, ip, dp, ip, ip, dp, dp, ctypes.c_int, P, P, dp]
         L.orc_problem_eval_ip.argtypes = [P, P, P, P, ctypes.c_int, P]
         L.orc_mises.restype = ctypes.c_double
         L.orc_damage_exp.restype = ctypes.c_double
@@ -348,3 +350,46 @@ def csr_to_dense(rp, ci, v, n, m=None):
 def set_mutation(m):
     """Plant mistake m in the oracle (0: none); for tests that show the parity rule rejects it."""
     lib().orc_set_mutation(int(m))
+
+
+def _eq_arrays(equations):
+    """[(dep, [(dof, coef), ...]), ...] -> (dep, term_ptr, term_dof, coef) int32 / float64 arrays."""
+    dep = np.array([e[0] for e in equations], np.int32)
+    tp = np.zeros(len(equations) + 1, np.int32)
+    td, tc = [], []
+    for i, (_, terms) in enumerate(equations):
+        for d, c in terms:
+            td.append(d)
+            tc.append(c)
+        tp[i + 1] = len(td)
+    return dep, tp, np.array(td, np.int32), np.array(tc, np.float64)
+
+
+def unit_constraint_matrix(n_dofs, equations):
+    """Constraints::BuildUnitConstraintMatrix of one dof type (dense n_dofs x n_independent)."""
+    dep, tp, td, tc = _eq_arrays(equations)
+    n_ind = lib().orc_unit_constraint_matrix(n_dofs, len(dep), dep, tp, td, tc, None)
+    if n_ind < 0:
+        raise ValueError("dependent dof out of range")
+    C = np.zeros((n_dofs, n_ind))
+    lib().orc_unit_constraint_matrix(n_dofs, len(dep), dep, tp, td, tc, _ptr(C))
+    return C
+
+
+def solve_constrained(n_dofs, equations, row_ptr, col, val, f, rhs_old=None, rhs_new=None):
+    """NuTo::Solve (rhs_new None) or NuTo::SolveTrialState on the monolithic system: n_dofs and
+    equations per slot ([slot0, slot1]), K in CSR, f of length sum(n_dofs)."""
+    eqs = [e for s in equations for e in s]
+    dep, tp, td, tc = _eq_arrays(eqs)
+    n = np.array(n_dofs, np.int32)
+    ne = np.array([len(s) for s in equations], np.int32)
+    x = np.zeros(int(n.sum()))
+    trial = rhs_new is not None
+    ro = None if rhs_old is None else np.ascontiguousarray(rhs_old, np.float64)
+    rn = None if rhs_new is None else np.ascontiguousarray(rhs_new, np.float64)
+    m = lib().orc_solve_constrained(len(n_dofs), n, ne, dep, tp, td, tc, np.ascontiguousarray(row_ptr, np.int32),
+                                    np.ascontiguousarray(col, np.int32), np.ascontiguousarray(val, np.float64),
+                                    np.ascontiguousarray(f, np.float64), int(trial), _ptr(ro), _ptr(rn), x)
+    if m < 0:
+        raise ValueError("dependent dof out of range")
+    return x

@@ -1607,6 +1607,79 @@ static int EvalIp(const Problem& P, const double* u, const double* e, const doub
     return ncomp;
 }
 
+// ---------------------------------------------------------------------------------------------
+// constraints and the constrained solve
+// ---------------------------------------------------------------------------------------------
+// Constraints::BuildUnitConstraintMatrix (Constraints.cpp:89-137) for one dof type, dense
+// row-major nDofs x nInd: JK numbering (GetJKNumbering, Constraints.cpp:53-86) = independent dofs
+// ascending, then the dependent dofs in equation order; a 1 at (jk(i), i) for i < nInd; -coef at
+// (dependent dof, reverseJK(term dof)) for every term with |coef| > 1e-18, duplicates summed.
+// Returns nInd, or -1 for a dependent dof out of range.
+static int UnitConstraintMatrix(int nDofs, int nEq, const int* dep, const int* termPtr, const int* termDof,
+                                const double* coef, std::vector<double>& C)
+{
+    std::vector<char> isDep(nDofs, 0);
+    for (int e = 0; e < nEq; e++)
+    {
+        if (dep[e] < 0 || dep[e] >= nDofs)
+            return -1;
+        isDep[dep[e]] = 1;
+    }
+    std::vector<int> jk, rev(nDofs, -1);
+    for (int g = 0; g < nDofs; g++)
+        if (!isDep[g])
+            jk.push_back(g);
+    const int nInd = (int)jk.size();
+    for (int e = 0; e < nEq; e++)
+        jk.push_back(dep[e]);
+    for (int i = 0; i < nDofs; i++)
+        rev[jk[i]] = i;
+    C.assign((size_t)nDofs * nInd, 0.0);
+    for (int i = 0; i < nInd; i++)
+        C[(size_t)jk[i] * nInd + i] += 1.0;
+    for (int e = 0; e < nEq; e++)
+        for (int t = termPtr[e]; t < termPtr[e + 1]; t++)
+            if (std::abs(coef[t]) > 1.e-18)
+                C[(size_t)dep[e] * nInd + rev[termDof[t]]] += -coef[t];
+    return nInd;
+}
+
+// Dense solve of A x = b (n x n, row-major) by Gaussian elimination with partial pivoting.
+static void DenseSolve(int n, std::vector<double> A, std::vector<double> b, std::vector<double>& x)
+{
+    for (int k = 0; k < n; k++)
+    {
+        int p = k;
+        for (int i = k + 1; i < n; i++)
+            if (std::abs(A[(size_t)i * n + k]) > std::abs(A[(size_t)p * n + k]))
+                p = i;
+        if (p != k)
+        {
+            for (int j = 0; j < n; j++)
+                std::swap(A[(size_t)k * n + j], A[(size_t)p * n + j]);
+            std::swap(b[k], b[p]);
+        }
+        const double piv = A[(size_t)k * n + k];
+        for (int i = k + 1; i < n; i++)
+        {
+            const double l = A[(size_t)i * n + k] / piv;
+            if (l == 0.0)
+                continue;
+            for (int j = k; j < n; j++)
+                A[(size_t)i * n + j] -= l * A[(size_t)k * n + j];
+            b[i] -= l * b[k];
+        }
+    }
+    x.assign(n, 0.0);
+    for (int i = n - 1; i >= 0; i--)
+    {
+        double s = b[i];
+        for (int j = i + 1; j < n; j++)
+            s -= A[(size_t)i * n + j] * x[j];
+        x[i] = s / A[(size_t)i * n + i];
+    }
+}
+
 }  // namespace orc
 
 // =============================================================================================
@@ -1817,6 +1890,97 @@ void* orc_problem(void* mp, const Params* p, int ruleKind, int ruleN, int layer0
         P->ndof[t] = (int)P->numbers[t].size();
     }
     return P;
+}
+// Constraints::BuildUnitConstraintMatrix of one dof type, dense row-major nDofs x nInd into C
+// (NULL: query nInd); returns nInd or -1.
+int orc_unit_constraint_matrix(int nDofs, int nEq, const int* dep, const int* termPtr, const int* termDof,
+                               const double* coef, double* C)
+{
+    std::vector<double> c;
+    const int nInd = UnitConstraintMatrix(nDofs, nEq, dep, termPtr, termDof, coef, c);
+    if (nInd >= 0 && C)
+        std::copy(c.begin(), c.end(), C);
+    return nInd;
+}
+
+// NuTo::Solve (trial 0) / NuTo::SolveTrialState (trial 1) (Solve.cpp:8-78) on the monolithic system
+// of nSlots dof types (n[s] dofs each, rows [slot 0; slot 1]): C block diagonal of the slots' unit
+// constraint matrices, Kmod = C^T K C, fmod = C^T f (trial: C^T (f + K delta), delta = rhsNew -
+// rhsOld at the dependent dofs), u by a direct solve, x = C u (trial: - delta). Equations of the
+// slots are concatenated: nEq[s] each, termPtr global (first entry 0). K in CSR.
+// Returns the number of independent dofs, or -1.
+int orc_solve_constrained(int nSlots, const int* n, const int* nEq, const int* dep, const int* termPtr,
+                          const int* termDof, const double* coef, const int* kRow, const int* kCol, const double* kVal,
+                          const double* f, int trial, const double* rhsOld, const double* rhsNew, double* x)
+{
+    int N = 0;
+    for (int s = 0; s < nSlots; s++)
+        N += n[s];
+    std::vector<std::vector<double>> Cs(nSlots);
+    std::vector<int> nInd(nSlots), eqOff(nSlots + 1, 0);
+    int M = 0;
+    for (int s = 0; s < nSlots; s++)
+    {
+        eqOff[s + 1] = eqOff[s] + nEq[s];
+        // shift the slot's term pointers to start at 0
+        std::vector<int> tp(nEq[s] + 1);
+        for (int e = 0; e <= nEq[s]; e++)
+            tp[e] = termPtr[eqOff[s] + e] - termPtr[eqOff[s]];
+        nInd[s] = UnitConstraintMatrix(n[s], nEq[s], dep + eqOff[s], tp.data(), termDof + termPtr[eqOff[s]],
+                                       coef + termPtr[eqOff[s]], Cs[s]);
+        if (nInd[s] < 0)
+            return -1;
+        M += nInd[s];
+    }
+    std::vector<double> C((size_t)N * M, 0.0);  // monolithic, block diagonal
+    for (int s = 0, ro = 0, co = 0; s < nSlots; ro += n[s], co += nInd[s], s++)
+        for (int i = 0; i < n[s]; i++)
+            for (int j = 0; j < nInd[s]; j++)
+                C[(size_t)(ro + i) * M + co + j] = Cs[s][(size_t)i * nInd[s] + j];
+    std::vector<double> delta(N, 0.0), g(f, f + N);
+    if (trial)
+    {
+        for (int s = 0, ro = 0; s < nSlots; ro += n[s], s++)
+            for (int e = 0; e < nEq[s]; e++)
+            {
+                const int k = eqOff[s] + e;
+                delta[ro + dep[k]] = rhsNew[k] - (rhsOld ? rhsOld[k] : 0.0);
+            }
+        for (int r = 0; r < N; r++)
+            for (int k = kRow[r]; k < kRow[r + 1]; k++)
+                g[r] += kVal[k] * delta[kCol[k]];
+    }
+    // KC (N x M), Kmod = C^T (K C), fmod = C^T g
+    std::vector<double> KC((size_t)N * M, 0.0);
+    for (int r = 0; r < N; r++)
+        for (int k = kRow[r]; k < kRow[r + 1]; k++)
+        {
+            const double v = kVal[k];
+            const double* crow = &C[(size_t)kCol[k] * M];
+            for (int j = 0; j < M; j++)
+                KC[(size_t)r * M + j] += v * crow[j];
+        }
+    std::vector<double> Kmod((size_t)M * M, 0.0), fmod(M, 0.0);
+    for (int r = 0; r < N; r++)
+        for (int i = 0; i < M; i++)
+        {
+            const double c = C[(size_t)r * M + i];
+            if (c == 0.0)
+                continue;
+            fmod[i] += c * g[r];
+            for (int j = 0; j < M; j++)
+                Kmod[(size_t)i * M + j] += c * KC[(size_t)r * M + j];
+        }
+    std::vector<double> u;
+    DenseSolve(M, Kmod, fmod, u);
+    for (int r = 0; r < N; r++)
+    {
+        double s = 0.0;
+        for (int j = 0; j < M; j++)
+            s += C[(size_t)r * M + j] * u[j];
+        x[r] = s - (trial ? delta[r] : 0.0);
+    }
+    return M;
 }
 // lumped mass of dof type 0 (length ndof[0]) and its magnitude scale
 void orc_problem_lumped_mass(void* Pp, double rho, double* M, double* Mb)
