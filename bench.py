@@ -67,6 +67,43 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes_read": 8 * n, "copy_calibration_bytes_written": 8 * n}), flush=True)
 
 
+def solve_bench(ctx, X, C, ranks, nn, K, R, iters, length):
+    """ConstrainedSystemSolver::SolveTrialState on the assembled system: bottom face clamped, the top
+    face's vertical displacement stepped by 1e-3 mm, eeq free; BiCGSTAB with the diagonal
+    preconditioner for a fixed number of iterations (the tolerance is out of reach), timed after one
+    untimed solve that builds the constraint matrix."""
+    import numpy as np
+    import torch
+
+    import nuto_amd as N
+
+    xs = np.zeros((nn, 3))
+    xs[ranks.ravel()] = X[C.ravel()]
+    bottom = np.nonzero(np.abs(xs[:, 2]) < 1e-9)[0]
+    top = np.nonzero(np.abs(xs[:, 2] - length) < 1e-9)[0]
+    eqs = [(int(3 * r + c), []) for r in bottom for c in range(3)] + [(int(3 * r + 2), []) for r in top]
+    ctx.set_constraints(0, eqs)
+    rhs = np.zeros(len(eqs))
+    rhs[3 * len(bottom):] = 1e-3
+    rhs_d = torch.from_numpy(rhs).cuda()
+    x = torch.empty(ctx.n_rows, dtype=torch.float64, device="cuda")
+    ctx.solve_trial_state(K, R, None, rhs_d, method=N.SOLVER_BICGSTAB, tol=1e-300, max_iter=2, out=x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, info = ctx.solve_trial_state(K, R, None, rhs_d, method=N.SOLVER_BICGSTAB, tol=1e-300, max_iter=iters, out=x)
+    torch.cuda.synchronize()
+    ms = 1000.0 * (time.perf_counter() - t0)
+    spmv_bytes = 12 * ctx.nnz + 16 * ctx.n_rows  # K values + columns, x gathered and y written once
+    per_it = ms / max(info["iterations"], 1)
+    return {"workload": f"hex8 gradient damage {len(C)} cells: bottom face clamped, top face stepped, "
+                        "BiCGSTAB + diagonal preconditioner (SolveTrialState), fixed iterations",
+            "equations": len(eqs), "independent_dofs": info["independent_dofs"],
+            "iterations": info["iterations"], "ms": ms, "ms_per_iteration": per_it,
+            "k_products_per_iteration": 2,
+            "k_product_GBps_lower_bound": 2 * spmv_bytes / (per_it * 1e-3) / 1e9,
+            "relative_residual": info["relative_residual"]}
+
+
 def adjoint_ratio(n, steps, warmup):
     """Adjoint vs forward on the differentiation workload: hex8 gradient damage on an n^3 mesh of an
     n mm cube with per-cell fields E ~ LN(30000, 0.1), f_t ~ LN(3, 0.1), G_f ~ LN(0.1, 0.15);
@@ -248,6 +285,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--adjoint-n", type=int, default=64, help="cells per direction of the adjoint workload")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--solve-iters", type=int, default=20,
+                    help="BiCGSTAB iterations of the constrained solve on the assembled system (0: skip)")
     ap.add_argument("--batch-meshes", type=int, default=256, help="meshes of the mixed-type batch (0: skip)")
     ap.add_argument("--batch-gdm-meshes", type=int, default=64,
                     help="meshes of the gradient-damage residual + adjoint batch (0: skip)")
@@ -343,6 +382,8 @@ def main():
         tflops = flops / (kernel_ms * 1e-3) / 1e12
         rec["fp64"] = {"flops_per_step": flops, "achieved_tflops": tflops, "peak_tflops": FP64_PEAK_TFLOPS,
                        "frac": tflops / FP64_PEAK_TFLOPS}
+    if args.solve_iters > 0:
+        rec["solve"] = solve_bench(ctx, X, C, ranks, nn, K, R, args.solve_iters, 100.0 * n / 100)
     if not args.no_adjoint:
         del R, K
         torch.cuda.empty_cache()

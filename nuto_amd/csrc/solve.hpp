@@ -196,12 +196,12 @@ public:
             (void)hipHostFree(mHost);
     }
 
+    // C and C^T: one or two entries per row, one lane per row
     void Spmv(const Csr& A, const double* x, double* y)
     {
         if (A.nRows == 0)
             return;
-        const long long threads = (long long)A.nRows * 4;
-        hipLaunchKernelGGL(kCsrSpmv<4>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, mS, A.nRows, A.row,
+        hipLaunchKernelGGL(kCsrSpmv<1>, dim3((unsigned)((A.nRows + 255) / 256)), dim3(256), 0, mS, A.nRows, A.row,
                            A.col, A.val, x, y);
     }
     void SpmvK(const double* x, double* y)
