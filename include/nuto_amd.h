@@ -278,6 +278,49 @@ int nuto_device_free(nuto_ctx* ctx, void* ptr);
 int nuto_copy_to_device(nuto_ctx* ctx, void* dst, const void* src, unsigned long long bytes);
 int nuto_copy_to_host(nuto_ctx* ctx, void* dst, const void* src, unsigned long long bytes);
 
+/* ---- mesh input: gmsh MSH 2 files (host only) ------------------------------------------------
+ * MeshGmsh (MeshGmsh.h:17-60, MeshGmsh.cpp:440-550): ASCII and binary files, $PhysicalNames,
+ * $Nodes and $Elements; nodes and elements in file order, point elements dropped, node orders of
+ * gmsh types 8, 10 and 12 changed to the interpolations' (MeshGmsh.cpp:398-436), elements with
+ * det J <= 0 at the natural origin rejected (MeshGmsh.cpp:85-91). Mesh dimension: 3 if any
+ * z != 0, else 2 if any y != 0, else 1 (MeshGmsh.cpp:130-147). */
+typedef struct nuto_gmsh nuto_gmsh;
+/* Reads path into *mesh. *mesh is set also on failure: read the message with
+ * nuto_gmsh_last_error, then free it. */
+int nuto_gmsh_read(const char* path, nuto_gmsh** mesh);
+void nuto_gmsh_free(nuto_gmsh* mesh);
+const char* nuto_gmsh_last_error(const nuto_gmsh* mesh);
+int nuto_gmsh_sizes(const nuto_gmsh* mesh, int* dim, int* n_nodes, int* n_elements, int* conn_size);
+/* coords: n_nodes x dim */
+int nuto_gmsh_nodes(const nuto_gmsh* mesh, double* coords);
+/* gmsh element type, CSR offsets (n_elements + 1) into conn (0-based node indices), first tag;
+ * any pointer may be NULL */
+int nuto_gmsh_elements(const nuto_gmsh* mesh, int* gmsh_type, int* offsets, int* conn, int* physical);
+/* MeshGmsh::GetPhysicalGroup(int) (MeshGmsh.cpp:26-32): element indices of a physical group;
+ * elements may be NULL to query n. An unknown id is an error. */
+int nuto_gmsh_group(nuto_gmsh* mesh, int id, int* n, int* elements);
+/* MeshGmsh::GetPhysicalGroup(std::string) (MeshGmsh.cpp:34-43): id of a named group, name
+ * matched case-insensitively. */
+int nuto_gmsh_group_id(nuto_gmsh* mesh, const char* name, int* id);
+/* Connectivity (n_cells x nodes per cell) of the elements of a group (group < 0: all elements)
+ * that are of the gmsh type of coord_type (quad4: 3, hex8: 5, tet4: 4, tet10: 11), ready for
+ * nuto_set_mesh's conn; conn may be NULL to query n_cells. */
+int nuto_gmsh_cells(nuto_gmsh* mesh, int group, int coord_type, int* n_cells, int* conn);
+
+/* ---- Morton (Z-order) codes (NuTo::MortonOrder, MortonOrder.h:11-107) ------------------------
+ * 3D: 10 bits per axis, x in bit 0, y in bit 1, z in bit 2 (EncodeMorton3D, DecodeMorton3X/Y/Z);
+ * 2D: 16 bits per axis, x in bit 0 (Part1By1, DecodeMorton2X/Y). Inputs are cut to those bits. */
+unsigned nuto_morton_encode3d(unsigned x, unsigned y, unsigned z);
+unsigned nuto_morton_encode2d(unsigned x, unsigned y);
+void nuto_morton_decode3d(unsigned code, unsigned* x, unsigned* y, unsigned* z);
+void nuto_morton_decode2d(unsigned code, unsigned* x, unsigned* y);
+/* Neighbor3D / NegNeighbor3D: the code of the cell shifted by (dx, dy, dz); 0xffffffff when a
+ * coordinate would turn negative. */
+unsigned nuto_morton_neighbor3d(unsigned key, int dx, int dy, int dz);
+/* The permutation that sorts n points (n x dim, dim 1..3) along a Morton curve over their bounding
+ * box, quantised to 2^10 steps per axis in 3D and 2^16 in 1D and 2D; ties keep the input order. */
+int nuto_morton_order(int n, int dim, const double* coords, int* order);
+
 #ifdef __cplusplus
 }
 #endif

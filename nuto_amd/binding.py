@@ -143,6 +143,30 @@ def lib():
         L.nuto_device_free.argtypes = [vp, vp]
         L.nuto_copy_to_device.argtypes = [vp, vp, vp, ctypes.c_ulonglong]
         L.nuto_copy_to_host.argtypes = [vp, vp, vp, ctypes.c_ulonglong]
+        gp = ctypes.c_void_p
+        L.nuto_gmsh_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(gp)]
+        L.nuto_gmsh_free.argtypes = [gp]
+        L.nuto_gmsh_free.restype = None
+        L.nuto_gmsh_last_error.argtypes = [gp]
+        L.nuto_gmsh_last_error.restype = ctypes.c_char_p
+        L.nuto_gmsh_sizes.argtypes = [gp, ip, ip, ip, ip]
+        L.nuto_gmsh_nodes.argtypes = [gp, vp]
+        L.nuto_gmsh_elements.argtypes = [gp, vp, vp, vp, vp]
+        L.nuto_gmsh_group.argtypes = [gp, ctypes.c_int, ip, vp]
+        L.nuto_gmsh_group_id.argtypes = [gp, ctypes.c_char_p, ip]
+        L.nuto_gmsh_cells.argtypes = [gp, ctypes.c_int, ctypes.c_int, ip, vp]
+        u = ctypes.c_uint
+        L.nuto_morton_encode3d.argtypes = [u, u, u]
+        L.nuto_morton_encode3d.restype = u
+        L.nuto_morton_encode2d.argtypes = [u, u]
+        L.nuto_morton_encode2d.restype = u
+        L.nuto_morton_decode3d.argtypes = [u, ctypes.POINTER(u), ctypes.POINTER(u), ctypes.POINTER(u)]
+        L.nuto_morton_decode3d.restype = None
+        L.nuto_morton_decode2d.argtypes = [u, ctypes.POINTER(u), ctypes.POINTER(u)]
+        L.nuto_morton_decode2d.restype = None
+        L.nuto_morton_neighbor3d.argtypes = [u, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.nuto_morton_neighbor3d.restype = u
+        L.nuto_morton_order.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp]
         _lib = L
     return _lib
 
@@ -199,6 +223,127 @@ def cell_dofs(ranks, ncomp):
     ranks = np.asarray(ranks, np.int32)
     return (ranks[:, :, None] * ncomp + np.arange(ncomp, dtype=np.int32)[None, None, :]).reshape(
         ranks.shape[0], -1).astype(np.int32)
+
+
+# ---- mesh input and Morton codes (host) ---------------------------------------------------------
+GMSH_CELL_TYPE = {QUAD4: 3, HEX8: 5, TET4: 4, TET10: 11}  # gmsh element type of each coordinate element
+
+
+class GmshMesh:
+    """A gmsh MSH 2 file read by nuto_gmsh_read (MeshGmsh, MeshGmsh.cpp:440-550).
+
+    coords: n_nodes x dim (file order); elements: gmsh types, CSR offsets and 0-based node indices
+    (node orders of the interpolations); physical: first tag of each element.
+    Reading errors raise NutoError with MeshGmsh's message.
+    """
+
+    def __init__(self, path):
+        L = lib()
+        self._h = ctypes.c_void_p()
+        r = L.nuto_gmsh_read(os.fsencode(path), ctypes.byref(self._h))
+        if r:
+            msg = L.nuto_gmsh_last_error(self._h).decode()
+            L.nuto_gmsh_free(self._h)
+            self._h = None
+            raise NutoError(r, msg)
+        dim, nn, ne, nconn = (ctypes.c_int() for _ in range(4))
+        L.nuto_gmsh_sizes(self._h, *(ctypes.byref(v) for v in (dim, nn, ne, nconn)))
+        self.dim = dim.value
+        self.coords = np.zeros((nn.value, self.dim))
+        L.nuto_gmsh_nodes(self._h, _np_ptr(self.coords))
+        self.gmsh_type = np.zeros(ne.value, np.int32)
+        self.offsets = np.zeros(ne.value + 1, np.int32)
+        self.conn = np.zeros(nconn.value, np.int32)
+        self.physical = np.zeros(ne.value, np.int32)
+        L.nuto_gmsh_elements(self._h, *(_np_ptr(a) for a in (self.gmsh_type, self.offsets, self.conn, self.physical)))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().nuto_gmsh_free(self._h)
+            self._h = None
+
+    def _check(self, r):
+        if r:
+            raise NutoError(r, lib().nuto_gmsh_last_error(self._h).decode())
+
+    @property
+    def n_nodes(self):
+        return self.coords.shape[0]
+
+    @property
+    def n_elements(self):
+        return self.gmsh_type.shape[0]
+
+    def element_nodes(self, i):
+        return self.conn[self.offsets[i]:self.offsets[i + 1]]
+
+    def physical_group(self, group):
+        """Element indices of a physical group, by id or (case-insensitive) name
+        (MeshGmsh::GetPhysicalGroup, MeshGmsh.cpp:446-461)."""
+        if isinstance(group, str):
+            gid = ctypes.c_int()
+            self._check(lib().nuto_gmsh_group_id(self._h, group.encode(), ctypes.byref(gid)))
+            group = gid.value
+        n = ctypes.c_int()
+        self._check(lib().nuto_gmsh_group(self._h, int(group), ctypes.byref(n), None))
+        out = np.zeros(n.value, np.int32)
+        self._check(lib().nuto_gmsh_group(self._h, int(group), ctypes.byref(n), _np_ptr(out)))
+        return out
+
+    def cells(self, coord_type, group=None):
+        """(coords, conn) of the elements of coord_type (of a group, by id or name), ready for
+        Context.set_cells (nuto_set_cells)."""
+        if isinstance(group, str):
+            gid = ctypes.c_int()
+            self._check(lib().nuto_gmsh_group_id(self._h, group.encode(), ctypes.byref(gid)))
+            group = gid.value
+        g = -1 if group is None else int(group)
+        n = ctypes.c_int()
+        self._check(lib().nuto_gmsh_cells(self._h, g, coord_type, ctypes.byref(n), None))
+        conn = np.zeros((n.value, NUM_NODES[coord_type]), np.int32)
+        self._check(lib().nuto_gmsh_cells(self._h, g, coord_type, ctypes.byref(n), _np_ptr(conn)))
+        return self.coords, conn
+
+
+def morton_encode3d(x, y, z):
+    """MortonOrder::EncodeMorton3D (MortonOrder.h:14-17): 10 bits per axis, x lowest."""
+    return lib().nuto_morton_encode3d(x, y, z)
+
+
+def morton_encode2d(x, y):
+    """The 2D code whose DecodeMorton2X/Y (MortonOrder.h:56-64) are x and y: 16 bits per axis."""
+    return lib().nuto_morton_encode2d(x, y)
+
+
+def morton_decode3d(code):
+    """MortonOrder::DecodeMorton3X/Y/Z (MortonOrder.h:66-79)."""
+    v = [ctypes.c_uint() for _ in range(3)]
+    lib().nuto_morton_decode3d(code, *(ctypes.byref(a) for a in v))
+    return tuple(a.value for a in v)
+
+
+def morton_decode2d(code):
+    """MortonOrder::DecodeMorton2X/Y (MortonOrder.h:56-64)."""
+    v = [ctypes.c_uint() for _ in range(2)]
+    lib().nuto_morton_decode2d(code, *(ctypes.byref(a) for a in v))
+    return tuple(a.value for a in v)
+
+
+def morton_neighbor3d(key, dx, dy, dz):
+    """MortonOrder::NegNeighbor3D (MortonOrder.h:91-103); 0xffffffff below zero."""
+    return lib().nuto_morton_neighbor3d(key, dx, dy, dz)
+
+
+def morton_order(coords):
+    """Permutation sorting points (n x dim) along a Morton curve over their bounding box."""
+    X = np.ascontiguousarray(coords, np.float64)
+    if X.ndim == 1:
+        X = X[:, None]
+    order = np.zeros(X.shape[0], np.int32)
+    r = lib().nuto_morton_order(X.shape[0], X.shape[1], _np_ptr(X), _np_ptr(order))
+    if r:
+        raise NutoError(r, "invalid points")
+    return order
 
 
 class Context:

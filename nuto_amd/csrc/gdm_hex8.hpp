@@ -42,6 +42,7 @@
 #include "devbuf.hpp"
 #include "laws.hpp"
 #include "mesh.hpp"
+#include "morton.hpp"
 
 namespace nuto_amd
 {
@@ -1101,9 +1102,9 @@ __global__ void __launch_bounds__(256) kGdmHex8CountNonAffine(const double* geo,
         atomicAdd(count, 1);
 }
 
-// Nodes sorted along a Morton (Z-order) curve of their coordinates: each node takes the
-// coordinates of its first incident cell's coordinate node (hex8 dof nodes are the coordinate
-// nodes), quantised to 2^10 steps per axis over the bounding box; ties keep node order.
+// Nodes sorted along a Morton (Z-order) curve of their coordinates (MortonSort): each node takes
+// the coordinates of its first incident cell's coordinate node (hex8 dof nodes are the coordinate
+// nodes); nodes without cells go first.
 inline hipError_t MortonOrder(const NodeGraph& g, int nCells, int nCoord, const double* x, const double* y,
                               const double* z, const int* cconn, hipStream_t stream, std::vector<int>& order)
 {
@@ -1118,50 +1119,20 @@ inline hipError_t MortonOrder(const NodeGraph& g, int nCells, int nCoord, const 
         (err = hipStreamSynchronize(stream)) != hipSuccess)
         return err;
     const int n = g.nNodes;
-    std::vector<int> cn(n, -1);
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    std::vector<double> pts((size_t)n * 3, 0.0);
+    std::vector<char> valid(n, 0);
     for (int p = 0; p < n; p++)
     {
         if (g.incPtr[p + 1] == g.incPtr[p])
             continue;
         const int k = g.incPtr[p];
-        cn[p] = hc[(size_t)g.incCell[k] * 8 + g.incLocal[k]];
-        const double v[3] = {hx[cn[p]], hy[cn[p]], hz[cn[p]]};
-        for (int d = 0; d < 3; d++)
-        {
-            lo[d] = std::min(lo[d], v[d]);
-            hi[d] = std::max(hi[d], v[d]);
-        }
+        const int c = hc[(size_t)g.incCell[k] * 8 + g.incLocal[k]];
+        pts[(size_t)p * 3] = hx[c];
+        pts[(size_t)p * 3 + 1] = hy[c];
+        pts[(size_t)p * 3 + 2] = hz[c];
+        valid[p] = 1;
     }
-    auto spread = [](unsigned long long v) {  // 10 bits -> every third of 30
-        v &= 0x3ff;
-        v = (v | (v << 16)) & 0x30000ffULL;
-        v = (v | (v << 8)) & 0x300f00fULL;
-        v = (v | (v << 4)) & 0x30c30c3ULL;
-        v = (v | (v << 2)) & 0x9249249ULL;
-        return v;
-    };
-    std::vector<unsigned long long> key(n);
-    for (int p = 0; p < n; p++)
-    {
-        unsigned long long m = 0;
-        if (cn[p] >= 0)
-        {
-            const double v[3] = {hx[cn[p]], hy[cn[p]], hz[cn[p]]};
-            for (int d = 0; d < 3; d++)
-            {
-                const double w = hi[d] - lo[d];
-                const double t = w > 0.0 ? (v[d] - lo[d]) / w : 0.0;
-                const unsigned long long q = (unsigned long long)std::min(1023.0, std::max(0.0, t * 1024.0));
-                m |= spread(q) << d;
-            }
-        }
-        key[p] = (m << 32) | (unsigned long long)p;
-    }
-    std::sort(key.begin(), key.end());
-    order.resize(n);
-    for (int i = 0; i < n; i++)
-        order[i] = (int)(key[i] & 0xffffffffULL);
+    MortonSort(n, 3, pts.data(), order, valid.data());
     return hipSuccess;
 }
 

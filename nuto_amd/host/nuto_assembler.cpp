@@ -260,5 +260,70 @@ std::vector<double> SimpleAssembler::Eval(const CellGroup& cells, const Momentum
     return RunEval({f.disp}, state, nullptr, cells.NumCells(), dim, field);
 }
 
+// ---- MeshGmsh ----------------------------------------------------------------------------------
+MeshGmsh::MeshGmsh(const std::string& fileName)
+{
+    const int r = nuto_gmsh_read(fileName.c_str(), &mMesh);
+    if (r != NUTO_OK)
+    {
+        const std::string msg = mMesh ? nuto_gmsh_last_error(mMesh) : "out of memory";
+        nuto_gmsh_free(mMesh);
+        mMesh = nullptr;
+        throw Exception("MeshGmsh::MeshGmsh", msg);
+    }
+    int nNodes = 0, nElements = 0, connSize = 0;
+    nuto_gmsh_sizes(mMesh, &mDim, &nNodes, &nElements, &connSize);
+    mCoords.resize((size_t)nNodes * mDim);
+    mType.resize(nElements);
+    mOffsets.resize(nElements + 1);
+    mConn.resize(connSize);
+    nuto_gmsh_nodes(mMesh, mCoords.data());
+    nuto_gmsh_elements(mMesh, mType.data(), mOffsets.data(), mConn.data(), nullptr);
+}
+
+MeshGmsh::~MeshGmsh()
+{
+    nuto_gmsh_free(mMesh);
+}
+
+std::vector<int> MeshGmsh::GetPhysicalGroup(int physicalGroupId) const
+{
+    int n = 0;
+    if (nuto_gmsh_group(mMesh, physicalGroupId, &n, nullptr) != NUTO_OK)
+        throw Exception("MeshGmsh::GetPhysicalGroup", nuto_gmsh_last_error(mMesh));
+    std::vector<int> elements(n);
+    nuto_gmsh_group(mMesh, physicalGroupId, &n, elements.data());
+    return elements;
+}
+
+std::vector<int> MeshGmsh::GetPhysicalGroup(std::string physicalName) const
+{
+    int id = 0;
+    if (nuto_gmsh_group_id(mMesh, physicalName.c_str(), &id) != NUTO_OK)
+        throw Exception("MeshGmsh::GetPhysicalGroup", nuto_gmsh_last_error(mMesh));
+    return GetPhysicalGroup(id);
+}
+
+std::vector<int> MeshGmsh::ElementNodes(int element) const
+{
+    if (element < 0 || element >= NumElements())
+        throw Exception("MeshGmsh::ElementNodes", "element index out of range");
+    return std::vector<int>(mConn.begin() + mOffsets[element], mConn.begin() + mOffsets[element + 1]);
+}
+
+CellGroup MeshGmsh::Cells(int coordType, int physicalGroupId) const
+{
+    int n = 0;
+    if (nuto_gmsh_cells(mMesh, physicalGroupId, coordType, &n, nullptr) != NUTO_OK)
+        throw Exception("MeshGmsh::Cells", nuto_gmsh_last_error(mMesh));
+    const int nodes[] = {4, 8, 4, 10, 20};
+    CellGroup g;
+    g.coordType = coordType;
+    g.coords = mCoords;
+    g.conn.resize((size_t)n * nodes[coordType]);
+    nuto_gmsh_cells(mMesh, physicalGroupId, coordType, &n, g.conn.data());
+    return g;
+}
+
 }  // namespace host
 }  // namespace nuto_amd

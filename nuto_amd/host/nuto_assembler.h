@@ -141,6 +141,39 @@ enum class IpField
     EqStrain = NUTO_IP_EQ_STRAIN  // ModifiedMisesStrainNorm::Value(strain)
 };
 
+// MeshGmsh (MeshGmsh.h:16-60, MeshGmsh.cpp:440-550): a gmsh MSH 2 file (ASCII or binary) read
+// through nuto_gmsh_read; reading errors throw Exception with MeshGmsh's messages. Elements are
+// indexed in file order (point elements dropped); GetPhysicalGroup returns element indices.
+class MeshGmsh
+{
+public:
+    explicit MeshGmsh(const std::string& fileName);
+    ~MeshGmsh();
+    MeshGmsh(const MeshGmsh&) = delete;
+    MeshGmsh& operator=(const MeshGmsh&) = delete;
+
+    // MeshGmsh::GetPhysicalGroup(int) / (std::string): unknown ids and names throw
+    std::vector<int> GetPhysicalGroup(int physicalGroupId) const;
+    std::vector<int> GetPhysicalGroup(std::string physicalName) const;
+
+    int Dimension() const { return mDim; }
+    int NumNodes() const { return (int)mCoords.size() / mDim; }
+    int NumElements() const { return (int)mType.size(); }
+    const std::vector<double>& Coordinates() const { return mCoords; }  // nodes x dim, file order
+    int GmshType(int element) const { return mType[element]; }
+    std::vector<int> ElementNodes(int element) const;  // 0-based, interpolation node order
+
+    // the elements of coordType's gmsh type (quad4, hex8, tet4, tet10) of a group (< 0: all) as a
+    // cell group: coordinates and connectivity, no dof numbering yet
+    CellGroup Cells(int coordType, int physicalGroupId = -1) const;
+
+private:
+    nuto_gmsh* mMesh = nullptr;
+    int mDim = 1;
+    std::vector<double> mCoords;
+    std::vector<int> mType, mOffsets, mConn;
+};
+
 class SimpleAssembler
 {
 public:

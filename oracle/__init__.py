@@ -198,7 +198,7 @@ class Mesh:
         return cls(lib().orc_mesh_from_arrays(coord_type, X.shape[0], X, conn.shape[0], conn))
 
     def __del__(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and lib is not None:  # module globals are gone at interpreter exit
             lib().orc_mesh_free(self.h)
             self.h = None
 
@@ -252,7 +252,7 @@ class Problem:
         self.ncells = mesh.info()["n_cells"]
 
     def __del__(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and lib is not None:  # module globals are gone at interpreter exit
             lib().orc_problem_free(self.h)
             self.h = None
 

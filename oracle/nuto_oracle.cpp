@@ -493,6 +493,9 @@ struct Jacobian
 {
     int d = 0;
     double J[9] = {0}, inv[9] = {0}, det = 0;
+    // magnitude shadow of inv: the adjugate's products taken in absolute value, over |det|. An
+    // inverse entry that is a cancellation result is compared against the size of its terms.
+    double invMag[9] = {0};
 };
 
 static Jacobian MakeJacobian(int dim, int n, const double* X /* node-major */, const double* dN /* n x dim */)
@@ -517,6 +520,8 @@ static Jacobian MakeJacobian(int dim, int n, const double* X /* node-major */, c
         jac.inv[1] = -m[1] * id;
         jac.inv[3] = m[0] * id;
         jac.det = det;
+        for (int i = 0; i < 4; i++)
+            jac.invMag[i] = std::abs(jac.inv[i]);
     }
     else
     {
@@ -531,6 +536,13 @@ static Jacobian MakeJacobian(int dim, int n, const double* X /* node-major */, c
         for (int i = 0; i < 3; i++)
             for (int j = 0; j < 3; j++)
                 jac.inv[i * 3 + j] = (i == 0 ? c0[j] : cof(j, i)) * id;
+        auto cofMag = [&](int i, int j) {
+            int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            return std::abs(M(i1, j1) * M(i2, j2)) + std::abs(M(i1, j2) * M(i2, j1));
+        };
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                jac.invMag[i * 3 + j] = cofMag(j, i) * std::abs(id);
         auto h = [&](int a, int b, int c) { return M(0, a) * (M(1, b) * M(2, c) - M(1, c) * M(2, b)); };
         jac.det = h(0, 1, 2) - h(1, 0, 2) + h(2, 0, 1);
     }
@@ -548,6 +560,24 @@ static Mat GlobalDerivatives(const Jacobian& jac, const double* dN, int n)
             double s = 0.;
             for (int i = 0; i < d; i++)
                 s += dN[k * d + i] * jac.inv[i * d + j];
+            o(k, j) = s;
+        }
+    return o;
+}
+
+// magnitude shadow of dN/dX: |dN/dxi| * |J^-1| (shadow), so that a derivative that cancels to
+// (nearly) zero, e.g. the barycentric gradient of a tetrahedron's vertex opposite a face that lies
+// in a coordinate plane, is compared against the size of its terms
+static Mat GlobalDerivativesMag(const Jacobian& jac, const double* dN, int n)
+{
+    int d = jac.d;
+    Mat o(n, d);
+    for (int k = 0; k < n; k++)
+        for (int j = 0; j < d; j++)
+        {
+            double s = 0.;
+            for (int i = 0; i < d; i++)
+                s += std::abs(dN[k * d + i]) * jac.invMag[i * d + j];
             o(k, j) = s;
         }
     return o;
@@ -1119,7 +1149,7 @@ static void IntegrateCell(const Problem& P, int c, const double* u, const double
 
         // per dof type: N, dN/dX
         std::vector<double> Nv[2];
-        Mat dNdX[2];
+        Mat dNdX[2], dNdXm[2];
         for (int t = 0; t < P.ntypes; t++)
         {
             int ty = m.dofs[P.layer[t]].type;
@@ -1128,6 +1158,7 @@ static void IntegrateCell(const Problem& P, int c, const double* u, const double
             double dN[60];
             DShapeDim(ty, xi, dN);
             dNdX[t] = GlobalDerivatives(jac, dN, nnD[t]);
+            dNdXm[t] = GlobalDerivativesMag(jac, dN, nnD[t]);
         }
         std::vector<double> Rip[2], Rmag[2];
         Mat Kip[2][2], Kmag[2][2];
@@ -1135,9 +1166,10 @@ static void IntegrateCell(const Problem& P, int c, const double* u, const double
         if (P.p.kind == 0 || P.p.kind == 2)
         {
             Mat B = StrainB(dNdX[0]);
+            const Mat Bm = StrainB(dNdXm[0]);  // shadow of B
             Mat strainM = Mul(B, Col(nodeVal[0]));
             const double* strain = strainM.v.data();
-            Mat strainMag = MulAbs(B, Col(nodeVal[0]));
+            Mat strainMag = MulAbs(Bm, Col(nodeVal[0]));
             if (P.p.kind == 0)
             {
                 // MomentumBalance::Gradient/Hessian0 (MomentumBalance.h:26-46) with LinearElastic
@@ -1145,12 +1177,12 @@ static void IntegrateCell(const Problem& P, int c, const double* u, const double
                 if (wantR)
                 {
                     Rip[0] = Mul(T(B), Mul(C, strainM)).v;
-                    Rmag[0] = MulAbs(T(B), MulAbs(C, strainMag)).v;
+                    Rmag[0] = MulAbs(T(Bm), MulAbs(C, strainMag)).v;
                 }
                 if (wantK)
                 {
                     Kip[0][0] = Mul(Mul(T(B), C), B);
-                    Kmag[0][0] = MulAbs(MulAbs(T(B), C), B);
+                    Kmag[0][0] = MulAbs(MulAbs(T(Bm), C), Bm);
                 }
             }
             else
@@ -1168,7 +1200,7 @@ static void IntegrateCell(const Problem& P, int c, const double* u, const double
                 if (wantR)
                 {
                     Rip[0] = Mul(T(B), Col(led.Stress(strain, omega))).v;
-                    Rmag[0] = MulAbs(T(B), Col(std::vector<double>(lm.sig, lm.sig + 6))).v;
+                    Rmag[0] = MulAbs(T(Bm), Col(std::vector<double>(lm.sig, lm.sig + 6))).v;
                 }
                 if (wantK)
                 {
@@ -1188,7 +1220,7 @@ static void IntegrateCell(const Problem& P, int c, const double* u, const double
                     for (int i = 0; i < 6; i++)
                         for (int j = 0; j < 6; j++)
                             tm(i, j) = std::abs(D(i, j)) + (active ? lm.dsw[i] * std::abs(dw) * lm.m[j] : 0.);
-                    Kmag[0][0] = MulAbs(MulAbs(T(B), tm), B);
+                    Kmag[0][0] = MulAbs(MulAbs(T(Bm), tm), Bm);
                 }
             }
         }
@@ -1200,6 +1232,7 @@ static void IntegrateCell(const Problem& P, int c, const double* u, const double
             DamageExp dl{law.kappa0, law.beta, P.p.alpha};
             Mat Bd = StrainB(dNdX[0]);
             Mat Be = T(dNdX[1]);  // Nabla::Gradient
+            const Mat Bdm = StrainB(dNdXm[0]), Bem = T(dNdXm[1]);  // shadows
             Mat Ne(1, nnD[1]);
             Ne.v = Nv[1];
             double eeq = Mul(Ne, Col(nodeVal[1])).v[0];
@@ -1213,7 +1246,7 @@ static void IntegrateCell(const Problem& P, int c, const double* u, const double
                     strainM.v[i] *= 0.5;
             const double* strain = strainM.v.data();
             const double g = 1., dgdw = 0.;
-            Mat strainMag = MulAbs(Bd, Col(nodeVal[0]));
+            Mat strainMag = MulAbs(Bdm, Col(nodeVal[0]));
             LawMag lm = LawMagnitudes(strainMag.v.data(), omega, law.E, P.p.nu, P.p.k, P.p.damage_application,
                                       MisesA(strain, P.p.nu, P.p.k));
             if (wantR)
@@ -1222,9 +1255,9 @@ static void IntegrateCell(const Problem& P, int c, const double* u, const double
                 Mat r1 = Scale(T(Ne), eeq - norm.Value(strain));
                 Mat r2 = Mul(T(Be), Scale(eeqGrad, (g_mutation == 5 ? -1.0 : 1.0) * P.p.c * g));
                 Rip[1] = Add(r1, r2).v;
-                Rmag[0] = MulAbs(T(Bd), Col(std::vector<double>(lm.sig, lm.sig + 6))).v;
+                Rmag[0] = MulAbs(T(Bdm), Col(std::vector<double>(lm.sig, lm.sig + 6))).v;
                 Mat m1 = Scale(Abs(T(Ne)), std::abs(eeq) + lm.eeq);
-                Mat m2 = MulAbs(T(Be), Scale(MulAbs(Be, Col(nodeVal[1])), P.p.c));
+                Mat m2 = MulAbs(T(Bem), Scale(MulAbs(Bem, Col(nodeVal[1])), P.p.c));
                 Rmag[1] = Add(m1, m2).v;
             }
             if (wantK)
@@ -1240,13 +1273,13 @@ static void IntegrateCell(const Problem& P, int c, const double* u, const double
                 Kip[1][1] = Add(ee, ee3);
                 Mat dsw = Col(led.DstressDomega(strain));
                 Kip[0][1] = Mul(Mul(T(Bd), Scale(Scale(dsw, dwdk), dkde)), Ne);
-                Kmag[0][0] = MulAbs(MulAbs(T(Bd), led.DstressDstrain(strain, omega)), Bd);
+                Kmag[0][0] = MulAbs(MulAbs(T(Bdm), led.DstressDstrain(strain, omega)), Bdm);
                 Mat mm(1, 6);
                 mm.v.assign(lm.m, lm.m + 6);
-                Kmag[1][0] = MulAbs(MulAbs(T(Ne), mm), Bd);
-                Kmag[1][1] = Add(MulAbs(T(Ne), Ne), Scale(MulAbs(T(Be), Be), P.p.c));
+                Kmag[1][0] = MulAbs(MulAbs(T(Ne), mm), Bdm);
+                Kmag[1][1] = Add(MulAbs(T(Ne), Ne), Scale(MulAbs(T(Bem), Bem), P.p.c));
                 Mat dm = Col(std::vector<double>(lm.dsw, lm.dsw + 6));
-                Kmag[0][1] = MulAbs(MulAbs(T(Bd), Scale(dm, std::abs(dwdk) * dkde)), Ne);
+                Kmag[0][1] = MulAbs(MulAbs(T(Bdm), Scale(dm, std::abs(dwdk) * dkde)), Ne);
             }
         }
         // result += (f * detJ) * w ; first IP assigns ("uninitialized addition")
