@@ -27,6 +27,7 @@ extern "C" {
 #define NUTO_ERR_DEVICE 3      /* HIP runtime error */
 #define NUTO_ERR_MEMORY 4      /* device or host allocation failed */
 #define NUTO_ERR_UNSUPPORTED 5 /* combination not provided by these kernels */
+#define NUTO_ERR_NO_CONVERGENCE 6 /* NewtonRaphson::NoConvergence (NewtonRaphson.h:12-20) */
 
 /* element types (interpolations): InterpolationQuadLinear, InterpolationBrickLinear,
  * InterpolationTetrahedronLinear, InterpolationTetrahedronQuadratic, InterpolationBrickQuadratic */
@@ -207,6 +208,33 @@ int nuto_solve(nuto_ctx* ctx, const double* K, const double* f, double* x, const
 int nuto_solve_trial_state(nuto_ctx* ctx, const double* K, const double* f, const double* rhs_old,
                            const double* rhs_new, double* x, const nuto_solver_options* options,
                            nuto_solve_info* info);
+
+/* One load step of QuasistaticSolver::DoStep (QuasistaticSolver.cpp:116-147) on the device: the
+ * trial state x - SolveTrialState(K(x), R(x), rhs_old, rhs_new) from the last converged state
+ * (TrialState, :52-68), then NewtonRaphson::Solve (NewtonRaphson.h:62-94) with LineSearch
+ * (LineSearch.h:36-62: x = x0 - alpha dx, alpha halved up to max_line_search_steps times, accepted
+ * once |R| < (1 - alpha) |R0|) or, with max_line_search_steps == 0, NoLineSearch; the residual norm
+ * is |C^T R| (QuasistaticSolver::Norm, :86-92). On convergence the history is updated
+ * (UpdateHistory, :80-83) and x is overwritten; otherwise x and kappa are left unchanged and the
+ * call returns NUTO_ERR_NO_CONVERGENCE, which AdaptiveSolve (AdaptiveSolve.cpp:18-63) answers by
+ * halving the time step. x: the monolithic dof vector [u; e] (n_rows, device); kappa: the history
+ * (device, NULL for linear elasticity); rhs_old / rhs_new as for nuto_solve_trial_state. */
+typedef struct nuto_newton_options
+{
+    double tolerance;          /* |C^T R| below which the step has converged (QuasistaticSolver::mTolerance, 1e-10 if <= 0) */
+    int max_iterations;        /* Newton iterations (DoStep passes 6; 6 if <= 0) */
+    int max_line_search_steps; /* LineSearch steps (6 by default in DoStep); 0: NoLineSearch; < 0: 6 */
+    nuto_solver_options linear;/* the solver of every linear system */
+} nuto_newton_options;
+typedef struct nuto_newton_info
+{
+    int iterations;            /* DoStep's return value: Newton iterations to convergence */
+    int linear_iterations;     /* Krylov iterations over all linear solves of the step */
+    int residual_evaluations;  /* assemblies of R (trial state excluded) */
+    double residual_norm;      /* |C^T R| at the returned state */
+} nuto_newton_info;
+int nuto_quasistatic_step(nuto_ctx* ctx, double* x, double* kappa, const double* rhs_old, const double* rhs_new,
+                          const nuto_newton_options* options, nuto_newton_info* info);
 
 /* Adjoint of BuildVector for the gradient-damage integrand (no counterpart in the reference, which
  * has no derivatives w.r.t. material parameters; its K^T lambda is the transpose of BuildMatrix,

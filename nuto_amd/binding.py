@@ -20,10 +20,11 @@ PLANE_STRESS, PLANE_STRAIN = 0, 1
 FULL, UNILATERAL = 0, 1
 # methods of Context.solve (nuto_solve): EigenConjugateGradient, EigenBiCGSTAB
 SOLVER_CG, SOLVER_BICGSTAB = 0, 1
+NO_CONVERGENCE = 6  # NutoError.code of nuto_quasistatic_step without convergence (NewtonRaphson::NoConvergence)
 # fields of Context.eval_ip (nuto_eval_ip)
 IP_STRAIN, IP_STRESS, IP_KAPPA, IP_DAMAGE, IP_EQ_STRAIN = 0, 1, 2, 3, 4
 
-_ERRORS = {1: "argument", 2: "state", 3: "device", 4: "memory", 5: "unsupported"}
+_ERRORS = {1: "argument", 2: "state", 3: "device", 4: "memory", 5: "unsupported", 6: "no convergence"}
 
 
 class NutoError(RuntimeError):
@@ -39,6 +40,16 @@ class SolverOptions(ctypes.Structure):
 class SolveInfo(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("relative_residual", ctypes.c_double),
                 ("independent_dofs", ctypes.c_int)]
+
+
+class NewtonOptions(ctypes.Structure):
+    _fields_ = [("tolerance", ctypes.c_double), ("max_iterations", ctypes.c_int),
+                ("max_line_search_steps", ctypes.c_int), ("linear", SolverOptions)]
+
+
+class NewtonInfo(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("linear_iterations", ctypes.c_int),
+                ("residual_evaluations", ctypes.c_int), ("residual_norm", ctypes.c_double)]
 
 
 class Params(ctypes.Structure):
@@ -136,6 +147,8 @@ def lib():
         L.nuto_eval_ip.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp]
         L.nuto_set_constraints.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]
         L.nuto_solve.argtypes = [vp, vp, vp, vp, ctypes.POINTER(SolverOptions), ctypes.POINTER(SolveInfo)]
+        L.nuto_quasistatic_step.argtypes = [vp, vp, vp, vp, vp, ctypes.POINTER(NewtonOptions),
+                                            ctypes.POINTER(NewtonInfo)]
         L.nuto_solve_trial_state.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.POINTER(SolverOptions),
                                               ctypes.POINTER(SolveInfo)]
         L.nuto_build_vector_matrix.argtypes = [vp, vp, vp, vp, vp, vp]
@@ -346,6 +359,40 @@ def morton_order(coords):
     return order
 
 
+def adaptive_solve(do_step, t_end, t_start=0.0, post_process=None, dt=0.1, dt_max=0.1, dt_min=1e-6,
+                   increase_factor=1.5, decrease_factor=0.5):
+    """AdaptiveSolve::Solve (AdaptiveSolve.cpp:18-63): calls do_step(t + dt), which returns the Newton
+    iterations (Context.quasistatic_step's "iterations") or raises NutoError with code NO_CONVERGENCE;
+    on failure dt is halved (below dt_min: an error), after a step with fewer than 3 iterations dt
+    grows by increase_factor up to dt_max, and the last step ends exactly at t_end. post_process(t)
+    runs at t_start and after every converged step. Returns the list of converged times."""
+    t = t_start
+    times = []
+    if post_process:
+        post_process(t)
+    while True:
+        try:
+            iterations = do_step(t + dt)
+        except NutoError as e:
+            if e.code != NO_CONVERGENCE:
+                raise
+            dt *= decrease_factor
+            if dt < dt_min:
+                raise NutoError(NO_CONVERGENCE, f"Time step smaller than prescribed minimal time step {dt_min}.")
+            continue
+        t += dt
+        times.append(t)
+        if post_process:
+            post_process(t)
+        if t >= t_end:
+            break
+        if iterations < 3 and dt < dt_max:
+            dt = min(dt * increase_factor, dt_max)
+        if t + dt > t_end:
+            dt = t_end - t
+    return times
+
+
 class Context:
     """One assembly context on one device (the C ABI nuto_ctx)."""
 
@@ -525,6 +572,20 @@ class Context:
                                                ctypes.byref(info)))
         return x, {"iterations": info.iterations, "relative_residual": info.relative_residual,
                    "independent_dofs": info.independent_dofs}
+
+    def quasistatic_step(self, x, kappa=None, rhs_old=None, rhs_new=None, tol=1e-10, max_iter=6,
+                         line_search_steps=6, method=SOLVER_BICGSTAB, linear_tol=0.0, linear_max_iter=0):
+        """QuasistaticSolver::DoStep on the device: trial state, Newton-Raphson with line search
+        (line_search_steps=0: none), history update. x ([u; e], n_rows) and kappa are updated in place
+        on convergence; otherwise NutoError with code NO_CONVERGENCE is raised and both are unchanged.
+        Returns the info dict (iterations is DoStep's return value)."""
+        o = NewtonOptions(float(tol), int(max_iter), int(line_search_steps),
+                          SolverOptions(int(method), float(linear_tol), int(linear_max_iter)))
+        info = NewtonInfo()
+        self._chk(lib().nuto_quasistatic_step(self.h, _dev_ptr(x), _dev_ptr(kappa), _dev_ptr(rhs_old),
+                                              _dev_ptr(rhs_new), ctypes.byref(o), ctypes.byref(info)))
+        return {"iterations": info.iterations, "linear_iterations": info.linear_iterations,
+                "residual_evaluations": info.residual_evaluations, "residual_norm": info.residual_norm}
 
     def eval_ip(self, field, u, e=None, kappa=None, out=None):
         """Cell::Eval of one field (IP_STRAIN, IP_STRESS, IP_KAPPA, IP_DAMAGE, IP_EQ_STRAIN) at every

@@ -1053,6 +1053,121 @@ void SolveConstrained(nuto_ctx* ctx, const double* K, const double* f, const dou
     }
 }
 
+// QuasistaticSolver::DoStep (QuasistaticSolver.cpp:116-147) with NewtonRaphson::Solve
+// (NewtonRaphson.h:62-94) and LineSearch / NoLineSearch (LineSearch.h:36-62,78-88). Every vector
+// stays on the device; the host reads one norm per residual evaluation.
+void QuasistaticStep(nuto_ctx* ctx, double* x, double* kappa, const double* rhsOld, const double* rhsNew,
+                     const nuto_newton_options* o, nuto_newton_info* info)
+{
+    Require(ctx->patternBuilt, NUTO_ERR_STATE, "pattern not built (call nuto_build_pattern)");
+    Require(x != nullptr, NUTO_ERR_ARGUMENT, "x is NULL");
+    Require(ctx->params.kind == kLinearElastic || kappa != nullptr, NUTO_ERR_ARGUMENT,
+            "this integrand keeps a history: kappa is NULL");
+    const double tol = o && o->tolerance > 0.0 ? o->tolerance : 1e-10;
+    const int maxIt = o && o->max_iterations > 0 ? o->max_iterations : 6;
+    const int lsMax = !o || o->max_line_search_steps < 0 ? 6 : o->max_line_search_steps;
+    const nuto_solver_options lin = o ? o->linear : nuto_solver_options{NUTO_SOLVER_BICGSTAB, 0.0, 0};
+    BuildConstraints(ctx);
+    const long long n = ctx->nRows;
+    const int m = ctx->nInd;
+    const long long nU = ctx->f[0].nDofs;
+    const bool two = ctx->params.kind == kGradientDamage;
+    double* hist = ctx->params.kind == kLinearElastic ? nullptr : kappa;
+
+    DevBuf K, R, dx, xk, x0, ctr;
+    Check(K.Alloc((size_t)ctx->nnz * 8), "Newton buffers");
+    Check(R.Alloc((size_t)n * 8), "Newton buffers");
+    Check(dx.Alloc((size_t)n * 8), "Newton buffers");
+    Check(xk.Alloc((size_t)n * 8), "Newton buffers");
+    Check(x0.Alloc((size_t)n * 8), "Newton buffers");
+    Check(ctr.Alloc((size_t)std::max(m, 1) * 8), "Newton buffers");
+    Csr Kc{ctx->nRows, ctx->rowPtr.As<int>(), ctx->col.As<int>(), K.As<double>()};
+    Csr Cc{ctx->nRows, ctx->cRow.As<int>(), ctx->cCol.As<int>(), ctx->cVal.As<double>()};
+    Csr Ct{m, ctx->ctRow.As<int>(), ctx->ctCol.As<int>(), ctx->ctVal.As<double>()};
+    ReducedSolver rs(ctx->stream, Kc, Cc, Ct, n, m);
+    Check(rs.Alloc(), "Newton buffers");
+
+    auto E = [&](const double* v) -> const double* { return two ? v + nU : nullptr; };
+    // QuasistaticSolver::Norm: |C^T v|
+    auto Norm = [&](const double* v) {
+        rs.Spmv(Ct, v, ctr.As<double>());
+        hipError_t err = hipSuccess;
+        const double d = m > 0 ? rs.Dot(m, ctr.As<double>(), ctr.As<double>(), err) : 0.0;
+        Check(err, "residual norm");
+        return std::sqrt(d);
+    };
+    int linIts = 0, evals = 0;
+    auto LinearSolve = [&](const double* Kv, const double* f, const double* ro, const double* rn, bool trial) {
+        nuto_solve_info li{};
+        SolveConstrained(ctx, Kv, f, ro, rn, trial, dx.As<double>(), &lin, &li);
+        linIts += li.iterations;
+    };
+    auto Residual = [&](const double* v) {
+        Assemble(ctx, v, E(v), hist, R.As<double>(), nullptr, true, false);
+        evals++;
+        return Norm(R.As<double>());
+    };
+
+    // TrialState: hessian and gradient at the last converged state, x - SolveTrialState(...)
+    Assemble(ctx, x, E(x), hist, R.As<double>(), K.As<double>(), true, true);
+    LinearSolve(K.As<double>(), R.As<double>(), rhsOld, rhsNew, true);
+    Check(hipMemcpyAsync(xk.p, x, (size_t)n * 8, hipMemcpyDeviceToDevice, ctx->stream), "trial state");
+    rs.Axpby(n, -1.0, dx.As<double>(), 1.0, xk.As<double>());
+
+    // NewtonRaphson::Solve
+    double norm = Residual(xk.As<double>());
+    int it = 0;
+    bool converged = norm < tol;
+    while (!converged && it < maxIt)
+    {
+        Assemble(ctx, xk.As<double>(), E(xk.As<double>()), hist, nullptr, K.As<double>(), false, true);
+        LinearSolve(K.As<double>(), R.As<double>(), nullptr, nullptr, false);
+        ++it;
+        if (lsMax == 0)
+        {
+            rs.Axpby(n, -1.0, dx.As<double>(), 1.0, xk.As<double>());
+            norm = Residual(xk.As<double>());
+            converged = norm < tol;
+            continue;
+        }
+        // LineSearch: the norm at x0 is the one just computed (the reference re-assembles it; the
+        // assembly is deterministic, so the value is the same)
+        const double previous = norm;
+        Check(hipMemcpyAsync(x0.p, xk.p, (size_t)n * 8, hipMemcpyDeviceToDevice, ctx->stream), "line search");
+        double alpha = 1.0;
+        for (int step = 0; step < lsMax; step++)
+        {
+            Check(hipMemcpyAsync(xk.p, x0.p, (size_t)n * 8, hipMemcpyDeviceToDevice, ctx->stream), "line search");
+            rs.Axpby(n, -alpha, dx.As<double>(), 1.0, xk.As<double>());
+            norm = Residual(xk.As<double>());
+            if (norm < tol)
+            {
+                converged = true;
+                break;
+            }
+            alpha *= 0.5;
+            if (norm < (1.0 - alpha) * previous)
+                break;
+        }
+    }
+    if (info)
+    {
+        info->iterations = it;
+        info->linear_iterations = linIts;
+        info->residual_evaluations = evals;
+        info->residual_norm = norm;
+    }
+    Require(converged, NUTO_ERR_NO_CONVERGENCE, "No convergence after " + std::to_string(it) + " iterations.");
+    Require(Norm(xk.As<double>()) <= 1e10, NUTO_ERR_NO_CONVERGENCE, "floating point exception");
+    if (hist)
+    {
+        RowArgs a = MakeRowArgs(ctx, xk.As<double>(), E(xk.As<double>()), hist, nullptr, nullptr);
+        Dispatch(ctx, a, false, false, 1, hist);  // UpdateHistory
+    }
+    Check(hipMemcpyAsync(x, xk.p, (size_t)n * 8, hipMemcpyDeviceToDevice, ctx->stream), "Newton result");
+    Check(hipStreamSynchronize(ctx->stream), "Newton step");
+}
+
 }  // namespace
 
 extern "C" {
@@ -1068,6 +1183,12 @@ int nuto_solve_trial_state(nuto_ctx* ctx, const double* K, const double* f, cons
                            nuto_solve_info* info)
 {
     return Guard(ctx, [&] { SolveConstrained(ctx, K, f, rhs_old, rhs_new, true, x, options, info); });
+}
+
+int nuto_quasistatic_step(nuto_ctx* ctx, double* x, double* kappa, const double* rhs_old, const double* rhs_new,
+                          const nuto_newton_options* options, nuto_newton_info* info)
+{
+    return Guard(ctx, [&] { QuasistaticStep(ctx, x, kappa, rhs_old, rhs_new, options, info); });
 }
 
 int nuto_vjp(nuto_ctx* ctx, const double* u, const double* e, const double* kappa, const double* lambda,

@@ -393,3 +393,78 @@ def solve_constrained(n_dofs, equations, row_ptr, col, val, f, rhs_old=None, rhs
     if m < 0:
         raise ValueError("dependent dof out of range")
     return x
+
+
+class NoConvergence(RuntimeError):
+    """NewtonRaphson::NoConvergence (NewtonRaphson.h:12-20)."""
+
+
+def quasistatic_step(problem, n_dofs, equations, x, kappa=None, rhs_old=None, rhs_new=None, tol=1e-10, max_iter=6,
+                     line_search_steps=6):
+    """QuasistaticSolver::DoStep (QuasistaticSolver.cpp:116-147) on the oracle: the trial state
+    x - SolveTrialState(K(x), R(x)) (TrialState, :52-68), NewtonRaphson::Solve (NewtonRaphson.h:62-94)
+    with LineSearch (LineSearch.h:36-62) or, for line_search_steps == 0, NoLineSearch (:78-88), the
+    norm |C^T R| (QuasistaticSolver::Norm, :86-92), and the history update on convergence
+    (GradientDamage::Update, kappa = max(kappa, eeq), read back through Cell::Eval of kappa).
+    The linear systems are solved directly (solve_constrained). x is the monolithic [u; e] vector,
+    kappa the history (cell + ip * n_cells). Returns (x, kappa, iterations); raises NoConvergence."""
+    nd = list(n_dofs)
+    two = len(nd) > 1 and nd[1] > 0
+    Cs = [unit_constraint_matrix(nd[s], equations[s]) for s in range(len(nd))]
+
+    def split(v):
+        return v[:nd[0]], (v[nd[0]:nd[0] + nd[1]] if two else None)
+
+    def norm(v):
+        parts, o = [], 0
+        for s, C in enumerate(Cs):
+            parts.append(C.T @ v[o:o + nd[s]])
+            o += nd[s]
+        return float(np.sqrt(sum(float(p @ p) for p in parts)))
+
+    def assemble(v, want_r=True, want_k=True):
+        u, e = split(v)
+        return problem.assemble(u, e, kappa, want_r=want_r, want_k=want_k)
+
+    # TrialState
+    a = assemble(x)
+    dx = solve_constrained(nd, equations, a["row_ptr"], a["col"], a["val"], a["R"],
+                           rhs_old=np.zeros(sum(len(q) for q in equations)) if rhs_old is None else rhs_old,
+                           rhs_new=np.zeros(sum(len(q) for q in equations)) if rhs_new is None else rhs_new)
+    xk = x - dx
+    # NewtonRaphson::Solve
+    r = assemble(xk, want_k=False)["R"]
+    nr = norm(r)
+    it = 0
+    converged = nr < tol
+    while not converged and it < max_iter:
+        k = assemble(xk, want_r=False)
+        dx = solve_constrained(nd, equations, k["row_ptr"], k["col"], k["val"], r)
+        it += 1
+        if line_search_steps == 0:
+            xk = xk - dx
+            r = assemble(xk, want_k=False)["R"]
+            nr = norm(r)
+            converged = nr < tol
+            continue
+        x0, previous, alpha = xk.copy(), nr, 1.0
+        for _ in range(line_search_steps):
+            xk = x0 - alpha * dx
+            r = assemble(xk, want_k=False)["R"]
+            nr = norm(r)
+            if nr < tol:
+                converged = True
+                break
+            alpha *= 0.5
+            if nr < (1.0 - alpha) * previous:
+                break
+    if not converged:
+        raise NoConvergence(f"No convergence after {it} iterations.")
+    if norm(xk) > 1e10:
+        raise NoConvergence("floating point exception")
+    new_kappa = None
+    if kappa is not None:
+        u, e = split(xk)
+        kap = problem.eval_ip(2, u, e, kappa)[:, :, 0]  # Cell::Eval of kappa = max(history, eeq)
+        new_kappa = np.ascontiguousarray(kap.T).ravel()  # cell + ip * n_cells
+    return xk, new_kappa, it
