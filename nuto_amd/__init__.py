@@ -11,11 +11,12 @@ from .binding import (DIM, NUM_NODES, FULL, GRADIENT_DAMAGE, HEX8, HEX20, IP_DAM
                       morton_neighbor3d, morton_order, structured_mesh, NO_CONVERGENCE, adaptive_solve)
 
 from . import autograd  # noqa: E402  (torch.autograd binding of the residual)
+from . import visualize  # noqa: E402  (Visualize::PostProcess: .vtu / .pvd output)
 
 __all__ = [
     "DIM", "NUM_NODES", "Batch", "Context", "NutoError", "Params", "cell_dofs", "dof_layer", "gradient_damage_params", "lib",
     "linear_elastic_params", "local_damage_params", "structured_mesh", "QUAD4", "HEX8", "TET4", "TET10", "HEX20",
     "MOMENTUM_LINEAR_ELASTIC", "GRADIENT_DAMAGE", "MOMENTUM_LOCAL_DAMAGE", "PLANE_STRESS", "PLANE_STRAIN", "FULL",
     "UNILATERAL", "GmshMesh", "morton_order", "morton_encode3d", "morton_encode2d", "morton_decode3d",
-    "morton_decode2d", "morton_neighbor3d", "NO_CONVERGENCE", "adaptive_solve", "IP_STRAIN", "IP_STRESS", "IP_KAPPA", "IP_DAMAGE", "IP_EQ_STRAIN", "SOLVER_CG", "SOLVER_BICGSTAB", "autograd",
+    "morton_decode2d", "morton_neighbor3d", "NO_CONVERGENCE", "adaptive_solve", "IP_STRAIN", "IP_STRESS", "IP_KAPPA", "IP_DAMAGE", "IP_EQ_STRAIN", "SOLVER_CG", "SOLVER_BICGSTAB", "autograd", "visualize",
 ]

@@ -104,3 +104,32 @@ def test_linear_elastic_step_without_newton_iterations():
     info = ctx.quasistatic_step(x, None, None, to_dev(rhs), **LIN)
     assert info["iterations"] == 0
     np.testing.assert_allclose(x.cpu().numpy(), field.ravel(), rtol=0, atol=1e-11 * np.abs(field).max())
+
+
+def test_post_process_of_device_fields(tmp_path):
+    """Damage and kappa straight from the device into .vtu cell data (Visualize::PostProcess with
+    Cell::Eval), equal to the oracle's IP fields averaged per cell."""
+    from test_visualize import read_vtu
+
+    p, eqs, n_fixed, kappa0 = bar()
+    ctx = p.context()
+    ctx.set_constraints(0, eqs)
+    x = to_dev(np.zeros(p.n_u + p.n_e))
+    kappa = to_dev(kappa0.copy())
+    rhs = np.zeros(len(eqs))
+    rhs[n_fixed:] = 5e-5 * 6 * 3
+    ctx.quasistatic_step(x, kappa, None, to_dev(rhs), **LIN)
+    u, e = x[:p.n_u], x[p.n_u:]
+    pp = N.visualize.PostProcess(str(tmp_path))
+    pp.define_visualizer("GDM", N.HEX8, p.X, p.C)
+    pp.add_dof("GDM", "Displacements", u, N.cell_dofs(p.ranks, 3), 3)
+    pp.add_cell_function("GDM", lambda: ctx.eval_ip(N.IP_DAMAGE, u, e, kappa), "Damage")
+    pp.add_cell_function("GDM", lambda: ctx.eval_ip(N.IP_KAPPA, u, e, kappa), "Kappa")
+    pp.plot(1.0)
+    d, _, ncells = read_vtu(str(tmp_path / "GDM0.vtu"))
+    un, en, kn = u.cpu().numpy(), e.cpu().numpy(), kappa.cpu().numpy()
+    ref_d = p.orc.eval_ip(3, un, en, kn).mean(1)[:, 0]
+    ref_k = p.orc.eval_ip(2, un, en, kn).mean(1)[:, 0]
+    assert ncells == p.ncells and ref_d.max() > 0
+    np.testing.assert_allclose(d["Damage"], ref_d, rtol=1e-6, atol=1e-9)
+    np.testing.assert_allclose(d["Kappa"], ref_k, rtol=1e-6, atol=1e-12)
