@@ -191,7 +191,9 @@ def batched(n_meshes, steps, warmup):
         return float(np.median([a.elapsed_time(z) for a, z in evs]))
 
     graph_ms = timed(-1)
-    per_mesh_ms = timed(4)
+    # the plain per-mesh launches: the faster of the two kernel forms, one mesh at a time
+    per_mesh_mode = 6 if 0 <= ms_modes[6] < ms_modes[4] else 4
+    per_mesh_ms = timed(per_mesh_mode)
     counts = {}
     for ct, dt, *_ in meshes:
         k = {(N.TET4, N.TET4): "tet4", (N.TET4, N.TET10): "tet10", (N.HEX8, N.HEX8): "hex8",
@@ -201,8 +203,10 @@ def batched(n_meshes, steps, warmup):
                         f"local isotropic damage R+K", "cells": cells, "nnz": sizes["K"],
             "batched_ms": graph_ms, "per_mesh_launch_ms": per_mesh_ms, "speedup": per_mesh_ms / graph_ms,
             "cells_per_s": cells / (graph_ms * 1e-3), "tuned_mode": best,
+            "per_mesh_mode": per_mesh_mode,
             "autotune_ms": {"fused64": ms_modes[0], "fused128": ms_modes[1], "fused256": ms_modes[2],
-                            "per_type": ms_modes[3], "per_mesh": ms_modes[4]}, "setup_s": setup}
+                            "per_type": ms_modes[3], "per_mesh": ms_modes[4], "two_phase_per_type": ms_modes[5],
+                            "two_phase_per_mesh": ms_modes[6]}, "setup_s": setup}
 
 
 def batched_adjoint(n_meshes, steps, warmup):

@@ -286,10 +286,13 @@ int nuto_batch_copy_pattern(nuto_batch* batch, int mesh, int* row_ptr, int* col)
 int nuto_batch_set_material_fields(nuto_batch* batch, const double* E, const double* ft, const double* gf);
 /* R and K of every mesh. mode -1: the tuned launch (mode 3 until tuned) replayed from a HIP graph; 0..2: one fused launch
  * over all meshes with 64 / 128 / 256 nodes per workgroup; 3: one launch per element type over all
- * meshes of that type; 4: one launch per mesh (what separate nuto_build_vector_matrix calls do). */
+ * meshes of that type; 4: one launch per mesh (the row-owner kernel); 5: two-phase (cell matrices,
+ * then the row gather) per element type; 6: two-phase per mesh (what separate
+ * nuto_build_vector_matrix calls do). Gradient damage maps 0..2 and 5 to 3, and 6 to 4. */
 int nuto_batch_assemble(nuto_batch* batch, const double* u, const double* e, const double* kappa, double* R,
                         double* K, int mode);
-/* Times modes 0..4 (ms[5], -1 where not applicable) and makes the fastest of 0..3 the tuned mode. */
+/* Times modes 0..6 (ms[7], -1 where not applicable) and makes the fastest of 0..3 and 5 (the batched
+ * forms) the tuned mode. */
 int nuto_batch_autotune(nuto_batch* batch, const double* u, const double* e, const double* kappa, double* R,
                         double* K, double* ms, int* best);
 /* Gradient damage: nuto_vjp for every mesh (d_state layout of R, d_theta layout of info[15]). */

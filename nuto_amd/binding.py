@@ -695,7 +695,7 @@ class Batch:
     def autotune(self, u, e=None, kappa=None, R=None, K=None):
         R = self._new(self.sizes["R"]) if R is None else R
         K = self._new(self.sizes["K"]) if K is None else K
-        ms = np.zeros(5)
+        ms = np.zeros(7)
         best = ctypes.c_int()
         self._enter()
         self._chk(lib().nuto_batch_autotune(self.h, _dev_ptr(u), _dev_ptr(e), _dev_ptr(kappa), _dev_ptr(R),

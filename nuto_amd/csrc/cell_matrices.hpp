@@ -1,0 +1,316 @@
+// Two-phase generic assembly for 3D momentum balance (linear elasticity, local isotropic damage):
+// element matrices first, then a row gather into the CSR.
+//
+// Phase 1 (kCellMatrices): Cell::Integrate (Cell.h:105-119) of one cell per lane group. Lane j of
+// the group owns column j of the cell matrix K_e (dof j = 3 b + c of dof node b) and its R_e entry.
+// Per IP the group computes J, J^-1 and its own dN_b/dX, shares the gradients through LDS, forms
+// the strain, the law's stress and tangent (the same closed forms as the row-owner kernel), then
+// the stress of a unit displacement of its dof, dsig_j = D B_j (D with the local-damage rank-one
+// term t m^T), and accumulates column j: K_e(3 a + i, j) += (B_a^T dsig_j)_i for every node a.
+// The cell's state is computed once per IP per group (the row-owner form recomputes it for every
+// row node and column block). K_e (row-major) and R_e go to a staging buffer.
+//
+// Phase 2 (kGatherRows): one lane per dof node p adds the staged rows of p's incident cells, in
+// group order, into p's CSR rows (the slot table maps a cell's column nodes to row positions) and
+// sums R_e the same way: the nonzero pattern and the summation order over cells are those of the
+// row-owner kernel (and of setFromTriplets), IPs are summed per cell first as Cell::Integrate does.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "assemble_rows.hpp"
+
+namespace nuto_amd
+{
+
+template <int KIND, int NNC, int NND, int NIP>
+__global__ void __launch_bounds__(64) kCellMatrices(const RowArgs A, int c0, int c1, double* __restrict__ Ke,
+                                                    double* __restrict__ Re)
+{
+    constexpr int ND = NND * 3;     // dofs (columns) per cell
+    constexpr int CPW = 64 / ND;    // cells per wave
+    constexpr int S = 1 + NNC * 3 + NND + NND * 3;
+    static_assert(CPW >= 1, "a cell's columns must fit one wave");
+    __shared__ double sX[CPW][NNC * 3];
+    __shared__ double sU[CPW][ND];
+    __shared__ double sG[CPW][NND * 3];
+
+    const int lane = (int)threadIdx.x;
+    const int grp = lane / ND < CPW ? lane / ND : CPW - 1;  // lanes past the last group shadow it
+    const int j = lane - grp * ND < ND ? lane - grp * ND : ND - 1;
+    const int cellRaw = c0 + (int)blockIdx.x * CPW + lane / ND;
+    const bool act = lane / ND < CPW && cellRaw < c1;
+    const int cell = act ? cellRaw : (c0 + (int)blockIdx.x * CPW + grp < c1 ? c0 + (int)blockIdx.x * CPW + grp : c1 - 1);
+    const int b = j / 3, c = j - 3 * b;
+
+    if (act)
+    {
+        for (int k = j; k < NNC * 3; k += ND)
+        {
+            const int n = A.cconn[(size_t)cell * NNC + k / 3];
+            const int comp = k % 3;
+            sX[grp][k] = comp == 0 ? A.x[n] : (comp == 1 ? A.y[n] : A.z[n]);
+        }
+        const int q = A.dconn[(size_t)cell * NND + b];
+        sU[grp][j] = A.u[(size_t)q * 3 + c];
+    }
+    __syncthreads();
+
+    const LawParams& P = A.P;
+    double E = P.E, k0 = P.kappa0, beta = P.beta;
+    if (A.thE)
+    {
+        E = A.thE[cell];
+        k0 = A.thFt[cell] / E;
+        beta = A.thFt[cell] / A.thGf[cell];
+    }
+    const int cid = A.cellId ? A.cellId[cell] : cell;
+
+    double kcol[ND];
+#pragma unroll
+    for (int r = 0; r < ND; r++)
+        kcol[r] = 0.0;
+    double rj = 0.0;
+
+#pragma unroll 1
+    for (int ip = 0; ip < NIP; ip++)
+    {
+        const double* t = A.tab + ip * S;
+        const double w = t[0];
+        const double* dNc = t + 1;
+        const double* dNd = dNc + NNC * 3 + NND;
+        double J[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int m = 0; m < 3; m++)
+            {
+                double s = 0.0;
+#pragma unroll
+                for (int k = 0; k < NNC; k++)
+                    s += sX[grp][k * 3 + i] * dNc[k * 3 + m];
+                J[i][m] = s;
+            }
+        double inv[3][3];
+        const double detJ = Inverse<3>(J, inv);
+        const double s = detJ * w;
+        double gb[3];
+        Grad<3, NND>(dNd, b, inv, gb);
+        if (act && c == 0)
+        {
+            sG[grp][b * 3 + 0] = gb[0];
+            sG[grp][b * 3 + 1] = gb[1];
+            sG[grp][b * 3 + 2] = gb[2];
+        }
+        __syncthreads();
+
+        // strain (Voigt [xx, yy, zz, yz, zx, xy], engineering shear)
+        double eps[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int a = 0; a < NND; a++)
+        {
+            const double gx = sG[grp][a * 3], gy = sG[grp][a * 3 + 1], gz = sG[grp][a * 3 + 2];
+            const double ux = sU[grp][a * 3], uy = sU[grp][a * 3 + 1], uz = sU[grp][a * 3 + 2];
+            eps[0] += gx * ux;
+            eps[1] += gy * uy;
+            eps[2] += gz * uz;
+            eps[3] += gz * uy + gy * uz;
+            eps[4] += gz * ux + gx * uz;
+            eps[5] += gy * ux + gx * uy;
+        }
+
+        // law: stress, isotropic tangent (c1 tr + 2 c2 eps, shear c2 gamma), rank-one damage term
+        double sig[6], c1, c2;
+        double t6[6] = {0, 0, 0, 0, 0, 0}, m6[6] = {0, 0, 0, 0, 0, 0};
+        if constexpr (KIND == kLinearElastic)
+        {
+            const double f = E / ((1.0 + P.nu) * (1.0 - 2.0 * P.nu));
+            const double lam = f * P.nu, mu = E / (2.0 * (1.0 + P.nu));
+            const double tr = eps[0] + eps[1] + eps[2];
+            for (int i = 0; i < 3; i++)
+                sig[i] = lam * tr + 2.0 * mu * eps[i];
+            for (int i = 3; i < 6; i++)
+                sig[i] = mu * eps[i];
+            c1 = s * lam;
+            c2 = s * mu;
+        }
+        else
+        {
+            double mdev[6];
+            const double eq = Mises3(eps, P.nu, P.k, mdev);
+            const double kh = A.kappa[(size_t)cid + (size_t)ip * A.kappaLd];
+            const double kap = eq < kh ? kh : eq;
+            const double dk = eq >= kh ? 1.0 : 0.0;
+            double om, dom;
+            DamageExp(kap, k0, beta, P.alpha, om, dom);
+            DamageStress3 ds;
+            ElasticDamage3(eps, om, E, P.nu, P.damage_application, ds);
+            for (int i = 0; i < 6; i++)
+                sig[i] = ds.sig[i];
+            c1 = s * (ds.alpha_v - ds.alpha_d) / 3.0;
+            c2 = s * 0.5 * ds.alpha_d;
+            const double ct = s * dom * dk;
+            for (int i = 0; i < 6; i++)
+            {
+                t6[i] = ct * ds.dsdw[i];
+                m6[i] = mdev[i];
+            }
+        }
+
+        // R_e: column j of B^T sig
+        {
+            double bt[3];
+            BTv3(gb, sig, bt);
+            rj += s * (c == 0 ? bt[0] : (c == 1 ? bt[1] : bt[2]));
+        }
+
+        // dsig_j = D B_j for the unit displacement of dof j = 3 b + c
+        double de[6];
+        de[0] = c == 0 ? gb[0] : 0.0;
+        de[1] = c == 1 ? gb[1] : 0.0;
+        de[2] = c == 2 ? gb[2] : 0.0;
+        de[3] = c == 1 ? gb[2] : (c == 2 ? gb[1] : 0.0);
+        de[4] = c == 0 ? gb[2] : (c == 2 ? gb[0] : 0.0);
+        de[5] = c == 0 ? gb[1] : (c == 1 ? gb[0] : 0.0);
+        const double tr = de[0] + de[1] + de[2];
+        double ds6[6];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            ds6[i] = c1 * tr + 2.0 * c2 * de[i];
+#pragma unroll
+        for (int i = 3; i < 6; i++)
+            ds6[i] = c2 * de[i];
+        if constexpr (KIND == kLocalDamage)
+        {
+            const double mb = m6[0] * de[0] + m6[1] * de[1] + m6[2] * de[2] + m6[3] * de[3] + m6[4] * de[4] +
+                              m6[5] * de[5];
+#pragma unroll
+            for (int i = 0; i < 6; i++)
+                ds6[i] += t6[i] * mb;
+        }
+
+        // column j: K_e(3 a + i, j) += (B_a^T dsig_j)_i
+#pragma unroll
+        for (int a = 0; a < NND; a++)
+        {
+            const double ga[3] = {sG[grp][a * 3], sG[grp][a * 3 + 1], sG[grp][a * 3 + 2]};
+            double o[3];
+            BTv3(ga, ds6, o);
+            kcol[a * 3 + 0] += o[0];
+            kcol[a * 3 + 1] += o[1];
+            kcol[a * 3 + 2] += o[2];
+        }
+        __syncthreads();  // sG is rewritten by the next IP
+    }
+
+    if (act)
+    {
+        double* dst = Ke + (size_t)cell * ND * ND + j;
+#pragma unroll
+        for (int r = 0; r < ND; r++)
+            dst[(size_t)r * ND] = kcol[r];
+        Re[(size_t)cell * ND + j] = rj;
+    }
+}
+
+template <int NND, bool WR>
+__global__ void __launch_bounds__(64) kGatherRows(const RowArgs A, int p0, int p1, const double* __restrict__ Ke,
+                                                  const double* __restrict__ Re)
+{
+    constexpr int ND = NND * 3;
+    const int p = p0 + (int)(blockIdx.x * 64 + threadIdx.x);
+    if (p >= p1)
+        return;
+    const int beg = A.incPtr[p], end = A.incPtr[p + 1];
+    const int nA = A.adjPtr[p + 1] - A.adjPtr[p];
+    const int L = nA * 3;
+    const int ru0 = A.rowPtr[p * 3];
+    double racc[3] = {0.0, 0.0, 0.0};
+    for (int r = beg; r < end; ++r)
+    {
+        const int cell = A.incCell[r];
+        const int la = A.incLocal[r];
+        const unsigned short* sl = A.slot + (size_t)r * NND;
+        const double* src = Ke + ((size_t)cell * ND + la * 3) * ND;
+        if constexpr (WR)
+        {
+            const double* rs = Re + (size_t)cell * ND + la * 3;
+            racc[0] += rs[0];
+            racc[1] += rs[1];
+            racc[2] += rs[2];
+        }
+        for (int b = 0; b < NND; b++)
+        {
+            const unsigned short v = sl[b];
+            const int q = v & 0x7fff;
+            const bool first = (v & 0x8000) != 0;
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+            {
+                double* row = A.K + ru0 + (size_t)i * L + 3 * q;
+                const double* s = src + i * ND + b * 3;
+#pragma unroll
+                for (int jj = 0; jj < 3; jj++)
+                {
+                    if (first)
+                        row[jj] = s[jj];
+                    else
+                        row[jj] += s[jj];
+                }
+            }
+        }
+    }
+    if constexpr (WR)
+    {
+        A.R[(size_t)p * 3 + 0] = racc[0];
+        A.R[(size_t)p * 3 + 1] = racc[1];
+        A.R[(size_t)p * 3 + 2] = racc[2];
+    }
+}
+
+// Phase 2 with one wave per dof node: the node's three rows are summed in LDS (3 L doubles, L =
+// 3 |adj(p)|, dynamic shared memory of 3 maxL doubles), each incident cell's three staged rows are
+// read with coalesced loads and added at their slots in cell order, then the rows are written with
+// coalesced stores (the rows of p are contiguous in the CSR).
+template <int NND, bool WR>
+__global__ void __launch_bounds__(64) kGatherRowsWave(const RowArgs A, int p0, int p1, const double* __restrict__ Ke,
+                                                      const double* __restrict__ Re)
+{
+    constexpr int ND = NND * 3;
+    extern __shared__ double rows[];
+    const int p = p0 + (int)blockIdx.x;
+    const int lane = (int)threadIdx.x;
+    if (p >= p1)
+        return;
+    const int beg = A.incPtr[p], end = A.incPtr[p + 1];
+    const int L = (A.adjPtr[p + 1] - A.adjPtr[p]) * 3;
+    const int ru0 = A.rowPtr[p * 3];
+    for (int f = lane; f < 3 * L; f += 64)
+        rows[f] = 0.0;
+    double racc = 0.0;
+    __syncthreads();
+    for (int r = beg; r < end; ++r)
+    {
+        const int cell = A.incCell[r];
+        const int la = A.incLocal[r];
+        const unsigned short* sl = A.slot + (size_t)r * NND;
+        const double* src = Ke + ((size_t)cell * ND + la * 3) * ND;
+        for (int f = lane; f < 3 * ND; f += 64)
+        {
+            const int i = f / ND, rem = f - i * ND;
+            const int b = rem / 3, jj = rem - 3 * b;
+            const int q = sl[b] & 0x7fff;
+            rows[i * L + 3 * q + jj] += src[f];
+        }
+        if (WR && lane < 3)
+            racc += Re[(size_t)cell * ND + la * 3 + lane];
+        __syncthreads();
+    }
+    double* dst = A.K + ru0;
+    for (int f = lane; f < 3 * L; f += 64)
+        dst[f] = rows[f];
+    if (WR && lane < 3)
+        A.R[(size_t)p * 3 + lane] = racc;
+}
+
+}  // namespace nuto_amd

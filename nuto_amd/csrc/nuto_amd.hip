@@ -15,6 +15,7 @@
 #include "gdm_hex8_vjp.hpp"
 #include "batch.hpp"
 #include "cell_fields.hpp"
+#include "cell_matrices.hpp"
 #include "solve.hpp"
 #include "mesh.hpp"
 
@@ -82,6 +83,9 @@ struct nuto_ctx
     bool consBuilt = false;
     int nInd = 0, nEq = 0;
     DevBuf cRow, cCol, cVal, ctRow, ctCol, ctVal, consDep;
+    // staged cell matrices and vectors of the two-phase generic assembly
+    DevBuf keStage, reStage;
+    int maxAdj = -1;  // largest node neighbourhood (LDS rows of the wave gather), from the node graph
 };
 
 namespace
@@ -167,6 +171,7 @@ void InvalidatePattern(nuto_ctx* ctx)
     ctx->gdm.Release();
     ctx->gdmTried = false;
     ctx->consBuilt = false;
+    ctx->maxAdj = -1;
 }
 
 // IP tables: per IP w, dNc[nnC][dim], N[nnD], dNd[nnD][dim] (shape tables of the coordinate
@@ -266,12 +271,80 @@ RowArgs MakeRowArgs(nuto_ctx* ctx, const double* u, const double* e, const doubl
     return a;
 }
 
+// NUTO_AMD_TWO_PHASE=0 keeps the row-owner kernel for every generic 3D matrix assembly
+bool TwoPhaseOff()
+{
+    const char* v = std::getenv("NUTO_AMD_TWO_PHASE");
+    return v && v[0] == '0';
+}
+
+// staging for the cell matrices of cells [0, nCells) (grown on demand, kept with the context)
+void EnsureStage(nuto_ctx* ctx, int nd)
+{
+    const size_t ke = (size_t)ctx->nCells * nd * nd * sizeof(double), re = (size_t)ctx->nCells * nd * sizeof(double);
+    if (ctx->keStage.bytes < ke)
+        Check(ctx->keStage.Alloc(ke), "cell matrix staging");
+    if (ctx->reStage.bytes < re)
+        Check(ctx->reStage.Alloc(re), "cell vector staging");
+}
+
+// Two-phase assembly of the cells [c0, c1) and the rows of the dof nodes [p0, p1) (the rows'
+// incident cells must lie in [c0, c1): a whole mesh of a batch, or the whole group).
+template <int KIND, int NNC, int NND, int NIP>
+void LaunchTwoPhaseT(nuto_ctx* ctx, const RowArgs& a, bool wr, int c0, int c1, int p0, int p1, hipStream_t stream)
+{
+    constexpr int ND = NND * 3, CPW = 64 / ND;
+    if (c1 <= c0 || p1 <= p0)
+        return;
+    EnsureStage(ctx, ND);
+    double* ke = ctx->keStage.As<double>();
+    double* re = ctx->reStage.As<double>();
+    hipLaunchKernelGGL((kCellMatrices<KIND, NNC, NND, NIP>), dim3((unsigned)((c1 - c0 + CPW - 1) / CPW)), dim3(64), 0,
+                       stream, a, c0, c1, ke, re);
+    if (ctx->maxAdj < 0)
+    {
+        int m = 0;
+        for (int p = 0; p < ctx->graph.nNodes; p++)
+            m = std::max(m, ctx->graph.adjPtr[p + 1] - ctx->graph.adjPtr[p]);
+        ctx->maxAdj = m;
+    }
+    const size_t lds = (size_t)3 * 3 * ctx->maxAdj * sizeof(double);
+    if (lds <= 48 * 1024)
+    {
+        // one wave per node, rows summed in LDS
+        if (wr)
+            hipLaunchKernelGGL((kGatherRowsWave<NND, true>), dim3((unsigned)(p1 - p0)), dim3(64), lds, stream, a, p0, p1,
+                               ke, re);
+        else
+            hipLaunchKernelGGL((kGatherRowsWave<NND, false>), dim3((unsigned)(p1 - p0)), dim3(64), lds, stream, a, p0,
+                               p1, ke, re);
+    }
+    else
+    {
+        const dim3 grid((unsigned)((p1 - p0 + 63) / 64));
+        if (wr)
+            hipLaunchKernelGGL((kGatherRows<NND, true>), grid, dim3(64), 0, stream, a, p0, p1, ke, re);
+        else
+            hipLaunchKernelGGL((kGatherRows<NND, false>), grid, dim3(64), 0, stream, a, p0, p1, ke, re);
+    }
+    Check(hipGetLastError(), "two-phase assembly launch");
+}
+
 template <int KIND, int DIM, int NNC, int NND, int NIP, int BH>
 void LaunchRowsT(nuto_ctx* ctx, const RowArgs& a, bool wr, bool wk)
 {
     const int n = a.nNodes;
     if (n == 0)
         return;
+    if constexpr (DIM == 3 && KIND != kGradientDamage && NND * 3 <= 64)
+    {
+        if (wk && !TwoPhaseOff())
+        {
+            LaunchTwoPhaseT<KIND, NNC, NND, NIP>(ctx, a, wr, 0, a.nCells, 0, n, ctx->stream);
+            SyncCheck(ctx, "assembly kernel");
+            return;
+        }
+    }
     dim3 grid((n + 63) / 64), block(64);
     if (wr && wk)
         hipLaunchKernelGGL((kAssembleRows<KIND, DIM, NNC, NND, NIP, BH, true, true>), grid, block, 0, ctx->stream, a);
@@ -1281,7 +1354,9 @@ bool BatchTypeOf(int ct, int dt, int& type, int& ruleKind, int& ruleN)
     return true;
 }
 
-constexpr int kBatchModes = 5;  // 0..2 fused launch, 64/128/256 nodes per tile; 3 one launch per type; 4 per mesh
+// 0..2 fused launch, 64/128/256 nodes per tile; 3 one launch per type; 4 one launch per mesh;
+// 5 two-phase (cell matrices + row gather) per type; 6 two-phase per mesh
+constexpr int kBatchModes = 7;
 constexpr int kBatchBlock[3] = {64, 128, 256};
 
 }  // namespace
@@ -1432,6 +1507,19 @@ void LaunchRange(const RowArgs& a, int type, int first, int end, hipStream_t str
         hipLaunchKernelGGL((kBatchRowsRange<KIND, kBtHex20>), grid, block, 0, stream, a, first, end);
 }
 
+template <int KIND>
+void LaunchTwoPhaseType(nuto_ctx* ctx, const RowArgs& a, int type, int c0, int c1, int p0, int p1, hipStream_t stream)
+{
+    if (type == kBtTet4)
+        LaunchTwoPhaseT<KIND, 4, 4, 1>(ctx, a, true, c0, c1, p0, p1, stream);
+    else if (type == kBtTet10)
+        LaunchTwoPhaseT<KIND, 4, 10, 4>(ctx, a, true, c0, c1, p0, p1, stream);
+    else if (type == kBtHex8)
+        LaunchTwoPhaseT<KIND, 8, 8, 8>(ctx, a, true, c0, c1, p0, p1, stream);
+    else
+        LaunchTwoPhaseT<KIND, 8, 20, 27>(ctx, a, true, c0, c1, p0, p1, stream);
+}
+
 bool IsGdm(const nuto_batch* b)
 {
     return b->params.kind == kGradientDamage;
@@ -1443,6 +1531,8 @@ void LaunchBatch(nuto_batch* b, int mode, const double* u, const double* e, cons
     const int kind = b->params.kind;
     if (IsGdm(b) && mode < 3)
         mode = 3;
+    if (IsGdm(b) && mode >= 5)
+        mode -= 2;  // gradient damage has one kernel form: per group (3) or per mesh (4)
     if (mode < 3)
     {
         UploadArgs(b, u, e, kappa, R, K);
@@ -1456,6 +1546,29 @@ void LaunchBatch(nuto_batch* b, int mode, const double* u, const double* e, cons
         for (BatchGroup& g : b->groups)
         {
             RowArgs a = GroupArgs(b, g, u, e, kappa, R, K);
+            if (mode >= 5 && !IsGdm(b))
+            {
+                // two-phase: cell matrices, then the row gather, per element type (5) or per mesh (6)
+                if (mode == 5)
+                {
+                    if (kind == kLinearElastic)
+                        LaunchTwoPhaseType<kLinearElastic>(g.ctx, a, g.type, 0, g.nCells, 0, g.nNodes, b->stream);
+                    else
+                        LaunchTwoPhaseType<kLocalDamage>(g.ctx, a, g.type, 0, g.nCells, 0, g.nNodes, b->stream);
+                    continue;
+                }
+                for (int m : g.meshes)
+                {
+                    const BatchMesh& bm = b->meshes[m];
+                    const int c0 = bm.cellOff, c1 = bm.cellOff + bm.nCells;
+                    const int p0 = bm.nodeOff, p1 = bm.nodeOff + bm.nDof;
+                    if (kind == kLinearElastic)
+                        LaunchTwoPhaseType<kLinearElastic>(g.ctx, a, g.type, c0, c1, p0, p1, b->stream);
+                    else
+                        LaunchTwoPhaseType<kLocalDamage>(g.ctx, a, g.type, c0, c1, p0, p1, b->stream);
+                }
+                continue;
+            }
             if (mode == 3)
             {
                 if (IsGdm(b))
@@ -1907,7 +2020,7 @@ int nuto_batch_autotune(nuto_batch* b, const double* u, const double* e, const d
         for (int mode = 0; mode < kBatchModes; mode++)
         {
             double v = -1.0;
-            if (!(IsGdm(b) && mode < 3))
+            if (!(IsGdm(b) && (mode < 3 || mode >= 5)))
             {
                 LaunchBatch(b, mode, u, e, kappa, R, K);  // warm-up
                 std::vector<float> rep;
@@ -1923,7 +2036,7 @@ int nuto_batch_autotune(nuto_batch* b, const double* u, const double* e, const d
                 }
                 std::sort(rep.begin(), rep.end());
                 v = rep[rep.size() / 2];
-                if (mode != 4 && v < bestMs)  // the per-mesh launch is the reference, not a candidate
+                if (mode != 4 && mode != 6 && v < bestMs)  // per-mesh launches are the reference, not candidates
                 {
                     bestMs = v;
                     bestMode = mode;

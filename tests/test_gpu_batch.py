@@ -52,7 +52,7 @@ def mixed():
     return meshes, b, sizes, u, kap
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4, 5, 6])
 def test_every_mesh_matches_oracle(mixed, mode):
     meshes, b, sizes, u, kap = mixed
     R, K = b.assemble(to_dev(u), None, to_dev(kap), mode=mode)
@@ -74,16 +74,18 @@ def test_every_mesh_matches_oracle(mixed, mode):
 
 def test_launch_modes_are_deterministic(mixed):
     """Each mode reproduces its own bits; the per-type and per-mesh launches share one kernel per
-    element type and agree bit for bit (the fused kernel is compiled separately, FMA contraction
-    may differ in the last bit; it is held to the oracle bar above)."""
+    element type and agree bit for bit, for the row-owner (3, 4) and the two-phase form (5, 6) (the
+    fused kernel is compiled separately, FMA contraction may differ in the last bit; every mode is
+    held to the oracle bar above)."""
     _, b, sizes, u, kap = mixed
-    per_mesh = [t.cpu().numpy() for t in b.assemble(to_dev(u), None, to_dev(kap), mode=4)]
-    for mode in (0, 1, 2, 3):
+    per_mesh = {m: [t.cpu().numpy() for t in b.assemble(to_dev(u), None, to_dev(kap), mode=m)] for m in (4, 6)}
+    for mode in (0, 1, 2, 3, 5):
         a = [t.cpu().numpy() for t in b.assemble(to_dev(u), None, to_dev(kap), mode=mode)]
         c = [t.cpu().numpy() for t in b.assemble(to_dev(u), None, to_dev(kap), mode=mode)]
         assert np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1]), f"mode {mode}"
-        if mode == 3:
-            assert np.array_equal(a[0], per_mesh[0]) and np.array_equal(a[1], per_mesh[1])
+        if mode in (3, 5):
+            ref = per_mesh[mode + 1]
+            assert np.array_equal(a[0], ref[0]) and np.array_equal(a[1], ref[1]), f"mode {mode}"
 
 
 def test_graph_replays_match_eager(mixed):
@@ -104,7 +106,7 @@ def test_graph_replays_match_eager(mixed):
 def test_autotune_picks_a_batched_mode(mixed):
     _, b, sizes, u, kap = mixed
     ms, best = b.autotune(to_dev(u), None, to_dev(kap))
-    assert best in (0, 1, 2, 3) and all(v > 0 for v in ms)
+    assert best in (0, 1, 2, 3, 5) and all(v > 0 for v in ms)
     for rep in range(3):
         R, K = b.assemble(to_dev(u), None, to_dev(kap), mode=-1)
         ref = b.assemble(to_dev(u), None, to_dev(kap), mode=best)

@@ -116,9 +116,12 @@ def test_post_process_of_device_fields(tmp_path):
     ctx.set_constraints(0, eqs)
     x = to_dev(np.zeros(p.n_u + p.n_e))
     kappa = to_dev(kappa0.copy())
-    rhs = np.zeros(len(eqs))
-    rhs[n_fixed:] = 5e-5 * 6 * 3
-    ctx.quasistatic_step(x, kappa, None, to_dev(rhs), **LIN)
+    rhs_old = np.zeros(len(eqs))
+    for step in (1, 2):  # the load steps of the localising bar test
+        rhs = np.zeros(len(eqs))
+        rhs[n_fixed:] = 5e-5 * 6 * step
+        ctx.quasistatic_step(x, kappa, to_dev(rhs_old), to_dev(rhs), **LIN)
+        rhs_old = rhs
     u, e = x[:p.n_u], x[p.n_u:]
     pp = N.visualize.PostProcess(str(tmp_path))
     pp.define_visualizer("GDM", N.HEX8, p.X, p.C)
