@@ -31,9 +31,10 @@ __global__ void __launch_bounds__(64) kCellMatrices(const RowArgs A, int c0, int
     constexpr int CPW = 64 / ND;    // cells per wave
     constexpr int S = 1 + NNC * 3 + NND + NND * 3;
     static_assert(CPW >= 1, "a cell's columns must fit one wave");
-    __shared__ double sX[CPW][NNC * 3];
-    __shared__ double sU[CPW][ND];
-    __shared__ double sG[CPW][NND * 3];
+    // per node 4 doubles (x, y, z, pad): 16-byte aligned pairs for wide LDS reads
+    __shared__ __align__(16) double sX[CPW][NNC * 4];
+    __shared__ __align__(16) double sU[CPW][NND * 4];
+    __shared__ __align__(16) double sG[CPW][NND * 4];
 
     const int lane = (int)threadIdx.x;
     const int grp = lane / ND < CPW ? lane / ND : CPW - 1;  // lanes past the last group shadow it
@@ -49,10 +50,10 @@ __global__ void __launch_bounds__(64) kCellMatrices(const RowArgs A, int c0, int
         {
             const int n = A.cconn[(size_t)cell * NNC + k / 3];
             const int comp = k % 3;
-            sX[grp][k] = comp == 0 ? A.x[n] : (comp == 1 ? A.y[n] : A.z[n]);
+            sX[grp][(k / 3) * 4 + comp] = comp == 0 ? A.x[n] : (comp == 1 ? A.y[n] : A.z[n]);
         }
         const int q = A.dconn[(size_t)cell * NND + b];
-        sU[grp][j] = A.u[(size_t)q * 3 + c];
+        sU[grp][b * 4 + c] = A.u[(size_t)q * 3 + c];
     }
     __syncthreads();
 
@@ -65,6 +66,7 @@ __global__ void __launch_bounds__(64) kCellMatrices(const RowArgs A, int c0, int
         beta = A.thFt[cell] / A.thGf[cell];
     }
     const int cid = A.cellId ? A.cellId[cell] : cell;
+    const double m3K = 3.0 * E / (3.0 - 6.0 * P.nu), m2G = 2.0 * E / (2.0 + 2.0 * P.nu);
 
     double kcol[ND];
 #pragma unroll
@@ -88,7 +90,7 @@ __global__ void __launch_bounds__(64) kCellMatrices(const RowArgs A, int c0, int
                 double s = 0.0;
 #pragma unroll
                 for (int k = 0; k < NNC; k++)
-                    s += sX[grp][k * 3 + i] * dNc[k * 3 + m];
+                    s += sX[grp][k * 4 + i] * dNc[k * 3 + m];
                 J[i][m] = s;
             }
         double inv[3][3];
@@ -98,9 +100,9 @@ __global__ void __launch_bounds__(64) kCellMatrices(const RowArgs A, int c0, int
         Grad<3, NND>(dNd, b, inv, gb);
         if (act && c == 0)
         {
-            sG[grp][b * 3 + 0] = gb[0];
-            sG[grp][b * 3 + 1] = gb[1];
-            sG[grp][b * 3 + 2] = gb[2];
+            sG[grp][b * 4 + 0] = gb[0];
+            sG[grp][b * 4 + 1] = gb[1];
+            sG[grp][b * 4 + 2] = gb[2];
         }
         __syncthreads();
 
@@ -109,8 +111,8 @@ __global__ void __launch_bounds__(64) kCellMatrices(const RowArgs A, int c0, int
 #pragma unroll
         for (int a = 0; a < NND; a++)
         {
-            const double gx = sG[grp][a * 3], gy = sG[grp][a * 3 + 1], gz = sG[grp][a * 3 + 2];
-            const double ux = sU[grp][a * 3], uy = sU[grp][a * 3 + 1], uz = sU[grp][a * 3 + 2];
+            const double gx = sG[grp][a * 4], gy = sG[grp][a * 4 + 1], gz = sG[grp][a * 4 + 2];
+            const double ux = sU[grp][a * 4], uy = sU[grp][a * 4 + 1], uz = sU[grp][a * 4 + 2];
             eps[0] += gx * ux;
             eps[1] += gy * uy;
             eps[2] += gz * uz;
@@ -144,10 +146,10 @@ __global__ void __launch_bounds__(64) kCellMatrices(const RowArgs A, int c0, int
             double om, dom;
             DamageExp(kap, k0, beta, P.alpha, om, dom);
             DamageStress3 ds;
-            ElasticDamage3(eps, om, E, P.nu, P.damage_application, ds);
+            ElasticDamage3K(eps, om, m3K, m2G, P.damage_application, ds);
             for (int i = 0; i < 6; i++)
                 sig[i] = ds.sig[i];
-            c1 = s * (ds.alpha_v - ds.alpha_d) / 3.0;
+            c1 = s * (ds.alpha_v - ds.alpha_d) * (1.0 / 3.0);
             c2 = s * 0.5 * ds.alpha_d;
             const double ct = s * dom * dk;
             for (int i = 0; i < 6; i++)
@@ -193,7 +195,7 @@ __global__ void __launch_bounds__(64) kCellMatrices(const RowArgs A, int c0, int
 #pragma unroll
         for (int a = 0; a < NND; a++)
         {
-            const double ga[3] = {sG[grp][a * 3], sG[grp][a * 3 + 1], sG[grp][a * 3 + 2]};
+            const double ga[3] = {sG[grp][a * 4], sG[grp][a * 4 + 1], sG[grp][a * 4 + 2]};
             double o[3];
             BTv3(ga, ds6, o);
             kcol[a * 3 + 0] += o[0];

@@ -52,9 +52,10 @@ __device__ __host__ inline void DamageExp(double kappa, double k0, double beta, 
         return;
     }
     const double ex = exp(beta * (k0 - kappa));
-    const double r = k0 / kappa;
+    const double ik = 1.0 / kappa;
+    const double r = k0 * ik;
     w = 1.0 - r * (1.0 - alpha + alpha * ex);
-    dw = r * ((1.0 / kappa + beta) * alpha * ex + (1.0 - alpha) / kappa);
+    dw = r * ((ik + beta) * alpha * ex + (1.0 - alpha) * ik);
 }
 
 // ModifiedMisesStrainNorm<3> value and derivative (ModifiedMisesStrainNorm.h:70-84,136-146,231-268)
@@ -64,7 +65,7 @@ __device__ __host__ inline double Mises3(const double e[6], double nu, double k,
     const double K2 = 3.0 / (k * (1.0 + nu) * (1.0 + nu));
     const double I1 = e[0] + e[1] + e[2];
     const double a = e[0] - e[1], b = e[1] - e[2], cc = e[2] - e[0];
-    const double J2 = (a * a + b * b + cc * cc) / 6.0 + 0.25 * (e[3] * e[3] + e[4] * e[4] + e[5] * e[5]);
+    const double J2 = (a * a + b * b + cc * cc) * (1.0 / 6.0) + 0.25 * (e[3] * e[3] + e[4] * e[4] + e[5] * e[5]);
     const double A = sqrt(K1 * K1 * I1 * I1 + K2 * J2);
     if (A == 0.0)
     {
@@ -75,9 +76,10 @@ __device__ __host__ inline double Mises3(const double e[6], double nu, double k,
     {
         const double h = 0.5 / A;
         const double t = 2.0 * K1 * K1 * I1;
-        d[0] = K1 + h * (t + K2 * (2.0 * e[0] - e[1] - e[2]) / 3.0);
-        d[1] = K1 + h * (t + K2 * (2.0 * e[1] - e[0] - e[2]) / 3.0);
-        d[2] = K1 + h * (t + K2 * (2.0 * e[2] - e[0] - e[1]) / 3.0);
+        const double K23 = K2 * (1.0 / 3.0);
+        d[0] = K1 + h * (t + K23 * (2.0 * e[0] - e[1] - e[2]));
+        d[1] = K1 + h * (t + K23 * (2.0 * e[1] - e[0] - e[2]));
+        d[2] = K1 + h * (t + K23 * (2.0 * e[2] - e[0] - e[1]));
         d[3] = h * K2 * 0.5 * e[3];
         d[4] = h * K2 * 0.5 * e[4];
         d[5] = h * K2 * 0.5 * e[5];
@@ -92,12 +94,11 @@ struct DamageStress3
     double sig[6], dsdw[6];
 };
 
-__device__ __host__ inline void ElasticDamage3(const double e[6], double omega, double E, double nu, int app,
-                                               DamageStress3& o)
+// with the moduli 3K = 3E / (3 - 6 nu) and 2G = 2E / (2 + 2 nu) given
+__device__ __host__ inline void ElasticDamage3K(const double e[6], double omega, double m3K, double m2G, int app,
+                                                DamageStress3& o)
 {
-    const double m3K = 3.0 * E / (3.0 - 6.0 * nu);
-    const double m2G = 2.0 * E / (2.0 + 2.0 * nu);
-    const double eV = (e[0] + e[1] + e[2]) / 3.0;
+    const double eV = (e[0] + e[1] + e[2]) * (1.0 / 3.0);
     const double H = (app == 1 && eV < 0.0) ? 0.0 : 1.0;
     const double sV = (1.0 - omega * H) * m3K * eV;
     const double d2G = (1.0 - omega) * m2G;
@@ -114,6 +115,12 @@ __device__ __host__ inline void ElasticDamage3(const double e[6], double omega, 
     }
     o.alpha_v = m3K * (1.0 - omega * H);
     o.alpha_d = d2G;
+}
+
+__device__ __host__ inline void ElasticDamage3(const double e[6], double omega, double E, double nu, int app,
+                                               DamageStress3& o)
+{
+    ElasticDamage3K(e, omega, 3.0 * E / (3.0 - 6.0 * nu), 2.0 * E / (2.0 + 2.0 * nu), app, o);
 }
 
 }  // namespace nuto_amd
