@@ -1066,6 +1066,17 @@ void BuildConstraints(nuto_ctx* ctx)
 }
 
 // ConstrainedSystemSolver::Solve / SolveTrialState (Solve.cpp:8-78) with delta = NULL / given
+// the node structure of the context's K (every row of a dof node holds the same columns)
+NodeRows KNodeRows(const nuto_ctx* ctx)
+{
+    NodeRows r;
+    r.nNodes = ctx->f[0].nNodes;
+    r.c0 = ctx->f[0].ncomp;
+    r.c1 = ctx->f[1].set ? ctx->f[1].ncomp : 0;
+    r.nU = ctx->f[0].nNodes * ctx->f[0].ncomp;
+    return r;
+}
+
 void SolveConstrained(nuto_ctx* ctx, const double* K, const double* f, const double* rhsOld, const double* rhsNew,
                       bool trial, double* x, const nuto_solver_options* opt, nuto_solve_info* info)
 {
@@ -1080,7 +1091,7 @@ void SolveConstrained(nuto_ctx* ctx, const double* K, const double* f, const dou
     Csr Kc{ctx->nRows, ctx->rowPtr.As<int>(), ctx->col.As<int>(), K};
     Csr Cc{ctx->nRows, ctx->cRow.As<int>(), ctx->cCol.As<int>(), ctx->cVal.As<double>()};
     Csr Ct{m, ctx->ctRow.As<int>(), ctx->ctCol.As<int>(), ctx->ctVal.As<double>()};
-    ReducedSolver rs(ctx->stream, Kc, Cc, Ct, n, m);
+    ReducedSolver rs(ctx->stream, Kc, Cc, Ct, n, m, KNodeRows(ctx));
     Check(rs.Alloc(), "solver buffers");
     DevBuf invd, b, u, g, delta;
     Check(invd.Alloc((size_t)m * 8), "solver buffers");
@@ -1157,7 +1168,7 @@ void QuasistaticStep(nuto_ctx* ctx, double* x, double* kappa, const double* rhsO
     Csr Kc{ctx->nRows, ctx->rowPtr.As<int>(), ctx->col.As<int>(), K.As<double>()};
     Csr Cc{ctx->nRows, ctx->cRow.As<int>(), ctx->cCol.As<int>(), ctx->cVal.As<double>()};
     Csr Ct{m, ctx->ctRow.As<int>(), ctx->ctCol.As<int>(), ctx->ctVal.As<double>()};
-    ReducedSolver rs(ctx->stream, Kc, Cc, Ct, n, m);
+    ReducedSolver rs(ctx->stream, Kc, Cc, Ct, n, m, KNodeRows(ctx));
     Check(rs.Alloc(), "Newton buffers");
 
     auto E = [&](const double* v) -> const double* { return two ? v + nU : nullptr; };

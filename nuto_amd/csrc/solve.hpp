@@ -40,6 +40,53 @@ __global__ void __launch_bounds__(256) kCsrSpmv(int nRows, const int* __restrict
         y[r] = s;
 }
 
+// y = K x for the assembled monolithic K: the C0 field-0 rows of dof node p (rows C0 p + i) and
+// its field-1 row (nU + p, when C1 = 1) hold the same L columns, so one wave per node reads each
+// column index and x entry once for all of them (values stream coalesced, 8 B per nonzero plus
+// 4 / (C0 + C1) B of indices). Lane l sums the entries l, l + 64, ... of each row in order, then
+// the 64 partial sums are added by an xor butterfly (a fixed order).
+template <int C0, int C1>
+__global__ void __launch_bounds__(256) kNodeSpmv(int nNodes, int nU, const int* __restrict__ rowPtr,
+                                                 const int* __restrict__ col, const double* __restrict__ val,
+                                                 const double* __restrict__ x, double* __restrict__ y)
+{
+    constexpr int F = C0 + C1;
+    const int p = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    const int l = (int)(threadIdx.x & 63);
+    double acc[F];
+#pragma unroll
+    for (int i = 0; i < F; i++)
+        acc[i] = 0.0;
+    if (p < nNodes)
+    {
+        const int r0 = rowPtr[C0 * p];
+        const int L = rowPtr[C0 * p + 1] - r0;
+        const int re = C1 ? rowPtr[nU + p] : 0;
+        for (int k = l; k < L; k += 64)
+        {
+            const double xv = x[col[r0 + k]];
+#pragma unroll
+            for (int i = 0; i < C0; i++)
+                acc[i] += val[r0 + i * L + k] * xv;
+            if constexpr (C1 == 1)
+                acc[C0] += val[re + k] * xv;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < F; i++)
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1)
+            acc[i] += __shfl_xor(acc[i], m, 64);
+    if (p < nNodes && l == 0)
+    {
+#pragma unroll
+        for (int i = 0; i < C0; i++)
+            y[C0 * p + i] = acc[i];
+        if constexpr (C1 == 1)
+            y[nU + p] = acc[C0];
+    }
+}
+
 // 1 / diag(C^T K C)_j, diag_j = sum over the entries (a, c_a), (b, c_b) of column j of C of
 // c_a c_b K(a, b) (column j of C = row j of C^T; K's columns are ascending in every row: binary
 // search)
@@ -164,11 +211,17 @@ struct Csr
 
 // Krylov solver on the operator C^T K C. Buffers are allocated per solve; the stream orders every
 // launch, and the host reads the scalars it branches on (one small copy per reduction).
+// node structure of an assembled K (rows per node of field 0 and 1), for the node product
+struct NodeRows
+{
+    int nNodes = 0, nU = 0, c0 = 0, c1 = 0;
+};
+
 class ReducedSolver
 {
 public:
-    ReducedSolver(hipStream_t s, Csr K, Csr C, Csr Ct, long long nFull, int nInd)
-        : mS(s), mK(K), mC(C), mCt(Ct), mN(nFull), mM(nInd)
+    ReducedSolver(hipStream_t s, Csr K, Csr C, Csr Ct, long long nFull, int nInd, NodeRows nodes = NodeRows())
+        : mS(s), mK(K), mC(C), mCt(Ct), mN(nFull), mM(nInd), mNodes(nodes)
     {
     }
 
@@ -208,6 +261,26 @@ public:
     {
         if (mK.nRows == 0)
             return;
+        const NodeRows& nr = mNodes;
+        if (nr.nNodes > 0)
+        {
+            const dim3 grid((unsigned)((nr.nNodes + 3) / 4)), block(256);
+            if (nr.c0 == 3 && nr.c1 == 1)
+            {
+                hipLaunchKernelGGL((kNodeSpmv<3, 1>), grid, block, 0, mS, nr.nNodes, nr.nU, mK.row, mK.col, mK.val, x, y);
+                return;
+            }
+            if (nr.c0 == 3 && nr.c1 == 0)
+            {
+                hipLaunchKernelGGL((kNodeSpmv<3, 0>), grid, block, 0, mS, nr.nNodes, nr.nU, mK.row, mK.col, mK.val, x, y);
+                return;
+            }
+            if (nr.c0 == 2 && nr.c1 == 0)
+            {
+                hipLaunchKernelGGL((kNodeSpmv<2, 0>), grid, block, 0, mS, nr.nNodes, nr.nU, mK.row, mK.col, mK.val, x, y);
+                return;
+            }
+        }
         const long long threads = (long long)mK.nRows * 16;
         hipLaunchKernelGGL(kCsrSpmv<16>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, mS, mK.nRows, mK.row,
                            mK.col, mK.val, x, y);
@@ -365,6 +438,7 @@ private:
     Csr mK, mC, mCt;
     long long mN;
     int mM;
+    NodeRows mNodes;
     double* mHost = nullptr;
 };
 
