@@ -1,6 +1,7 @@
 // C++ host mirror of SimpleAssembler over the nuto_amd C ABI (see nuto_assembler.h).
 #include "nuto_assembler.h"
 
+#include <algorithm>
 #include <atomic>
 #include <memory>
 
@@ -258,6 +259,170 @@ std::vector<double> SimpleAssembler::Eval(const CellGroup& cells, const Momentum
     const nuto_params p{NUTO_MOMENTUM_LINEAR_ELASTIC, dim, f.E, f.nu, f.planeState, 0, 0, 0, 0, 0, 0};
     Setup(cells, {f.disp}, p, dim);
     return RunEval({f.disp}, state, nullptr, cells.NumCells(), dim, field);
+}
+
+// ---- Constraints, QuasistaticSolver ----------------------------------------------------------------
+const std::vector<Equation>& Constraints::Get(const DofType& dof) const
+{
+    static const std::vector<Equation> none;
+    auto it = mEquations.find(dof.Id());
+    return it == mEquations.end() ? none : it->second;
+}
+
+void QuasistaticSolver::Check(int status, const char* caller) const
+{
+    if (status == NUTO_ERR_NO_CONVERGENCE)
+        throw NoConvergence(caller, nuto_last_error(mCtx));
+    if (status != NUTO_OK)
+        throw Exception(caller, nuto_last_error(mCtx));
+}
+
+void QuasistaticSolver::Init(const CellGroup& cells, const std::vector<DofType>& dofs, const nuto_params& p, int dim,
+                             int device)
+{
+    if (nuto_create(&mCtx, device) != NUTO_OK)
+        throw Exception("QuasistaticSolver", "could not create a device context");
+    mDofs = dofs;
+    for (const DofType& d : dofs)
+        if (mDofInfo.Total(d) == 0)
+            throw Exception("QuasistaticSolver", "You did not provide a dof numbering for DofType " + d.GetName());
+    const int nCells = cells.NumCells();
+    Check(nuto_set_cells(mCtx, cells.coordType, nCells, (int)(cells.coords.size() / dim), cells.coords.data(),
+                         cells.conn.data(), cells.cellIds.empty() ? nullptr : cells.cellIds.data()),
+          "QuasistaticSolver::SetCells");
+    for (size_t s = 0; s < dofs.size(); s++)
+    {
+        auto it = cells.dofNumbering.find(dofs[s].Id());
+        if (it == cells.dofNumbering.end())
+            throw Exception("QuasistaticSolver", "the cell group has no numbering for DofType " + dofs[s].GetName());
+        Check(nuto_set_dof_type(mCtx, (int)s, it->second.first, dofs[s].GetNum(), mDofInfo.Total(dofs[s]),
+                                it->second.second.data()),
+              "QuasistaticSolver::SetDofType");
+    }
+    Check(nuto_set_integrand(mCtx, &p), "QuasistaticSolver::SetIntegrand");
+    const bool tet = cells.coordType == NUTO_TET4;
+    Check(nuto_set_rule(mCtx, tet ? 1 : 0, tet ? 1 : 2), "QuasistaticSolver::SetRule");
+    Check(nuto_build_pattern(mCtx, &mNnz, &mRows), "QuasistaticSolver::BuildPattern");
+    Check(nuto_device_alloc(mCtx, &mX, (size_t)mRows * sizeof(double)), "QuasistaticSolver");
+    std::vector<double> zero(mRows, 0.0);
+    Check(nuto_copy_to_device(mCtx, mX, zero.data(), zero.size() * sizeof(double)), "QuasistaticSolver");
+}
+
+QuasistaticSolver::QuasistaticSolver(const CellGroup& cells, GradientDamage& f, DofInfo dofInfo, int device)
+    : mDofInfo(std::move(dofInfo))
+    , mGdm(&f)
+{
+    const nuto_params p{NUTO_GRADIENT_DAMAGE, 3, f.E, f.nu, 0, f.k, f.kappa0, f.beta, f.alpha, f.c,
+                        f.damageApplication};
+    Init(cells, {f.disp, f.eeq}, p, 3, device);
+    const size_t nk = (size_t)cells.NumCells() * (cells.coordType == NUTO_TET4 ? 1 : 8);
+    if (f.kappas.size() != nk)
+        f.kappas.assign(nk, 0.0);
+    mKappaBytes = nk * sizeof(double);
+    Check(nuto_device_alloc(mCtx, &mKappa, mKappaBytes), "QuasistaticSolver");
+    Check(nuto_copy_to_device(mCtx, mKappa, f.kappas.data(), mKappaBytes), "QuasistaticSolver");
+}
+
+QuasistaticSolver::QuasistaticSolver(const CellGroup& cells, const MomentumBalance& f, DofInfo dofInfo, int device)
+    : mDofInfo(std::move(dofInfo))
+{
+    const int dim = cells.coordType == NUTO_QUAD4 ? 2 : 3;
+    const nuto_params p{NUTO_MOMENTUM_LINEAR_ELASTIC, dim, f.E, f.nu, f.planeState, 0, 0, 0, 0, 0, 0};
+    Init(cells, {f.disp}, p, dim, device);
+}
+
+QuasistaticSolver::~QuasistaticSolver()
+{
+    if (mCtx)
+    {
+        if (mX)
+            nuto_device_free(mCtx, mX);
+        if (mKappa)
+            nuto_device_free(mCtx, mKappa);
+        nuto_destroy(mCtx);
+    }
+}
+
+void QuasistaticSolver::SetConstraints(Constraints constraints)
+{
+    mConstraints = std::move(constraints);
+    for (size_t s = 0; s < mDofs.size(); s++)
+    {
+        const std::vector<Equation>& eqs = mConstraints.Get(mDofs[s]);
+        std::vector<int> dep, ptr{0}, dof;
+        std::vector<double> coef;
+        for (const Equation& e : eqs)
+        {
+            dep.push_back(e.dependentDof);
+            for (const auto& t : e.terms)
+            {
+                dof.push_back(t.first);
+                coef.push_back(t.second);
+            }
+            ptr.push_back((int)dof.size());
+        }
+        Check(nuto_set_constraints(mCtx, (int)s, (int)dep.size(), dep.data(), ptr.data(), dof.data(), coef.data()),
+              "QuasistaticSolver::SetConstraints");
+    }
+}
+
+std::vector<double> QuasistaticSolver::Rhs(double t) const
+{
+    std::vector<double> r;
+    for (const DofType& d : mDofs)
+        for (const Equation& e : mConstraints.Get(d))
+            r.push_back(e.rhs(t));
+    return r;
+}
+
+int QuasistaticSolver::DoStep(double newGlobalTime, std::string solverType)
+{
+    const std::vector<double> ro = Rhs(mGlobalTime), rn = Rhs(newGlobalTime);
+    void *dro = nullptr, *drn = nullptr;
+    const size_t nb = std::max<size_t>(ro.size(), 1) * sizeof(double);
+    Check(nuto_device_alloc(mCtx, &dro, nb), "QuasistaticSolver::DoStep");
+    Check(nuto_device_alloc(mCtx, &drn, nb), "QuasistaticSolver::DoStep");
+    nuto_newton_info info{};
+    int status = NUTO_OK;
+    if (!ro.empty())
+    {
+        status = nuto_copy_to_device(mCtx, dro, ro.data(), ro.size() * sizeof(double));
+        if (status == NUTO_OK)
+            status = nuto_copy_to_device(mCtx, drn, rn.data(), rn.size() * sizeof(double));
+    }
+    if (status == NUTO_OK)
+    {
+        const int method = solverType == "EigenConjugateGradient" ? NUTO_SOLVER_CG : NUTO_SOLVER_BICGSTAB;
+        const nuto_newton_options o{mTolerance, 6, 6, {method, mLinearTolerance, 0}};
+        status = nuto_quasistatic_step(mCtx, static_cast<double*>(mX), static_cast<double*>(mKappa),
+                                       static_cast<const double*>(dro), static_cast<const double*>(drn), &o, &info);
+    }
+    const std::string msg = status == NUTO_OK ? "" : nuto_last_error(mCtx);
+    nuto_device_free(mCtx, dro);
+    nuto_device_free(mCtx, drn);
+    if (status == NUTO_ERR_NO_CONVERGENCE)
+        throw NoConvergence("QuasistaticSolver::DoStep", msg);
+    if (status != NUTO_OK)
+        throw Exception("QuasistaticSolver::DoStep", msg);
+    mGlobalTime = newGlobalTime;
+    if (mGdm)
+        Check(nuto_copy_to_host(mCtx, mGdm->kappas.data(), mKappa, mKappaBytes), "QuasistaticSolver::DoStep");
+    return info.iterations;
+}
+
+DofVector<double> QuasistaticSolver::State() const
+{
+    std::vector<double> all(mRows);
+    Check(nuto_copy_to_host(mCtx, all.data(), mX, (size_t)mRows * sizeof(double)), "QuasistaticSolver::State");
+    DofVector<double> x;
+    size_t off = 0;
+    for (const DofType& d : mDofs)
+    {
+        const size_t n = mDofInfo.Total(d);
+        x[d].assign(all.begin() + off, all.begin() + off + n);
+        off += n;
+    }
+    return x;
 }
 
 // ---- MeshGmsh ----------------------------------------------------------------------------------

@@ -8,11 +8,16 @@
 //                      ToEigen(K, {d0, d1}) does (DofMatrixSparseConvertEigen.h:40-67): CSR here
 //   DofInfo            NuTo::DofInfo (DofInfo.h:7-17): numIndependent / numDependent per DofType
 //   SimpleAssembler    BuildVector / BuildMatrix (SimpleAssembler.h:14-21, SimpleAssembler.cpp:38-156)
+//   Constraints        Constraint::Constraints / Equation (Constraints.h, Equation.h): per DofType,
+//                      1 x_dep + sum c x = rhs(t)
+//   QuasistaticSolver  QuasistaticSolver::DoStep (QuasistaticSolver.h:14-90) on the device
+//   MeshGmsh           MeshGmsh (MeshGmsh.h:16-60)
 // NuTo's VectorFunction / MatrixFunction are opaque std::functions; here the integrand is a typed
 // descriptor (GradientDamage, MomentumBalance), and the cell group is the mesh itself (CellGroup).
 // The node values that NuTo reads from its nodes (NodalValueMerger) are passed as a DofVector.
 #pragma once
 
+#include <functional>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -172,6 +177,73 @@ private:
     int mDim = 1;
     std::vector<double> mCoords;
     std::vector<int> mType, mOffsets, mConn;
+};
+
+// Constraint::Equation / Constraints (Equation.h, Constraints.h:24-100): for one DofType,
+// 1 * dependentDof + sum coefficient * dof = rhs(t) (RhsRamp(slope) is rhs(t) = slope * t)
+struct Equation
+{
+    int dependentDof = 0;
+    std::vector<std::pair<int, double>> terms;
+    std::function<double(double)> rhs = [](double) { return 0.0; };
+};
+
+class Constraints
+{
+public:
+    void Add(const DofType& dof, Equation e) { mEquations[dof.Id()].push_back(std::move(e)); }
+    const std::vector<Equation>& Get(const DofType& dof) const;
+
+private:
+    std::map<int, std::vector<Equation>> mEquations;
+};
+
+// NewtonRaphson::NoConvergence (NewtonRaphson.h:12-20)
+class NoConvergence : public Exception
+{
+public:
+    using Exception::Exception;
+};
+
+// QuasistaticSolver (QuasistaticSolver.h:14-90) for one cell group and integrand: the dof vector
+// and the history stay on the device between steps; DoStep runs nuto_quasistatic_step (trial
+// state, Newton-Raphson with line search, history update). The integrand's history
+// (GradientDamage::kappas) is copied back after every converged step, as NuTo updates it in place.
+class QuasistaticSolver
+{
+public:
+    QuasistaticSolver(const CellGroup& cells, GradientDamage& f, DofInfo dofInfo, int device = 0);
+    QuasistaticSolver(const CellGroup& cells, const MomentumBalance& f, DofInfo dofInfo, int device = 0);
+    ~QuasistaticSolver();
+    QuasistaticSolver(const QuasistaticSolver&) = delete;
+    QuasistaticSolver& operator=(const QuasistaticSolver&) = delete;
+
+    void SetConstraints(Constraints constraints);
+    void SetGlobalTime(double globalTime) { mGlobalTime = globalTime; }
+    double GlobalTime() const { return mGlobalTime; }
+    // solverType "EigenConjugateGradient" selects conjugate gradients, anything else BiCGSTAB
+    int DoStep(double newGlobalTime, std::string solverType = "EigenBiCGSTAB");
+    DofVector<double> State() const;  // the converged dof values (mX)
+
+    double mTolerance = 1.e-10;   // QuasistaticSolver::mTolerance
+    double mLinearTolerance = 1.e-12;
+
+private:
+    void Check(int status, const char* caller) const;
+    std::vector<double> Rhs(double t) const;
+    void Init(const CellGroup& cells, const std::vector<DofType>& dofs, const nuto_params& p, int dim, int device);
+
+    nuto_ctx* mCtx = nullptr;
+    DofInfo mDofInfo;
+    std::vector<DofType> mDofs;
+    GradientDamage* mGdm = nullptr;
+    Constraints mConstraints;
+    double mGlobalTime = 0.0;
+    int mRows = 0;
+    long long mNnz = 0;
+    void* mX = nullptr;      // device [u; e]
+    void* mKappa = nullptr;  // device history
+    size_t mKappaBytes = 0;
 };
 
 class SimpleAssembler

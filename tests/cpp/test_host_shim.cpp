@@ -236,6 +236,96 @@ static void LumpedMassAndEvalCase()
     orc_mesh_free(om);
 }
 
+// QuasistaticSolver::DoStep on the homogeneous gradient-damage patch: every boundary node's
+// displacement follows u = t A x (RhsRamp), eeq is free. At t = 1 the exact state is the linear field,
+// eeq = the modified Mises norm of A everywhere and kappa = eeq (the step damages the block).
+static void QuasistaticCase()
+{
+    Mesh m = MakeMesh(NUTO_HEX8, 3, 2, 2);
+    DofType disp("Displacements", 3), eeq("NonlocalEquivalentStrains", 1);
+    DofInfo info;
+    info.numIndependent[disp.Id()] = 3 * m.nDofNodes;
+    info.numIndependent[eeq.Id()] = m.nDofNodes;
+    CellGroup cells;
+    cells.coordType = NUTO_HEX8;
+    cells.coords = m.X;
+    cells.conn = m.C;
+    cells.dofNumbering[disp.Id()] = {NUTO_HEX8, CellDofs(m.ranks, 3)};
+    cells.dofNumbering[eeq.Id()] = {NUTO_HEX8, CellDofs(m.ranks, 1)};
+    GradientDamage gdm{disp, eeq, 4.0, 30000.0, 0.2, 10.0, 1e-4, 100.0, 0.99, 0, {}};
+    const double A[3][3] = {{2e-4, 0.5e-4, 0.0}, {0.5e-4, -0.4e-4, 0.3e-4}, {0.0, 0.3e-4, -0.6e-4}};
+    std::vector<double> xs(3 * m.nDofNodes);
+    for (int c = 0; c < m.nCells; c++)
+        for (int a = 0; a < 8; a++)
+            for (int d = 0; d < 3; d++)
+                xs[3 * m.ranks[c * 8 + a] + d] = m.X[3 * m.C[c * 8 + a] + d];
+    const double hi[3] = {3.0, 2.0, 2.0};
+    Constraints cons;
+    std::vector<double> field(3 * m.nDofNodes);
+    for (int r = 0; r < m.nDofNodes; r++)
+    {
+        bool boundary = false;
+        for (int d = 0; d < 3; d++)
+            boundary = boundary || std::fabs(xs[3 * r + d]) < 1e-12 || std::fabs(xs[3 * r + d] - hi[d]) < 1e-12;
+        for (int i = 0; i < 3; i++)
+        {
+            field[3 * r + i] = A[i][0] * xs[3 * r] + A[i][1] * xs[3 * r + 1] + A[i][2] * xs[3 * r + 2];
+            if (boundary)
+            {
+                const double v = field[3 * r + i];
+                cons.Add(disp, Equation{3 * r + i, {}, [v](double t) { return v * t; }});
+            }
+        }
+    }
+    QuasistaticSolver solver(cells, gdm, info);
+    solver.mLinearTolerance = 1e-14;
+    solver.SetConstraints(cons);
+    const int iterations = solver.DoStep(1.0);
+    DofVector<double> x = solver.State();
+    // modified Mises norm of A: K1 I1 + sqrt(K1^2 I1^2 + K2 J2), J2 = dev(A) : dev(A) / 2
+    const double nu = 0.2, k = 10.0, K1 = (k - 1.0) / (2.0 * k * (1.0 - 2.0 * nu)), K2 = 3.0 / (k * (1 + nu) * (1 + nu));
+    const double I1 = A[0][0] + A[1][1] + A[2][2];
+    double J2 = 0.0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+        {
+            const double dv = A[i][j] - (i == j ? I1 / 3.0 : 0.0);
+            J2 += 0.5 * dv * dv;
+        }
+    const double e = K1 * I1 + std::sqrt(K1 * K1 * I1 * I1 + K2 * J2);
+    EXPECT(iterations >= 1 && e > 1e-4, "nonlinear damaging step");
+    for (int i = 0; i < 3 * m.nDofNodes; i++)
+        if (std::fabs(x[disp][i] - field[i]) > 1e-10 * 6e-4)
+        {
+            EXPECT(false, "displacements of the patch");
+            break;
+        }
+    for (int r = 0; r < m.nDofNodes; r++)
+        if (std::fabs(x[eeq][r] - e) > 1e-9 * e)
+        {
+            EXPECT(false, "nonlocal strains of the patch");
+            break;
+        }
+    for (double kap : gdm.kappas)
+        if (std::fabs(kap - e) > 1e-9 * e)
+        {
+            EXPECT(false, "history of the patch");
+            break;
+        }
+    // a step that cannot converge throws NoConvergence and keeps the state
+    solver.mTolerance = 1e-30;
+    bool threw = false;
+    try
+    {
+        solver.DoStep(2.0);
+    }
+    catch (const NoConvergence&)
+    {
+        threw = true;
+    }
+    EXPECT(threw && solver.GlobalTime() == 1.0, "NoConvergence keeps the state");
+}
+
 int main()
 {
     try
@@ -243,6 +333,7 @@ int main()
         GradientDamageCase();
         MomentumBalanceCase();
         LumpedMassAndEvalCase();
+        QuasistaticCase();
     }
     catch (const std::exception& e)
     {
