@@ -309,7 +309,8 @@ void LaunchTwoPhaseT(nuto_ctx* ctx, const RowArgs& a, bool wr, int c0, int c1, i
         ctx->maxAdj = m;
     }
     const size_t lds = (size_t)3 * 3 * ctx->maxAdj * sizeof(double);
-    if (lds <= 48 * 1024)
+    const char* lane = std::getenv("NUTO_AMD_GATHER_LANE");  // 1: the lane-per-node gather (tests)
+    if (lds <= 48 * 1024 && !(lane && lane[0] == '1'))
     {
         // one wave per node, rows summed in LDS
         if (wr)
