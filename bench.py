@@ -268,16 +268,36 @@ def cpu_baseline(n, threads, budget_s=30.0):
     u = S.uniform(S.SEED_U, P.ndof[0], -5e-3, 5e-3)
     e = S.uniform(S.SEED_E, P.ndof[1], 0.0, 5e-4)
     k = S.uniform(S.SEED_KAPPA, P.ncells * 8, 0.0, 3e-4)
-    times = []
-    t_start = time.time()
-    while len(times) < 5 and (time.time() - t_start) < budget_s:
-        t0 = time.perf_counter()
-        P.assemble(u, e, k, want_r=True, want_k=True, nthreads=threads)
-        times.append(time.perf_counter() - t0)
-    t = float(np.median(times))
+    def median_s(want_r, want_k, nth, runs=5, budget=budget_s):
+        times = []
+        t_start = time.time()
+        while len(times) < runs and (time.time() - t_start) < budget:
+            t0 = time.perf_counter()
+            P.assemble(u, e, k, want_r=want_r, want_k=want_k, nthreads=nth)
+            times.append(time.perf_counter() - t0)
+        return float(np.median(times)), len(times)
+
+    t, runs = median_s(True, True, threads)
+    # BuildVector and BuildMatrix separately (the reference times them as two calls), and one core
+    tv, _ = median_s(True, False, threads, 3, budget_s / 4)
+    tm, _ = median_s(False, True, threads, 3, budget_s / 4)
+    t1, _ = median_s(True, True, 1, 1, budget_s)
     return {"value": P.ncells / t, "unit": "cells/s", "cores": threads, "kind": "port",
-            "sample": f"hex8 gradient damage {n}x{n}x{n} cells (1 mm spacing), R+K, median of {len(times)} runs "
-                      f"of the oracle with {threads} OpenMP threads"}
+            "sample": f"hex8 gradient damage {n}x{n}x{n} cells (1 mm spacing), R+K, median of {runs} runs "
+                      f"of the oracle with {threads} OpenMP threads",
+            "build_vector_cells_per_s": P.ncells / tv, "build_matrix_cells_per_s": P.ncells / tm,
+            "one_core_cells_per_s": P.ncells / t1, "cpu_model": _cpu_model()}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def main():
