@@ -127,7 +127,10 @@ int nuto_pattern_device(nuto_ctx* ctx, const int** row_ptr, const int** col);
  * u (slot 0) and e (slot 1, NULL for single-field integrands) and history kappa (NULL if unused). */
 int nuto_build_vector(nuto_ctx* ctx, const double* u, const double* e, const double* kappa, double* R);
 
-/* SimpleAssembler::BuildMatrix (SimpleAssembler.cpp:100-156): K values (length nnz) in the pattern. */
+/* SimpleAssembler::BuildMatrix (SimpleAssembler.cpp:100-156): K values (length nnz) in the pattern.
+ * 3D momentum-balance groups (linear elasticity, local damage) assemble in two phases through a
+ * staging buffer the context keeps (8 * n_cells * ndof_per_cell^2 bytes: 28.8 KB per hex20 cell,
+ * 4.6 KB per hex8 cell); NUTO_AMD_TWO_PHASE=0 in the environment selects the single-pass kernel. */
 int nuto_build_matrix(nuto_ctx* ctx, const double* u, const double* e, const double* kappa, double* K);
 
 /* BuildVector and BuildMatrix of the same state in one pass over the cells. */

@@ -7,6 +7,13 @@ The oracle restates NuTo's assembly path (SimpleAssembler.cpp:38-156 and the fil
 top of nuto_oracle.cpp). Element types: 0 quad4, 1 hex8, 2 tet4, 3 tet10, 4 hex20.
 Integrand kinds: 0 MomentumBalance + LinearElastic, 1 GradientDamage<3>,
 2 MomentumBalance<3> + LocalIsotropicDamage<3>.
+
+Beyond the assembly: lumped mass and IP fields (Problem.lumped_mass / eval_ip), the unit
+constraint matrix and the constrained direct solve (unit_constraint_matrix, solve_constrained),
+and the quasistatic load step (quasistatic_step: QuasistaticSolver::DoStep with NewtonRaphson and
+LineSearch, in Python over the C restatement). Their pins: tests/test_cell_outputs_oracle.py,
+tests/test_constraints_oracle.py (the reference's known-answer constraint matrix, a patch test)
+and tests/test_newton_oracle.py (the homogeneous damage patch in closed form).
 """
 import ctypes
 import os
