@@ -684,6 +684,9 @@ class Batch:
         return self._torch.empty(max(n, 1), dtype=self._torch.float64, device="cuda")
 
     def assemble(self, u, e=None, kappa=None, R=None, K=None, mode=-1):
+        """R and K of every mesh (nuto_batch_assemble). mode -1: the tuned launch replayed from a
+        HIP graph; 0..2 fused row-owner launch (64 / 128 / 256 nodes per workgroup); 3 row-owner per
+        element type; 4 row-owner per mesh; 5 two-phase per element type; 6 two-phase per mesh."""
         R = self._new(self.sizes["R"]) if R is None else R
         K = self._new(self.sizes["K"]) if K is None else K
         self._enter()
