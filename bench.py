@@ -321,9 +321,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: this build measures the single-GPU assembly kernel (asked for "
-              f"--gpus {args.gpus}). Multi-GPU sharding is described in DESIGN.md and not launched.",
-              file=sys.stderr)
+        print(f"bench.py runs on one GPU: it measures the single-GPU assembly (asked for --gpus {args.gpus}). "
+              "Several GPUs would each own a range of dof nodes plus one layer of ghost cells; that sharding "
+              "is not launched.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
