@@ -67,6 +67,30 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes_read": 8 * n, "copy_calibration_bytes_written": 8 * n}), flush=True)
 
 
+def copy_bandwidth(mib=2048, reps=10):
+    """Measured device-to-device copy bandwidth (read + write bytes / time, median of reps), the
+    practical HBM ceiling to set beside the 8 TB/s peak."""
+    import numpy as np
+    import torch
+
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for a, z in evs:
+        a.record()
+        dst.copy_(src)
+        z.record()
+    torch.cuda.synchronize()
+    ms = float(np.median([a.elapsed_time(z) for a, z in evs]))
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2.0 * 8 * n / (ms * 1e-3) / 1e9
+
+
 def solve_bench(ctx, X, C, ranks, nn, K, R, iters, length):
     """ConstrainedSystemSolver::SolveTrialState on the assembled system: bottom face clamped, the top
     face's vertical displacement stepped by 1e-3 mm, eeq free; BiCGSTAB with the diagonal
@@ -399,7 +423,8 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(n),
-                     "kernel_ms": kernel_ms, "algorithmic_bytes": bytes_alg},
+                     "kernel_ms": kernel_ms, "algorithmic_bytes": bytes_alg,
+                     "measured_copy_GBps": copy_bandwidth()},
         "setup_s": setup_s,
     }
     flops = measured_counter(n, "fp64_flops_per_launch")
