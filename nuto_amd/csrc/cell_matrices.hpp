@@ -3,8 +3,9 @@
 //
 // Phase 1 (kCellMatrices): Cell::Integrate (Cell.h:105-119) of one cell per lane group. Lane j of
 // the group owns column j of the cell matrix K_e (dof j = 3 b + c of dof node b) and its R_e entry.
-// Per IP the group computes J, J^-1 and its own dN_b/dX, shares the gradients through LDS, forms
-// the strain, the law's stress and tangent (the same closed forms as the row-owner kernel), then
+// Before the IP loop the group computes, for all IPs at once, J = X^T dN/dxi and the reference
+// displacement gradient U^T dN/dxi (one 3x3 entry per lane and pass). Per IP each lane then forms
+// J^-1, its own dN_b/dX (shared through LDS), the strain from (U^T dN/dxi) J^-1, the law's stress and tangent (the same closed forms as the row-owner kernel), then
 // the stress of a unit displacement of its dof, dsig_j = D B_j (D with the local-damage rank-one
 // term t m^T), and accumulates column j: K_e(3 a + i, j) += (B_a^T dsig_j)_i for every node a.
 // The cell's state is computed once per IP per group (the row-owner form recomputes it for every
@@ -35,6 +36,9 @@ __global__ void __launch_bounds__(64) kCellMatrices(const RowArgs A, int c0, int
     __shared__ __align__(16) double sX[CPW][NNC * 4];
     __shared__ __align__(16) double sU[CPW][NND * 4];
     __shared__ __align__(16) double sG[CPW][NND * 4];
+    // per IP: J = X^T dNc (9) and the reference displacement gradient U^T dNd (9), computed once per
+    // cell by the group's lanes together instead of by every lane
+    __shared__ __align__(16) double sJG[CPW][NIP * 18];
 
     const int lane = (int)threadIdx.x;
     const int grp = lane / ND < CPW ? lane / ND : CPW - 1;  // lanes past the last group shadow it
@@ -55,6 +59,30 @@ __global__ void __launch_bounds__(64) kCellMatrices(const RowArgs A, int c0, int
         const int q = A.dconn[(size_t)cell * NND + b];
         sU[grp][b * 4 + c] = A.u[(size_t)q * 3 + c];
     }
+    __syncthreads();
+    if (act)
+        for (int tt = j; tt < NIP * 18; tt += ND)
+        {
+            const int ip = tt / 18, k = tt - 18 * ip;
+            const double* dNc = A.tab + ip * S + 1;
+            const double* dNd = dNc + NNC * 3 + NND;
+            double v = 0.0;
+            if (k < 9)
+            {
+                const int i = k / 3, m = k - 3 * i;
+#pragma unroll
+                for (int n = 0; n < NNC; n++)
+                    v += sX[grp][n * 4 + i] * dNc[n * 3 + m];
+            }
+            else
+            {
+                const int i = (k - 9) / 3, m = k - 9 - 3 * i;
+#pragma unroll
+                for (int n = 0; n < NND; n++)
+                    v += sU[grp][n * 4 + i] * dNd[n * 3 + m];
+            }
+            sJG[grp][tt] = v;
+        }
     __syncthreads();
 
     const LawParams& P = A.P;
@@ -79,20 +107,14 @@ __global__ void __launch_bounds__(64) kCellMatrices(const RowArgs A, int c0, int
     {
         const double* t = A.tab + ip * S;
         const double w = t[0];
-        const double* dNc = t + 1;
-        const double* dNd = dNc + NNC * 3 + NND;
+        const double* dNd = t + 1 + NNC * 3 + NND;
+        const double* jg = &sJG[grp][ip * 18];
         double J[3][3];
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
             for (int m = 0; m < 3; m++)
-            {
-                double s = 0.0;
-#pragma unroll
-                for (int k = 0; k < NNC; k++)
-                    s += sX[grp][k * 4 + i] * dNc[k * 3 + m];
-                J[i][m] = s;
-            }
+                J[i][m] = jg[3 * i + m];
         double inv[3][3];
         const double detJ = Inverse<3>(J, inv);
         const double s = detJ * w;
@@ -106,20 +128,15 @@ __global__ void __launch_bounds__(64) kCellMatrices(const RowArgs A, int c0, int
         }
         __syncthreads();
 
-        // strain (Voigt [xx, yy, zz, yz, zx, xy], engineering shear)
-        double eps[6] = {0, 0, 0, 0, 0, 0};
+        // strain (Voigt [xx, yy, zz, yz, zx, xy], engineering shear) from the displacement gradient
+        // H = (U^T dNd) J^-1, H_ik = du_i / dx_k
+        double H[3][3];
 #pragma unroll
-        for (int a = 0; a < NND; a++)
-        {
-            const double gx = sG[grp][a * 4], gy = sG[grp][a * 4 + 1], gz = sG[grp][a * 4 + 2];
-            const double ux = sU[grp][a * 4], uy = sU[grp][a * 4 + 1], uz = sU[grp][a * 4 + 2];
-            eps[0] += gx * ux;
-            eps[1] += gy * uy;
-            eps[2] += gz * uz;
-            eps[3] += gz * uy + gy * uz;
-            eps[4] += gz * ux + gx * uz;
-            eps[5] += gy * ux + gx * uy;
-        }
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                H[i][k] = jg[9 + 3 * i] * inv[0][k] + jg[9 + 3 * i + 1] * inv[1][k] + jg[9 + 3 * i + 2] * inv[2][k];
+        const double eps[6] = {H[0][0], H[1][1], H[2][2], H[1][2] + H[2][1], H[0][2] + H[2][0], H[0][1] + H[1][0]};
 
         // law: stress, isotropic tangent (c1 tr + 2 c2 eps, shear c2 gamma), rank-one damage term
         double sig[6], c1, c2;
