@@ -208,16 +208,14 @@ __global__ void __launch_bounds__(64) kCellMatrices(const RowArgs A, int c0, int
                 ds6[i] += t6[i] * mb;
         }
 
-        // column j: K_e(3 a + i, j) += (B_a^T dsig_j)_i
+        // column j: K_e(3 a + i, j) += (B_a^T dsig_j)_i, three FMAs into each entry
 #pragma unroll
         for (int a = 0; a < NND; a++)
         {
-            const double ga[3] = {sG[grp][a * 4], sG[grp][a * 4 + 1], sG[grp][a * 4 + 2]};
-            double o[3];
-            BTv3(ga, ds6, o);
-            kcol[a * 3 + 0] += o[0];
-            kcol[a * 3 + 1] += o[1];
-            kcol[a * 3 + 2] += o[2];
+            const double gx = sG[grp][a * 4], gy = sG[grp][a * 4 + 1], gz = sG[grp][a * 4 + 2];
+            kcol[a * 3 + 0] = fma(gx, ds6[0], fma(gz, ds6[4], fma(gy, ds6[5], kcol[a * 3 + 0])));
+            kcol[a * 3 + 1] = fma(gy, ds6[1], fma(gz, ds6[3], fma(gx, ds6[5], kcol[a * 3 + 1])));
+            kcol[a * 3 + 2] = fma(gz, ds6[2], fma(gy, ds6[3], fma(gx, ds6[4], kcol[a * 3 + 2])));
         }
         __syncthreads();  // sG is rewritten by the next IP
     }
