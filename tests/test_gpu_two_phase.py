@@ -74,3 +74,15 @@ def test_two_phase_with_material_fields():
                 os.environ["NUTO_AMD_TWO_PHASE"] = old
         check_values(R.cpu().numpy(), ref["R"], ref["R_bound"], f"R {env}")
         check_values(K.cpu().numpy(), ref["val"], ref["bound"], f"K {env}")
+
+
+@pytest.mark.parametrize("ct,dt", [(N.TET4, N.TET10), (N.HEX8, N.HEX20)])
+def test_two_phase_unilateral_damage(ct, dt):
+    """LinearElasticDamage with UNILATERAL application (H = 0 in compression) in the cell kernel."""
+    X, C = make_mesh(ct, (2, 2, 2), (2.0, 2.0, 2.0), jitter=0.1, seed=9)
+    p = Pair(ct, dt, X, C, *ld_params(N.UNILATERAL), False)
+    u, _, kap = p.inputs(u_range=6e-4, k_range=2e-4)
+    ref = p.orc.assemble(u, None, kap)
+    R, K = assemble(p, u, kap, {})
+    check_values(R, ref["R"], ref["R_bound"], "R")
+    check_values(K, ref["val"], ref["bound"], "K")
