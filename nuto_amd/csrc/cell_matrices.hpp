@@ -39,6 +39,8 @@ __global__ void __launch_bounds__(64) kCellMatrices(const RowArgs A, int c0, int
     // per IP: J = X^T dNc (9) and the reference displacement gradient U^T dNd (9), computed once per
     // cell by the group's lanes together instead of by every lane
     __shared__ __align__(16) double sJG[CPW][NIP * 18];
+    // per IP: J^-1 (9) and detJ * w
+    __shared__ double sInv[CPW][NIP * 10];
 
     const int lane = (int)threadIdx.x;
     const int grp = lane / ND < CPW ? lane / ND : CPW - 1;  // lanes past the last group shadow it
@@ -84,6 +86,25 @@ __global__ void __launch_bounds__(64) kCellMatrices(const RowArgs A, int c0, int
             sJG[grp][tt] = v;
         }
     __syncthreads();
+    if (act)
+        for (int ip = j; ip < NIP; ip += ND)
+        {
+            const double* jg = &sJG[grp][ip * 18];
+            double Jm[3][3], inv[3][3];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int m = 0; m < 3; m++)
+                    Jm[i][m] = jg[3 * i + m];
+            const double detJ = Inverse<3>(Jm, inv);
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int m = 0; m < 3; m++)
+                    sInv[grp][ip * 10 + 3 * i + m] = inv[i][m];
+            sInv[grp][ip * 10 + 9] = detJ * A.tab[ip * S];
+        }
+    __syncthreads();
 
     const LawParams& P = A.P;
     double E = P.E, k0 = P.kappa0, beta = P.beta;
@@ -105,19 +126,15 @@ __global__ void __launch_bounds__(64) kCellMatrices(const RowArgs A, int c0, int
 #pragma unroll 1
     for (int ip = 0; ip < NIP; ip++)
     {
-        const double* t = A.tab + ip * S;
-        const double w = t[0];
-        const double* dNd = t + 1 + NNC * 3 + NND;
+        const double* dNd = A.tab + ip * S + 1 + NNC * 3 + NND;
         const double* jg = &sJG[grp][ip * 18];
-        double J[3][3];
+        double inv[3][3];
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
             for (int m = 0; m < 3; m++)
-                J[i][m] = jg[3 * i + m];
-        double inv[3][3];
-        const double detJ = Inverse<3>(J, inv);
-        const double s = detJ * w;
+                inv[i][m] = sInv[grp][ip * 10 + 3 * i + m];
+        const double s = sInv[grp][ip * 10 + 9];
         double gb[3];
         Grad<3, NND>(dNd, b, inv, gb);
         if (act && c == 0)
