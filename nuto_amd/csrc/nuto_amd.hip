@@ -288,6 +288,21 @@ void EnsureStage(nuto_ctx* ctx, int nd)
         Check(ctx->reStage.Alloc(re), "cell vector staging");
 }
 
+// the staging is there or could be allocated (a failed allocation clears the error state)
+bool TryStage(nuto_ctx* ctx, int nd)
+{
+    try
+    {
+        EnsureStage(ctx, nd);
+        return true;
+    }
+    catch (const Failure&)
+    {
+        (void)hipGetLastError();
+        return false;
+    }
+}
+
 // Two-phase assembly of the cells [c0, c1) and the rows of the dof nodes [p0, p1) (the rows'
 // incident cells must lie in [c0, c1): a whole mesh of a batch, or the whole group).
 template <int KIND, int NNC, int NND, int NIP>
@@ -339,7 +354,8 @@ void LaunchRowsT(nuto_ctx* ctx, const RowArgs& a, bool wr, bool wk)
         return;
     if constexpr (DIM == 3 && KIND != kGradientDamage && NND * 3 <= 64)
     {
-        if (wk && !TwoPhaseOff())
+        // the single-pass kernel serves when the staging does not fit in device memory
+        if (wk && !TwoPhaseOff() && TryStage(ctx, NND * 3))
         {
             LaunchTwoPhaseT<KIND, NNC, NND, NIP>(ctx, a, wr, 0, a.nCells, 0, n, ctx->stream);
             SyncCheck(ctx, "assembly kernel");
@@ -2032,7 +2048,14 @@ int nuto_batch_autotune(nuto_batch* b, const double* u, const double* e, const d
         for (int mode = 0; mode < kBatchModes; mode++)
         {
             double v = -1.0;
-            if (!(IsGdm(b) && (mode < 3 || mode >= 5)))
+            bool usable = !(IsGdm(b) && (mode < 3 || mode >= 5));
+            if (usable && mode >= 5)
+            {
+                // the two-phase modes need the staging of every group; without it they are not candidates
+                for (BatchGroup& g : b->groups)
+                    usable = usable && TryStage(g.ctx, 3 * NodesOf(g.dt));
+            }
+            if (usable)
             {
                 LaunchBatch(b, mode, u, e, kappa, R, K);  // warm-up
                 std::vector<float> rep;
