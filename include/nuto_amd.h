@@ -214,12 +214,12 @@ int nuto_solve_trial_state(nuto_ctx* ctx, const double* K, const double* f, cons
 
 /* One load step of QuasistaticSolver::DoStep (QuasistaticSolver.cpp:116-147) on the device: the
  * trial state x - SolveTrialState(K(x), R(x), rhs_old, rhs_new) from the last converged state
- * (TrialState, :52-68), then NewtonRaphson::Solve (NewtonRaphson.h:62-94) with LineSearch
- * (LineSearch.h:36-62: x = x0 - alpha dx, alpha halved up to max_line_search_steps times, accepted
+ * (TrialState, :51-66), then NewtonRaphson::Solve (NewtonRaphson.h:62-94) with LineSearch
+ * (LineSearch.h:39-63: x = x0 - alpha dx, alpha halved up to max_line_search_steps times, accepted
  * once |R| < (1 - alpha) |R0|) or, with max_line_search_steps == 0, NoLineSearch; the residual norm
- * is |C^T R| (QuasistaticSolver::Norm, :86-92). On convergence the history is updated
- * (UpdateHistory, :80-83) and x is overwritten; otherwise x and kappa are left unchanged and the
- * call returns NUTO_ERR_NO_CONVERGENCE, which AdaptiveSolve (AdaptiveSolve.cpp:18-63) answers by
+ * is |C^T R| (QuasistaticSolver::Norm, :84-90). On convergence the history is updated
+ * (UpdateHistory, :79-82) and x is overwritten; otherwise x and kappa are left unchanged and the
+ * call returns NUTO_ERR_NO_CONVERGENCE, which AdaptiveSolve (AdaptiveSolve.cpp:18-67) answers by
  * halving the time step. x: the monolithic dof vector [u; e] (n_rows, device); kappa: the history
  * (device, NULL for linear elasticity); rhs_old / rhs_new as for nuto_solve_trial_state. */
 typedef struct nuto_newton_options

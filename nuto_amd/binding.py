@@ -361,7 +361,7 @@ def morton_order(coords):
 
 def adaptive_solve(do_step, t_end, t_start=0.0, post_process=None, dt=0.1, dt_max=0.1, dt_min=1e-6,
                    increase_factor=1.5, decrease_factor=0.5):
-    """AdaptiveSolve::Solve (AdaptiveSolve.cpp:18-63): calls do_step(t + dt), which returns the Newton
+    """AdaptiveSolve::Solve (AdaptiveSolve.cpp:18-67): calls do_step(t + dt), which returns the Newton
     iterations (Context.quasistatic_step's "iterations") or raises NutoError with code NO_CONVERGENCE;
     on failure dt is halved (below dt_min: an error), after a step with fewer than 3 iterations dt
     grows by increase_factor up to dt_max, and the last step ends exactly at t_end. post_process(t)

@@ -1155,7 +1155,7 @@ void SolveConstrained(nuto_ctx* ctx, const double* K, const double* f, const dou
 }
 
 // QuasistaticSolver::DoStep (QuasistaticSolver.cpp:116-147) with NewtonRaphson::Solve
-// (NewtonRaphson.h:62-94) and LineSearch / NoLineSearch (LineSearch.h:36-62,78-88). Every vector
+// (NewtonRaphson.h:62-94) and LineSearch / NoLineSearch (LineSearch.h:39-63,79-88). Every vector
 // stays on the device; the host reads one norm per residual evaluation.
 void QuasistaticStep(nuto_ctx* ctx, double* x, double* kappa, const double* rhsOld, const double* rhsNew,
                      const nuto_newton_options* o, nuto_newton_info* info)

@@ -409,9 +409,9 @@ class NoConvergence(RuntimeError):
 def quasistatic_step(problem, n_dofs, equations, x, kappa=None, rhs_old=None, rhs_new=None, tol=1e-10, max_iter=6,
                      line_search_steps=6):
     """QuasistaticSolver::DoStep (QuasistaticSolver.cpp:116-147) on the oracle: the trial state
-    x - SolveTrialState(K(x), R(x)) (TrialState, :52-68), NewtonRaphson::Solve (NewtonRaphson.h:62-94)
-    with LineSearch (LineSearch.h:36-62) or, for line_search_steps == 0, NoLineSearch (:78-88), the
-    norm |C^T R| (QuasistaticSolver::Norm, :86-92), and the history update on convergence
+    x - SolveTrialState(K(x), R(x)) (TrialState, :51-66), NewtonRaphson::Solve (NewtonRaphson.h:62-94)
+    with LineSearch (LineSearch.h:39-63) or, for line_search_steps == 0, NoLineSearch (:78-88), the
+    norm |C^T R| (QuasistaticSolver::Norm, :84-90), and the history update on convergence
     (GradientDamage::Update, kappa = max(kappa, eeq), read back through Cell::Eval of kappa).
     The linear systems are solved directly (solve_constrained). x is the monolithic [u; e] vector,
     kappa the history (cell + ip * n_cells). Returns (x, kappa, iterations); raises NoConvergence."""

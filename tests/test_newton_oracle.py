@@ -1,5 +1,5 @@
 """The oracle's load step (QuasistaticSolver::DoStep: trial state, Newton-Raphson with line search,
-history update; QuasistaticSolver.cpp:116-147, NewtonRaphson.h:62-94, LineSearch.h:36-62), pinned
+history update; QuasistaticSolver.cpp:116-147, NewtonRaphson.h:62-94, LineSearch.h:39-63), pinned
 by closed forms.
 
 Homogeneous gradient-damage patch: every boundary node's displacement follows u = A x and the
@@ -95,7 +95,7 @@ def test_no_convergence():
 
 
 def test_adaptive_solve_time_stepping():
-    """AdaptiveSolve::Solve (AdaptiveSolve.cpp:18-63) with a step that fails above dt = 0.3 and
+    """AdaptiveSolve::Solve (AdaptiveSolve.cpp:18-67) with a step that fails above dt = 0.3 and
     reports 1 iteration: dt shrinks by 0.5 on failure, grows by 1.5 (to dt_max) after easy steps,
     and the last step is cut to end at t_end."""
     calls, post = [], []
