@@ -8,7 +8,7 @@
 //                      ToEigen(K, {d0, d1}) does (DofMatrixSparseConvertEigen.h:40-67): CSR here
 //   DofInfo            NuTo::DofInfo (DofInfo.h:7-17): numIndependent / numDependent per DofType
 //   SimpleAssembler    BuildVector / BuildMatrix (SimpleAssembler.h:14-21, SimpleAssembler.cpp:38-156)
-//   Constraints        Constraint::Constraints / Equation (Constraints.h, Equation.h): per DofType,
+//   Constraints        Constraint::Constraints / Equation (Constraints.h:47-130, Equation.h:12-66): per DofType,
 //                      1 x_dep + sum c x = rhs(t)
 //   QuasistaticSolver  QuasistaticSolver::DoStep (QuasistaticSolver.h:14-90) on the device
 //   MeshGmsh           MeshGmsh (MeshGmsh.h:16-60)
@@ -179,7 +179,7 @@ private:
     std::vector<int> mType, mOffsets, mConn;
 };
 
-// Constraint::Equation / Constraints (Equation.h, Constraints.h:24-100): for one DofType,
+// Constraint::Equation / Constraints (Equation.h:12-66, Constraints.h:47-130): for one DofType,
 // 1 * dependentDof + sum coefficient * dof = rhs(t) (RhsRamp(slope) is rhs(t) = slope * t)
 struct Equation
 {
