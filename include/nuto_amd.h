@@ -72,6 +72,8 @@ int nuto_mesh_structured(int coord_type, int nx, int ny, int nz, double lx, doub
 /* First-touch dof layer: AddDofInterpolation (MeshFemDofConvert.cpp:164-210) numbered by
  * DofNumbering::Build (DofNumbering.cpp:32-58). Writes per-cell dof node ranks
  * (n_cells x nodes(dof_type)); the dof numbers are rank * ncomp + component.
+ * Coordinate elements: quad4, hex8, tet4 under any dof element of their shape, or tet10 under
+ * tet10 (isoparametric: the dof nodes are the coordinate nodes).
  * Pass cell_ranks == NULL to query n_dof_nodes only. */
 int nuto_dof_layer(int coord_type, int n_cells, const int* coord_conn, int dof_type, int* cell_ranks,
                    int* n_dof_nodes);
@@ -89,7 +91,8 @@ int nuto_synchronize(nuto_ctx* ctx);
 
 /* The cell group: Group<CellInterface> of Cells built by CellStorage::AddCells over a MeshFem
  * (CellStorage.cpp:6-16, Cell.h:14-30). coords: n_coord_nodes x dim (node-major),
- * coord_conn: n_cells x nodes(coord_type), cell_ids: Cell::Id per cell (NULL: 0..n_cells-1). */
+ * coord_conn: n_cells x nodes(coord_type), cell_ids: Cell::Id per cell (NULL: 0..n_cells-1).
+ * coord_type: NUTO_QUAD4, NUTO_HEX8, NUTO_TET4 or NUTO_TET10. */
 int nuto_set_cells(nuto_ctx* ctx, int coord_type, int n_cells, int n_coord_nodes, const double* coords,
                    const int* coord_conn, const int* cell_ids);
 

@@ -106,9 +106,37 @@ bool DofLayer(int coordType, int nCells, const int* coordConn, int dofType, std:
     }
     const int nnC = NodesOf(coordType), nnD = NodesOf(dofType);
     const int nVert = FamilyOf(coordType) == 0 ? 4 : (FamilyOf(coordType) == 1 ? 8 : 4);
+    if (coordType == dofType && nnC != nVert)
+    {
+        // isoparametric quadratic layer (tet10 over tet10): every dof node is a coordinate node;
+        // numbered by first touch in cell order, as AddDofInterpolation matches the interpolated
+        // node coordinates (MeshFemDofConvert.cpp:164-210)
+        ranks.assign((size_t)nCells * nnD, -1);
+        int maxNode = -1;
+        for (size_t i = 0; i < (size_t)nCells * nnC; i++)
+        {
+            if (coordConn[i] < 0)
+            {
+                err = "negative node index in the connectivity";
+                return false;
+            }
+            maxNode = std::max(maxNode, coordConn[i]);
+        }
+        std::vector<int> rankOf((size_t)maxNode + 1, -1);
+        int next = 0;
+        for (size_t i = 0; i < (size_t)nCells * nnC; i++)
+        {
+            int& r = rankOf[coordConn[i]];
+            if (r < 0)
+                r = next++;
+            ranks[i] = r;
+        }
+        nNodes = next;
+        return true;
+    }
     if (nnC != nVert)
     {
-        err = "coordinate elements must be linear (quad4, hex8, tet4)";
+        err = "coordinate elements must be linear (quad4, hex8, tet4) or the dof element itself";
         return false;
     }
     // for each dof node: either a vertex (index < nVert) or the midpoint of a vertex pair

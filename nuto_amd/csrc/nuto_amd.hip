@@ -443,11 +443,13 @@ void Dispatch(nuto_ctx* ctx, const RowArgs& a, bool wr, bool wk, int mode, doubl
     NUTO_CASE(kLinearElastic, 3, kHex8, 8, kHex8, 8, 8, 8)
     NUTO_CASE(kLinearElastic, 3, kTet4, 4, kTet4, 4, 1, 4)
     NUTO_CASE(kLinearElastic, 3, kTet4, 4, kTet10, 10, 4, 5)
+    NUTO_CASE(kLinearElastic, 3, kTet10, 10, kTet10, 10, 4, 5)
     NUTO_CASE(kLinearElastic, 3, kHex8, 8, kHex20, 20, 27, 5)
     NUTO_CASE(kGradientDamage, 3, kHex8, 8, kHex8, 8, 8, 4)
     NUTO_CASE(kLocalDamage, 3, kHex8, 8, kHex8, 8, 8, 8)
     NUTO_CASE(kLocalDamage, 3, kTet4, 4, kTet4, 4, 1, 4)
     NUTO_CASE(kLocalDamage, 3, kTet4, 4, kTet10, 10, 4, 5)
+    NUTO_CASE(kLocalDamage, 3, kTet10, 10, kTet10, 10, 4, 5)
     NUTO_CASE(kLocalDamage, 3, kHex8, 8, kHex20, 20, 27, 5)
 #undef NUTO_CASE
     throw Failure{NUTO_ERR_UNSUPPORTED, "no kernel for this integrand / element / rule combination"};
@@ -629,8 +631,8 @@ int nuto_set_cells(nuto_ctx* ctx, int coord_type, int n_cells, int n_coord_nodes
                    const int* coord_conn, const int* cell_ids)
 {
     return Guard(ctx, [&] {
-        Require(coord_type == kQuad4 || coord_type == kHex8 || coord_type == kTet4, NUTO_ERR_ARGUMENT,
-                "coordinate elements must be quad4, hex8 or tet4");
+        Require(coord_type == kQuad4 || coord_type == kHex8 || coord_type == kTet4 || coord_type == kTet10,
+                NUTO_ERR_ARGUMENT, "coordinate elements must be quad4, hex8, tet4 or tet10");
         Require(n_cells >= 0 && n_coord_nodes >= 0, NUTO_ERR_ARGUMENT, "negative sizes");
         Require(coords && coord_conn, NUTO_ERR_ARGUMENT, "coords / coord_conn is NULL");
         const int nn = NodesOf(coord_type), dim = DimOf(coord_type);

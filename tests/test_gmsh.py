@@ -292,6 +292,26 @@ def test_cells_for_assembly():
     assert C.shape == (t.n_elements, 10) and X.shape == (2768, 3)
 
 
+def test_isoparametric_tet10_numbering():
+    """Ten-node coordinate elements with the ten-node dof interpolation: every dof node sits on a
+    coordinate node (Kronecker property), so AddDofInterpolation's coordinate matching
+    (MeshFemDofConvert.cpp:164-210) numbers the file's nodes by first touch in cell order."""
+    import oracle as O
+
+    X, C = N.GmshMesh(path("MeshCompanionGmsh.msh")).cells(N.TET10)
+    ranks, nn = N.dof_layer(N.TET10, C, N.TET10)
+    assert nn == 2768  # every node of the file is used
+    om = O.Mesh.from_arrays(N.TET10, X, C)
+    lay = om.dof_layer(om.add_dof(N.TET10, 3))
+    assert np.array_equal(lay["numbers"][lay["conn"], 0] // 3, ranks)
+    # first touch: the ranks in cell order are 0, 1, 2, ... at their first appearance
+    flat = ranks.ravel()
+    _, first = np.unique(flat, return_index=True)
+    assert np.array_equal(flat[np.sort(first)], np.arange(nn))
+    with pytest.raises(N.NutoError):  # hex20 dofs over tet10 coordinates
+        N.dof_layer(N.TET10, C, N.HEX20)
+
+
 # ---- Morton codes ---------------------------------------------------------------------------------
 def spread(v, step, bits):
     return sum(((v >> b) & 1) << (step * b) for b in range(bits))

@@ -2,11 +2,10 @@
 
 MeshCompanionGmsh.msh (applications/integrationtests/mechanics/meshes, binary MSH 2.2: 1637
 ten-node tetrahedra of a 24 mm cube with inclusions, 2768 nodes, two physical groups) is read by
-nuto_gmsh_read; its corner nodes give tet4 coordinate elements and the ten-node interpolation is
-the dof interpolation (the file's edges are curved by up to 0.41 mm, so the geometry is the
-straight-sided one; the tet10 coordinate interpolation is not one of this path's coordinate
-elements). Bar as in the other parity tests: pattern, numbering bit-equal, values within
-1e-12 * sum|contributions|.
+nuto_gmsh_read. It runs twice: with its corner nodes as tet4 coordinate elements and the ten-node
+dof interpolation (the straight-sided geometry), and isoparametrically, with the ten-node
+coordinate elements (its edges are curved by up to 0.41 mm, so J varies over each cell). Bar as in
+the other parity tests: pattern, numbering bit-equal, values within 1e-12 * sum|contributions|.
 """
 import os
 
@@ -33,6 +32,23 @@ def test_linear_elastic_tet10_dofs_on_gmsh_mesh():
     p = Pair(N.TET4, N.TET10, X, C, *le_params(3), False)
     u, _, _ = p.inputs(u_range=1e-2)
     run_pair(p, u, None, None)
+
+
+@pytest.mark.parametrize("law", ["linear_elastic", "local_damage"])
+def test_isoparametric_tet10_on_gmsh_mesh(law):
+    m = N.GmshMesh(MESH)
+    X, C = m.cells(N.TET10)
+    assert C.shape == (1637, 10)
+    params = le_params(3) if law == "linear_elastic" else ld_params()
+    p = Pair(N.TET10, N.TET10, X, C, *params, False)
+    u, _, kap = p.inputs(u_range=5e-3, k_range=3e-4)
+    # J = sum_k X_k dN_k over ten nodes: terms of ~|X| |dN| (coordinates up to 24 mm) cancel to
+    # entries of the element size (~1 mm), and device (FMA) and oracle round that sum differently,
+    # by ~1e-16 of the terms. Where a gradient component vanishes in exact arithmetic (faces in
+    # coordinate planes, straight edges) that noise reaches K entries that are themselves near zero
+    # (up to ~1e-7 max|K|), which the magnitude bound, built from the computed J, cannot see. Their
+    # differences stay below 6e-17 max|K| (measured); the absolute floor 1e-15 max|K| is ~20x that.
+    run_pair(p, u, None, kap, k_floor=1e-15)
 
 
 def test_local_damage_on_gmsh_groups():
