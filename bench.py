@@ -338,7 +338,7 @@ def main():
     ap.add_argument("--solve-iters", type=int, default=20,
                     help="BiCGSTAB iterations of the constrained solve on the assembled system (0: skip)")
     ap.add_argument("--batch-meshes", type=int, default=256, help="meshes of the mixed-type batch (0: skip)")
-    ap.add_argument("--batch-gdm-meshes", type=int, default=64,
+    ap.add_argument("--batch-gdm-meshes", type=int, default=256,
                     help="meshes of the gradient-damage residual + adjoint batch (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration) and exit")

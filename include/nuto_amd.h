@@ -283,6 +283,12 @@ int nuto_batch_set_integrand(nuto_batch* batch, const nuto_params* params); /* d
 int nuto_batch_add_mesh(nuto_batch* batch, int coord_type, int dof_type, int n_cells, int n_coord_nodes,
                         const double* coords, const int* coord_conn, int n_dof_nodes, const int* cell_ranks,
                         int* mesh);
+/* The meshes of one element pair, in the order added, fill groups (super-meshes with one CSR, one
+ * launch each) of at most max_dof_nodes dof nodes; a larger mesh forms a group of its own. 0 (the
+ * default): one group per element pair, except gradient damage, whose groups hold at most
+ * (2^31 - 1) / (16 * 32) nodes so that each group's CSR stays within int32 offsets (the node kernel
+ * takes at most 32 neighbours per node). Takes effect at the next nuto_batch_build. */
+int nuto_batch_set_group_limit(nuto_batch* batch, long long max_dof_nodes);
 int nuto_batch_build(nuto_batch* batch); /* patterns of all meshes, once */
 /* sizes[6]: lengths of u, eeq, R, K, history and d_theta */
 int nuto_batch_sizes(nuto_batch* batch, long long* sizes);
@@ -291,9 +297,9 @@ int nuto_batch_copy_pattern(nuto_batch* batch, int mesh, int* row_ptr, int* col)
 /* per-cell (E, f_t, G_f) over all cells, meshes in the order they were added (host arrays) */
 int nuto_batch_set_material_fields(nuto_batch* batch, const double* E, const double* ft, const double* gf);
 /* R and K of every mesh. mode -1: the tuned launch (mode 3 until tuned) replayed from a HIP graph; 0..2: one fused launch
- * over all meshes with 64 / 128 / 256 nodes per workgroup; 3: one launch per element type over all
- * meshes of that type; 4: one launch per mesh (the row-owner kernel); 5: two-phase (cell matrices,
- * then the row gather) per element type; 6: two-phase per mesh (what separate
+ * over all meshes with 64 / 128 / 256 nodes per workgroup; 3: one launch per group (the meshes of
+ * one element type); 4: one launch per mesh (the row-owner kernel); 5: two-phase (cell matrices,
+ * then the row gather) per group; 6: two-phase per mesh (what separate
  * nuto_build_vector_matrix calls do). Gradient damage maps 0..2 and 5 to 3, and 6 to 4. */
 int nuto_batch_assemble(nuto_batch* batch, const double* u, const double* e, const double* kappa, double* R,
                         double* K, int mode);

@@ -128,6 +128,7 @@ def lib():
         L.nuto_batch_set_integrand.argtypes = [vp, ctypes.POINTER(Params)]
         L.nuto_batch_add_mesh.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp,
                                           ctypes.c_int, vp, ip]
+        L.nuto_batch_set_group_limit.argtypes = [vp, ctypes.c_longlong]
         L.nuto_batch_build.argtypes = [vp]
         L.nuto_batch_sizes.argtypes = [vp, vp]
         L.nuto_batch_mesh_info.argtypes = [vp, ctypes.c_int, vp]
@@ -656,6 +657,10 @@ class Batch:
                                             ctypes.byref(m)))
         self.n_meshes = m.value + 1
         return m.value
+
+    def set_group_limit(self, max_dof_nodes):
+        """Dof nodes per group (super-mesh) of one element pair (nuto_batch_set_group_limit; 0: default)."""
+        self._chk(lib().nuto_batch_set_group_limit(self.h, int(max_dof_nodes)))
 
     def build(self):
         self._chk(lib().nuto_batch_build(self.h))

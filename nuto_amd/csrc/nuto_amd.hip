@@ -1,6 +1,7 @@
 // C ABI of the MI355X assembly path (declared in include/nuto_amd.h).
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -1384,6 +1385,11 @@ bool BatchTypeOf(int ct, int dt, int& type, int& ruleKind, int& ruleN)
     return true;
 }
 
+// Default dof nodes per group of a gradient-damage batch: the node kernel takes at most 32
+// neighbours per node, each adding 16 nonzeros to the node's four rows, so a group of this many
+// nodes keeps its CSR within int32 offsets (2^31 - 1 nonzeros).
+constexpr long long kGdmGroupNodes = 2147483647LL / (16 * 32);
+
 // 0..2 fused launch, 64/128/256 nodes per tile; 3 one launch per type; 4 one launch per mesh;
 // 5 two-phase (cell matrices + row gather) per type; 6 two-phase per mesh
 constexpr int kBatchModes = 7;
@@ -1410,6 +1416,7 @@ struct nuto_batch
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int graphKey = -1;  // mode the graph was captured for (+100 for forward + adjoint)
+    long long groupNodes = 0;  // dof nodes per group (nuto_batch_set_group_limit; 0: the default)
     const void* graphPtrs[10] = {};
     DevBuf lambda;      // forward + adjoint: lambda = R - r_obs
 };
@@ -1789,14 +1796,27 @@ int nuto_batch_add_mesh(nuto_batch* b, int coord_type, int dof_type, int n_cells
     });
 }
 
+int nuto_batch_set_group_limit(nuto_batch* b, long long max_dof_nodes)
+{
+    return GuardB(b, [&] {
+        Require(max_dof_nodes >= 0, NUTO_ERR_ARGUMENT, "negative group limit");
+        ReleaseGroups(b);
+        b->groupNodes = max_dof_nodes;
+    });
+}
+
 int nuto_batch_build(nuto_batch* b)
 {
     return GuardB(b, [&] {
         Require(b->integrandSet, NUTO_ERR_STATE, "no integrand set");
         ReleaseGroups(b);
         const bool gdm = IsGdm(b);
-        // groups in the order hex20, tet10, hex8, tet4 (heaviest rows first)
+        // groups in the order hex20, tet10, hex8, tet4 (heaviest rows first); the meshes of one element
+        // pair, in the order they were added, fill groups of at most `cap` dof nodes (a larger mesh
+        // forms a group of its own)
+        const long long cap = b->groupNodes > 0 ? b->groupNodes : (gdm ? kGdmGroupNodes : LLONG_MAX);
         const int order[4][2] = {{kHex8, kHex20}, {kTet4, kTet10}, {kHex8, kHex8}, {kTet4, kTet4}};
+        std::vector<BatchGroup> plan;
         for (const auto& ctdt : order)
         {
             BatchGroup g;
@@ -1804,13 +1824,29 @@ int nuto_batch_build(nuto_batch* b)
             g.dt = ctdt[1];
             int rk, rn;
             BatchTypeOf(g.ct, g.dt, g.type, rk, rn);
+            long long nodes = 0;
             for (size_t m = 0; m < b->meshes.size(); m++)
-                if (b->meshes[m].ct == g.ct && b->meshes[m].dt == g.dt)
-                    g.meshes.push_back((int)m);
-            if (g.meshes.empty())
-                continue;
-            Require(!gdm || (g.ct == kHex8 && g.dt == kHex8), NUTO_ERR_UNSUPPORTED,
-                    "gradient-damage batches hold hex8 meshes only");
+            {
+                if (b->meshes[m].ct != g.ct || b->meshes[m].dt != g.dt)
+                    continue;
+                Require(!gdm || (g.ct == kHex8 && g.dt == kHex8), NUTO_ERR_UNSUPPORTED,
+                        "gradient-damage batches hold hex8 meshes only");
+                if (!g.meshes.empty() && nodes + b->meshes[m].nDof > cap)
+                {
+                    plan.push_back(g);
+                    g.meshes.clear();
+                    nodes = 0;
+                }
+                g.meshes.push_back((int)m);
+                nodes += b->meshes[m].nDof;
+            }
+            if (!g.meshes.empty())
+                plan.push_back(g);
+        }
+        for (BatchGroup& g : plan)
+        {
+            int rk, rn;
+            BatchTypeOf(g.ct, g.dt, g.type, rk, rn);
             const int gi = (int)b->groups.size();
             std::vector<double> X;
             std::vector<int> C, Rk;

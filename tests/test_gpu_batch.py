@@ -113,14 +113,54 @@ def test_autotune_picks_a_batched_mode(mixed):
         assert torch.equal(R, ref[0]) and torch.equal(K, ref[1])
 
 
-def test_gdm_batch_residual_adjoint_matches_single_meshes():
+def test_group_limit_splits_types_same_results(mixed):
+    """With a small group limit every element type spreads over several groups (super-meshes): each
+    mesh's R and K slices are bit-identical to the one-group-per-type batch, per group (mode 3) and
+    two-phase (mode 5) alike, and graph replay (mode -1) matches."""
+    meshes, b, sizes, u, kap = mixed
+    b2 = N.Batch()
+    b2.set_integrand(N.local_damage_params())
+    b2.set_group_limit(600)  # tet10: 245 + 343 nodes, then 245; hex20: 515, then 624
+    for ct, dt, X, C, R, nn in meshes:
+        b2.add_mesh(ct, dt, X, C, R, nn)
+    sizes2 = b2.build()
+    assert sizes2["R"] == sizes["R"] and sizes2["K"] == sizes["K"]
+    groups = {b2.mesh_info(m)["group"] for m in range(len(meshes))}
+    assert len(groups) > len(W.TYPES)
+    u2, kap2 = np.empty_like(u), np.empty_like(kap)
+    for m, (ct, dt, X, C, ranks, nn) in enumerate(meshes):  # the same per-mesh inputs in b2's layout
+        i, j = b.mesh_info(m), b2.mesh_info(m)
+        u2[j["u"]:j["u"] + 3 * nn] = u[i["u"]:i["u"] + 3 * nn]
+        h = mesh_history(kap, i)
+        c = np.arange(j["cells"])
+        for ip in range(j["nip"]):
+            kap2[j["history"] + c + ip * j["history_ld"]] = h[ip * j["cells"] + c]
+    for mode in (3, 5, -1):
+        R, K = (t.cpu().numpy() for t in b.assemble(to_dev(u), None, to_dev(kap), mode=mode if mode >= 0 else 3))
+        R2, K2 = (t.cpu().numpy() for t in b2.assemble(to_dev(u2), None, to_dev(kap2), mode=mode))
+        for m in range(len(meshes)):
+            i, j = b.mesh_info(m), b2.mesh_info(m)
+            assert np.array_equal(R[i["R_u"]:i["R_u"] + i["rows"]], R2[j["R_u"]:j["R_u"] + j["rows"]]), (mode, m)
+            assert np.array_equal(K[i["K_u"]:i["K_u"] + i["nnz_u"]], K2[j["K_u"]:j["K_u"] + j["nnz_u"]]), (mode, m)
+
+
+@pytest.mark.parametrize("limit,groups", [(0, 1), (64, 2), (30, 3)])
+def test_gdm_batch_residual_adjoint_matches_single_meshes(limit, groups):
+    """Also with the meshes split over several groups (27, 36 and 60 dof nodes: a limit of 64 nodes
+    packs the first two together, 30 gives each its own group)."""
     dims = [(2, 2, 2), (3, 2, 2), (2, 3, 4)]
     meshes = []
     for n in dims:
         X, C = N.structured_mesh(N.HEX8, *n, tuple(float(v) for v in n))
         ranks, nn = N.dof_layer(N.HEX8, C, N.HEX8)
         meshes.append((N.HEX8, N.HEX8, X, C, ranks, nn))
-    b, sizes = build(meshes, N.gradient_damage_params())
+    b = N.Batch()
+    b.set_integrand(N.gradient_damage_params())
+    b.set_group_limit(limit)
+    for ct, dt, X, C, R, nn in meshes:
+        b.add_mesh(ct, dt, X, C, R, nn)
+    sizes = b.build()
+    assert len({b.mesh_info(m)["group"] for m in range(len(meshes))}) == groups
     n_cells = sum(m[3].shape[0] for m in meshes)
     E, ft, gf = S.material_fields(n_cells)
     b.set_material_fields(E, ft, gf)
