@@ -252,6 +252,25 @@ std::vector<double> SimpleAssembler::Eval(const CellGroup& cells, const Gradient
     return RunEval({f.disp, f.eeq}, state, &f.kappas, cells.NumCells(), 3, field);
 }
 
+void SimpleAssembler::UpdateHistory(const CellGroup& cells, GradientDamage& f, const DofVector<double>& state)
+{
+    const nuto_params p{NUTO_GRADIENT_DAMAGE, 3, f.E, f.nu, 0, f.k, f.kappa0, f.beta, f.alpha, f.c,
+                        f.damageApplication};
+    Setup(cells, {f.disp, f.eeq}, p, 3);
+    const int nu = mDofInfo.Total(f.disp), ne = mDofInfo.Total(f.eeq);
+    const size_t nk = (size_t)cells.NumCells() * (mTet ? 1 : 8);
+    if (f.kappas.size() != nk)
+        throw Exception("SimpleAssembler::UpdateHistory", "the history holds " + std::to_string(f.kappas.size()) +
+                                                              " values, the cell group needs " + std::to_string(nk));
+    DeviceArray u(mCtx, nu * sizeof(double)), e(mCtx, ne * sizeof(double)), k(mCtx, nk * sizeof(double));
+    Check(nuto_copy_to_device(mCtx, u.p, state[f.disp].data(), nu * sizeof(double)), "UpdateHistory");
+    Check(nuto_copy_to_device(mCtx, e.p, state[f.eeq].data(), ne * sizeof(double)), "UpdateHistory");
+    Check(nuto_copy_to_device(mCtx, k.p, f.kappas.data(), nk * sizeof(double)), "UpdateHistory");
+    Check(nuto_update_history(mCtx, u.d(), e.d(), k.d()), "SimpleAssembler::UpdateHistory");
+    Check(nuto_synchronize(mCtx), "SimpleAssembler::UpdateHistory");
+    Check(nuto_copy_to_host(mCtx, f.kappas.data(), k.p, nk * sizeof(double)), "UpdateHistory");
+}
+
 std::vector<double> SimpleAssembler::Eval(const CellGroup& cells, const MomentumBalance& f,
                                           const DofVector<double>& state, IpField field)
 {
@@ -488,6 +507,127 @@ CellGroup MeshGmsh::Cells(int coordType, int physicalGroupId) const
     g.conn.resize((size_t)n * nodes[coordType]);
     nuto_gmsh_cells(mMesh, physicalGroupId, coordType, &n, g.conn.data());
     return g;
+}
+
+void AddTo(DofMatrixSparse<double>& a, const DofMatrixSparse<double>& b)
+{
+    const int ra = a.rowPtr.empty() ? 0 : a.Rows(), rb = b.rowPtr.empty() ? 0 : b.Rows();
+    const int rows = std::max(ra, rb);
+    DofMatrixSparse<double> s;
+    s.rowPtr.assign(1, 0);
+    s.col.reserve(a.col.size() + b.col.size());
+    s.val.reserve(a.col.size() + b.col.size());
+    for (int i = 0; i < rows; i++)
+    {
+        int p = i < ra ? a.rowPtr[i] : 0, pe = i < ra ? a.rowPtr[i + 1] : 0;
+        int q = i < rb ? b.rowPtr[i] : 0, qe = i < rb ? b.rowPtr[i + 1] : 0;
+        while (p < pe || q < qe)  // both rows have ascending columns
+        {
+            if (q == qe || (p < pe && a.col[p] < b.col[q]))
+            {
+                s.col.push_back(a.col[p]);
+                s.val.push_back(a.val[p++]);
+            }
+            else if (p == pe || b.col[q] < a.col[p])
+            {
+                s.col.push_back(b.col[q]);
+                s.val.push_back(b.val[q++]);
+            }
+            else
+            {
+                s.col.push_back(a.col[p]);
+                s.val.push_back(a.val[p++] + b.val[q++]);
+            }
+        }
+        s.rowPtr.push_back((int)s.col.size());
+    }
+    a = std::move(s);
+}
+
+void TimeDependentProblem::AddGradientFunction(const CellGroup& group, const GradientDamage& f)
+{
+    mGradient.push_back({&group, &f, nullptr, nullptr});
+}
+
+void TimeDependentProblem::AddGradientFunction(const CellGroup& group, const MomentumBalance& f)
+{
+    mGradient.push_back({&group, nullptr, &f, nullptr});
+}
+
+void TimeDependentProblem::AddHessian0Function(const CellGroup& group, const GradientDamage& f)
+{
+    mHessian0.push_back({&group, &f, nullptr, nullptr});
+}
+
+void TimeDependentProblem::AddHessian0Function(const CellGroup& group, const MomentumBalance& f)
+{
+    mHessian0.push_back({&group, nullptr, &f, nullptr});
+}
+
+void TimeDependentProblem::AddUpdateFunction(const CellGroup& group, GradientDamage& f)
+{
+    mUpdate.push_back({&group, &f, nullptr, &f});
+}
+
+void TimeDependentProblem::CheckDofs(const Term& term, const std::vector<DofType>& dofs)
+{
+    const bool ok = term.gdm ? dofs.size() == 2 && dofs[0].Id() == term.gdm->disp.Id() &&
+                                   dofs[1].Id() == term.gdm->eeq.Id()
+                             : !dofs.empty() && dofs[0].Id() == term.mb->disp.Id();
+    if (!ok)
+        throw Exception("TimeDependentProblem",
+                        "the integrand's dof types are not a leading part of the requested dof types");
+}
+
+DofVector<double> TimeDependentProblem::Gradient(const DofVector<double>& dofValues, std::vector<DofType> dofs,
+                                                 double /*t*/, double /*dt*/)
+{
+    DofVector<double> gradient;
+    for (const DofType& d : dofs)
+        gradient[d].assign(mDofInfo.Total(d), 0.0);
+    for (const Term& term : mGradient)
+    {
+        CheckDofs(term, dofs);
+        const DofVector<double> r = term.gdm ? mAssembler.BuildVector(*term.group, *term.gdm, dofValues)
+                                             : mAssembler.BuildVector(*term.group, *term.mb, dofValues);
+        for (const DofType& d : dofs)
+        {
+            if (!r.Has(d))
+                continue;
+            const std::vector<double>& v = r[d];
+            std::vector<double>& g = gradient[d];
+            for (size_t i = 0; i < v.size(); i++)
+                g[i] += v[i];
+        }
+    }
+    return gradient;
+}
+
+DofMatrixSparse<double> TimeDependentProblem::Hessian0(const DofVector<double>& dofValues,
+                                                       std::vector<DofType> dofs, double /*t*/, double /*dt*/)
+{
+    int rows = 0;
+    for (const DofType& d : dofs)
+        rows += mDofInfo.Total(d);
+    DofMatrixSparse<double> hessian0;
+    hessian0.rowPtr.assign(rows + 1, 0);
+    for (const Term& term : mHessian0)
+    {
+        CheckDofs(term, dofs);
+        AddTo(hessian0, term.gdm ? mAssembler.BuildMatrix(*term.group, *term.gdm, dofValues)
+                                 : mAssembler.BuildMatrix(*term.group, *term.mb, dofValues));
+    }
+    return hessian0;
+}
+
+void TimeDependentProblem::UpdateHistory(const DofVector<double>& dofValues, std::vector<DofType> dofs,
+                                         double /*t*/, double /*dt*/)
+{
+    for (const Term& term : mUpdate)
+    {
+        CheckDofs(term, dofs);
+        mAssembler.UpdateHistory(*term.group, *term.update, dofValues);
+    }
 }
 
 }  // namespace host

@@ -10,6 +10,8 @@
 //   SimpleAssembler    BuildVector / BuildMatrix (SimpleAssembler.h:14-21, SimpleAssembler.cpp:38-156)
 //   Constraints        Constraint::Constraints / Equation (Constraints.h:47-130, Equation.h:12-66): per DofType,
 //                      1 x_dep + sum c x = rhs(t)
+//   TimeDependentProblem  Gradient / Hessian0 / UpdateHistory over (group, integrand) pairs
+//                      (TimeDependentProblem.cpp:34-85)
 //   QuasistaticSolver  QuasistaticSolver::DoStep (QuasistaticSolver.h:14-90) on the device
 //   MeshGmsh           MeshGmsh (MeshGmsh.h:16-60)
 // NuTo's VectorFunction / MatrixFunction are opaque std::functions; here the integrand is a typed
@@ -268,6 +270,10 @@ public:
     // DynamicMomentumBalance::Hessian2 (DynamicMomentumBalance.h:22-30): the displacement dofs only
     DofVector<double> BuildDiagonallyLumpedMatrix(const CellGroup& cells, const DynamicMomentumBalance& f);
 
+    // Cell::Apply(GradientDamage::Update) over the group (TimeDependentProblem.cpp:78-85,
+    // GradientDamage.h:122-125): f.kappas(cell, ip) = max(kappa, eeq(ip)), in place
+    void UpdateHistory(const CellGroup& cells, GradientDamage& f, const DofVector<double>& state);
+
     // Cell::Eval (Cell.h:80-93) of one field for every cell of the group: values [cell][ip][comp]
     std::vector<double> Eval(const CellGroup& cells, const GradientDamage& f, const DofVector<double>& state,
                              IpField field);
@@ -287,6 +293,61 @@ private:
     long long mNnz = 0;
     int mRows = 0;
     bool mTet = false;  // the rule of the last Setup: tetrahedron order 1 (else 2x2(x2) Gauss)
+};
+
+// DofMatrixSparse += (DofMatrixSparse.h:9-10 over Eigen's sparse sum): the union of both patterns,
+// entries present in both added, explicit zeros kept; a matrix with fewer rows (a leading part of
+// the dof types, e.g. {displacements} of {displacements, eeq}) adds to the leading rows.
+void AddTo(DofMatrixSparse<double>& a, const DofMatrixSparse<double>& b);
+
+// TimeDependentProblem (TimeDependentProblem.h:20-90, TimeDependentProblem.cpp:34-85): (cell group,
+// integrand) pairs. Gradient and Hessian0 assemble every pair with SimpleAssembler and sum the
+// results in the order the pairs were added (DofVector +=, DofMatrixSparse += as AddTo);
+// UpdateHistory updates every pair's history in place. Groups and integrands are held by
+// reference: the caller keeps them alive, as NuTo's Bind captures the integrand by reference
+// (Bind.h:14,21). The node values are the dofValues argument (NuTo merges them into its nodes
+// first, NodalValueMerger::Merge). dofs: the result's dof types in order; each pair's integrand acts
+// on a leading part of them ({displacements} or {displacements, nonlocal eq. strain}), otherwise
+// Exception. t and dt are bound into the integrands as TimeDependentProblem's Apply does; these
+// integrands do not depend on time.
+class TimeDependentProblem
+{
+public:
+    explicit TimeDependentProblem(DofInfo dofInfo = DofInfo(), int device = 0)
+        : mDofInfo(dofInfo)
+        , mAssembler(dofInfo, device)
+    {
+    }
+    void SetDofInfo(DofInfo dofInfo)
+    {
+        mDofInfo = dofInfo;
+        mAssembler.SetDofInfo(dofInfo);
+    }
+
+    void AddGradientFunction(const CellGroup& group, const GradientDamage& f);
+    void AddGradientFunction(const CellGroup& group, const MomentumBalance& f);
+    void AddHessian0Function(const CellGroup& group, const GradientDamage& f);
+    void AddHessian0Function(const CellGroup& group, const MomentumBalance& f);
+    void AddUpdateFunction(const CellGroup& group, GradientDamage& f);
+
+    DofVector<double> Gradient(const DofVector<double>& dofValues, std::vector<DofType> dofs, double t, double dt);
+    DofMatrixSparse<double> Hessian0(const DofVector<double>& dofValues, std::vector<DofType> dofs, double t,
+                                     double dt);
+    void UpdateHistory(const DofVector<double>& dofValues, std::vector<DofType> dofs, double t, double dt);
+
+private:
+    struct Term
+    {
+        const CellGroup* group;
+        const GradientDamage* gdm;   // one of gdm / mb
+        const MomentumBalance* mb;
+        GradientDamage* update;      // history owner of an update term
+    };
+    static void CheckDofs(const Term& term, const std::vector<DofType>& dofs);
+
+    DofInfo mDofInfo;
+    SimpleAssembler mAssembler;
+    std::vector<Term> mGradient, mHessian0, mUpdate;
 };
 
 }  // namespace host

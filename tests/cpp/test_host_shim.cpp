@@ -22,6 +22,7 @@ void orc_problem_R(void* Pp, double* R, double* Rb);
 void orc_problem_K(void* Pp, int* rp, int* ci, double* v, double* b);
 void orc_problem_lumped_mass(void* Pp, double rho, double* M, double* Mb);
 int orc_problem_eval_ip(void* Pp, const double* u, const double* e, const double* kappa, int field, double* out);
+void orc_problem_set_theta(void* Pp, const double* E, const double* ft, const double* gf);
 }
 
 using namespace nuto_amd::host;
@@ -326,6 +327,115 @@ static void QuasistaticCase()
     EXPECT(threw && solver.GlobalTime() == 1.0, "NoConvergence keeps the state");
 }
 
+// TimeDependentProblem with two cell groups of one mesh (cells left / right of x = 2) and two
+// gradient-damage materials: the summed R and K (union of the groups' patterns) against the oracle
+// on the whole mesh with per-cell material fields; UpdateHistory against Eval(Kappa).
+static void TimeDependentProblemCase()
+{
+    Mesh m = MakeMesh(NUTO_HEX8, 4, 3, 2);
+    DofType disp("Displacements", 3), eeq("NonlocalEqStrain", 1);
+    DofInfo info;
+    info.numIndependent[disp.Id()] = 3 * m.nDofNodes;
+    info.numIndependent[eeq.Id()] = m.nDofNodes;
+    CellGroup groups[2];
+    std::vector<int> cellsOf[2];
+    for (int c = 0; c < m.nCells; c++)
+    {
+        double xc = 0.0;
+        for (int k = 0; k < 8; k++)
+            xc += m.X[(size_t)m.C[c * 8 + k] * 3] / 8.0;
+        const int g = xc < 2.0 ? 0 : 1;
+        cellsOf[g].push_back(c);
+        groups[g].conn.insert(groups[g].conn.end(), m.C.begin() + c * 8, m.C.begin() + c * 8 + 8);
+    }
+    const double Eg[2] = {30000.0, 45000.0}, k0g[2] = {1e-4, 2e-4}, betag[2] = {100.0, 150.0};
+    std::vector<GradientDamage> f;
+    unsigned long long seed = 777;
+    for (int g = 0; g < 2; g++)
+    {
+        groups[g].coordType = NUTO_HEX8;
+        groups[g].coords = m.X;
+        std::vector<int> ranks;
+        for (int c : cellsOf[g])
+            ranks.insert(ranks.end(), m.ranks.begin() + c * 8, m.ranks.begin() + c * 8 + 8);
+        groups[g].dofNumbering[disp.Id()] = {NUTO_HEX8, CellDofs(ranks, 3)};
+        groups[g].dofNumbering[eeq.Id()] = {NUTO_HEX8, CellDofs(ranks, 1)};
+        f.push_back(GradientDamage{disp, eeq, 4.0, Eg[g], 0.2, 10.0, k0g[g], betag[g], 0.99, 0, {}});
+        for (size_t i = 0; i < cellsOf[g].size() * 8; i++)
+            f[g].kappas.push_back(Uniform(seed, 0.0, 3e-4));
+    }
+    EXPECT(!cellsOf[0].empty() && !cellsOf[1].empty(), "both groups hold cells");
+    DofVector<double> state;
+    for (int i = 0; i < 3 * m.nDofNodes; i++)
+        state[disp].push_back(Uniform(seed, -5e-3, 5e-3));
+    for (int i = 0; i < m.nDofNodes; i++)
+        state[eeq].push_back(Uniform(seed, 0.0, 5e-4));
+
+    TimeDependentProblem problem(info);
+    for (int g = 0; g < 2; g++)
+    {
+        problem.AddGradientFunction(groups[g], f[g]);
+        problem.AddHessian0Function(groups[g], f[g]);
+        problem.AddUpdateFunction(groups[g], f[g]);
+    }
+    DofVector<double> R = problem.Gradient(state, {disp, eeq}, 0.0, 0.0);
+    DofMatrixSparse<double> K = problem.Hessian0(state, {disp, eeq}, 0.0, 0.0);
+    std::vector<double> Rall = R[disp];
+    Rall.insert(Rall.end(), R[eeq].begin(), R[eeq].end());
+
+    // oracle: the whole mesh, per-cell E, f_t = kappa0 E, G_f = f_t / beta, history in global cell order
+    std::vector<double> E(m.nCells), ft(m.nCells), gf(m.nCells), kap((size_t)m.nCells * 8);
+    for (int g = 0; g < 2; g++)
+        for (size_t l = 0; l < cellsOf[g].size(); l++)
+        {
+            const int c = cellsOf[g][l];
+            E[c] = Eg[g];
+            ft[c] = k0g[g] * Eg[g];
+            gf[c] = ft[c] / betag[g];
+            for (int ip = 0; ip < 8; ip++)
+                kap[c + (size_t)ip * m.nCells] = f[g].kappas[l + ip * cellsOf[g].size()];
+        }
+    void* om = orc_mesh_from_arrays(NUTO_HEX8, m.nNodes, m.X.data(), m.nCells, m.C.data());
+    const int Ld = orc_mesh_add_dof(om, NUTO_HEX8, 3), Le = orc_mesh_add_dof(om, NUTO_HEX8, 1);
+    nuto_params p{NUTO_GRADIENT_DAMAGE, 3, 30000.0, 0.2, 0, 10.0, 1e-4, 100.0, 0.99, 4.0, 0};
+    void* P = orc_problem(om, &p, 0, 2, Ld, Le);
+    orc_problem_set_theta(P, E.data(), ft.data(), gf.data());
+    orc_problem_assemble(P, state[disp].data(), state[eeq].data(), kap.data(), 1, 1, 1);
+    Compare("time dependent problem, two groups", P, 4 * m.nDofNodes, Rall, K);
+    orc_problem_free(P);
+    orc_mesh_free(om);
+
+    // UpdateHistory: kappa <- max(kappa, eeq) = GradientDamage::Kappa before the update
+    std::vector<std::vector<double>> expect;
+    SimpleAssembler asmbl(info);
+    for (int g = 0; g < 2; g++)
+        expect.push_back(asmbl.Eval(groups[g], f[g], state, IpField::Kappa));
+    problem.UpdateHistory(state, {disp, eeq}, 0.0, 0.0);
+    for (int g = 0; g < 2; g++)
+    {
+        bool same = expect[g].size() == f[g].kappas.size();
+        const size_t n = cellsOf[g].size();
+        for (size_t c = 0; same && c < n; c++)
+            for (int ip = 0; ip < 8; ip++)  // Eval: [cell][ip]; history: cell + ip * n
+            {
+                const double a = expect[g][c * 8 + ip], b = f[g].kappas[c + ip * n];
+                same = same && std::fabs(a - b) <= 1e-14 * std::fabs(a);
+            }
+        EXPECT(same, "UpdateHistory equals Kappa before the update");
+    }
+
+    bool threw = false;
+    try
+    {
+        problem.Gradient(state, {disp}, 0.0, 0.0);  // a gradient-damage term needs {disp, eeq}
+    }
+    catch (const Exception&)
+    {
+        threw = true;
+    }
+    EXPECT(threw, "dof types that do not fit the integrand throw Exception");
+}
+
 int main()
 {
     try
@@ -334,6 +444,7 @@ int main()
         MomentumBalanceCase();
         LumpedMassAndEvalCase();
         QuasistaticCase();
+        TimeDependentProblemCase();
     }
     catch (const std::exception& e)
     {
