@@ -588,7 +588,7 @@ __device__ inline void PhaseBAffine(const double* state, int r, int b, const dou
         sinv[2][c] = sz ? inv[2][c] : -inv[2][c];
     }
     const double* cellRec = state + r * 8 * kSt;
-#pragma unroll 1
+#pragma unroll 2
     for (int t = 0; t < 8; t++)
     {
         const double d0 = kShapeRel.v[t][0], d1 = kShapeRel.v[t][1], d2 = kShapeRel.v[t][2], Nb = kShapeRel.v[t][3];

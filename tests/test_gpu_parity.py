@@ -64,6 +64,30 @@ def test_quad4_linear_elastic_config1_material():
     run_pair(p, u, None, None)
 
 
+def test_quad4_linear_elastic_config1_full_size():
+    """The plane-stress workload at its full size: 200 x 200 quad4 cells of the unit square, 2x2
+    Gauss, E = 30000, nu = 0.2, u uniform in [-1e-3, 1e-3] (80,802 dofs, 1,444,804 nonzeros)."""
+    X, C = make_mesh(N.QUAD4, (200, 200))
+    dev, orc = le_params(2)
+    p = Pair(N.QUAD4, N.QUAD4, X, C, dev, orc, False)
+    assert p.n_u == 80802
+    u = S.uniform(S.SEED_U, p.n_u, -1e-3, 1e-3)
+    ctx, R, K, mr, mk = run_pair(p, u, None, None)
+    assert K.numel() == 1444804
+
+
+def test_hex8_gradient_damage_headline_cells():
+    """16^3 cells of 1 mm (the cell size of the 100 mm cube workload) with its material and state
+    distributions: most nodes are interior, so the node kernel's full 8-cell neighbourhoods, the
+    Morton traversal and the affine mode are exercised against the oracle."""
+    n = 16
+    X, C = make_mesh(N.HEX8, (n, n, n), (float(n),) * 3)
+    dev, orc = gdm_params(N.FULL)
+    p = Pair(N.HEX8, N.HEX8, X, C, dev, orc, True)
+    u, e, kap = p.inputs()
+    run_pair(p, u, e, kap)
+
+
 def test_quad4_fixture_4x4():
     f = np.load(os.path.join(GOLD, "quad4_le_4x4.npz"))
     X, C = make_mesh(N.QUAD4, (4, 4))
