@@ -33,6 +33,26 @@ struct DeviceArray
     DeviceArray& operator=(const DeviceArray&) = delete;
     double* d() const { return static_cast<double*>(p); }
 };
+// The integration rule the group's cells use, as NuTo's tests construct it for each element pair:
+// tetrahedra order 1, order 2 when an interpolation is the ten-node one; tensor-product Gauss with
+// 2 points per direction, 3 when an interpolation is the twenty-node brick. Returns the IPs per cell.
+int DefaultRule(const CellGroup& cells, const std::vector<DofType>& dofs, int& kind, int& n)
+{
+    bool quadratic = cells.coordType == NUTO_TET10;
+    for (const DofType& d : dofs)
+    {
+        auto it = cells.dofNumbering.find(d.Id());
+        if (it != cells.dofNumbering.end() && (it->second.first == NUTO_TET10 || it->second.first == NUTO_HEX20))
+            quadratic = true;
+    }
+    const bool tet = cells.coordType == NUTO_TET4 || cells.coordType == NUTO_TET10;
+    kind = tet ? 1 : 0;
+    n = tet ? (quadratic ? 2 : 1) : (quadratic ? 3 : 2);
+    if (tet)
+        return quadratic ? 4 : 1;
+    const int dim = cells.coordType == NUTO_QUAD4 ? 2 : 3;
+    return dim == 2 ? n * n : n * n * n;
+}
 }  // namespace
 
 DofType::DofType(std::string name, int num) : mName(std::move(name)), mNum(num), mId(gNextDofTypeId++)
@@ -92,9 +112,9 @@ void SimpleAssembler::Setup(const CellGroup& cells, const std::vector<DofType>& 
               "SimpleAssembler::SetDofType");
     }
     Check(nuto_set_integrand(mCtx, &p), "SimpleAssembler::SetIntegrand");
-    const bool tet = cells.coordType == NUTO_TET4;
-    mTet = tet;
-    Check(nuto_set_rule(mCtx, tet ? 1 : 0, tet ? 1 : 2), "SimpleAssembler::SetRule");
+    int kind = 0, n = 0;
+    mNip = DefaultRule(cells, dofs, kind, n);
+    Check(nuto_set_rule(mCtx, kind, n), "SimpleAssembler::SetRule");
     Check(nuto_build_pattern(mCtx, &mNnz, &mRows), "SimpleAssembler::BuildPattern");
 }
 
@@ -230,8 +250,7 @@ std::vector<double> SimpleAssembler::RunEval(const std::vector<DofType>& dofs, c
         k.reset(new DeviceArray(mCtx, kappas->size() * sizeof(double)));
         Check(nuto_copy_to_device(mCtx, k->p, kappas->data(), kappas->size() * sizeof(double)), "Eval");
     }
-    // IPs of the rule Setup chose: tetrahedron order 1 (1 point) or 2x2(x2) Gauss
-    const int nip = mTet ? 1 : (dim == 2 ? 4 : 8);
+    const int nip = mNip;  // IPs of the rule Setup chose
     const int nc = (field == IpField::Strain || field == IpField::Stress) ? (dim == 2 ? 3 : 6) : 1;
     const size_t count = (size_t)nCells * nip * nc;
     DeviceArray o(mCtx, count * sizeof(double));
@@ -258,7 +277,7 @@ void SimpleAssembler::UpdateHistory(const CellGroup& cells, GradientDamage& f, c
                         f.damageApplication};
     Setup(cells, {f.disp, f.eeq}, p, 3);
     const int nu = mDofInfo.Total(f.disp), ne = mDofInfo.Total(f.eeq);
-    const size_t nk = (size_t)cells.NumCells() * (mTet ? 1 : 8);
+    const size_t nk = (size_t)cells.NumCells() * mNip;
     if (f.kappas.size() != nk)
         throw Exception("SimpleAssembler::UpdateHistory", "the history holds " + std::to_string(f.kappas.size()) +
                                                               " values, the cell group needs " + std::to_string(nk));
@@ -319,8 +338,9 @@ void QuasistaticSolver::Init(const CellGroup& cells, const std::vector<DofType>&
               "QuasistaticSolver::SetDofType");
     }
     Check(nuto_set_integrand(mCtx, &p), "QuasistaticSolver::SetIntegrand");
-    const bool tet = cells.coordType == NUTO_TET4;
-    Check(nuto_set_rule(mCtx, tet ? 1 : 0, tet ? 1 : 2), "QuasistaticSolver::SetRule");
+    int kind = 0, n = 0;
+    mNip = DefaultRule(cells, dofs, kind, n);
+    Check(nuto_set_rule(mCtx, kind, n), "QuasistaticSolver::SetRule");
     Check(nuto_build_pattern(mCtx, &mNnz, &mRows), "QuasistaticSolver::BuildPattern");
     Check(nuto_device_alloc(mCtx, &mX, (size_t)mRows * sizeof(double)), "QuasistaticSolver");
     std::vector<double> zero(mRows, 0.0);
@@ -334,7 +354,7 @@ QuasistaticSolver::QuasistaticSolver(const CellGroup& cells, GradientDamage& f, 
     const nuto_params p{NUTO_GRADIENT_DAMAGE, 3, f.E, f.nu, 0, f.k, f.kappa0, f.beta, f.alpha, f.c,
                         f.damageApplication};
     Init(cells, {f.disp, f.eeq}, p, 3, device);
-    const size_t nk = (size_t)cells.NumCells() * (cells.coordType == NUTO_TET4 ? 1 : 8);
+    const size_t nk = (size_t)cells.NumCells() * mNip;
     if (f.kappas.size() != nk)
         f.kappas.assign(nk, 0.0);
     mKappaBytes = nk * sizeof(double);

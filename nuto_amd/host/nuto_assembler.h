@@ -246,6 +246,7 @@ private:
     void* mX = nullptr;      // device [u; e]
     void* mKappa = nullptr;  // device history
     size_t mKappaBytes = 0;
+    int mNip = 0;            // IPs per cell of the group's rule
 };
 
 class SimpleAssembler
@@ -292,7 +293,7 @@ private:
     nuto_ctx* mCtx = nullptr;
     long long mNnz = 0;
     int mRows = 0;
-    bool mTet = false;  // the rule of the last Setup: tetrahedron order 1 (else 2x2(x2) Gauss)
+    int mNip = 0;  // IPs per cell of the rule of the last Setup (tetrahedron or tensor-product Gauss)
 };
 
 // DofMatrixSparse += (DofMatrixSparse.h:9-10 over Eigen's sparse sum): the union of both patterns,

@@ -1,6 +1,7 @@
 // C++ test of nuto_amd::host::SimpleAssembler (the SimpleAssembler mirror over the C ABI) against
 // the oracle's C API: pattern bit-equal, R and K within 1e-12 of the contribution magnitudes, and
 // the reference's error behaviour (missing dof numbering, missing DofVector entry). Needs a GPU.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -187,6 +188,57 @@ static void MomentumBalanceCase()
     void* P = orc_problem(om, &p, 0, 2, Ld, -1);
     orc_problem_assemble(P, state[disp].data(), nullptr, nullptr, 1, 1, 1);
     Compare("momentum balance", P, 2 * m.nDofNodes, R[disp], K);
+    orc_problem_free(P);
+    orc_mesh_free(om);
+}
+
+// Ten-node displacements on tetrahedra (bricks split into six tetrahedra): the mirror picks the
+// order-2 tetrahedron rule for the quadratic interpolation; R, K and Eval (4 IPs per cell) against
+// the oracle.
+static void QuadraticTetrahedraCase()
+{
+    int nNodes = 0, nCells = 0;
+    nuto_mesh_structured(NUTO_TET4, 2, 2, 1, 2.0, 2.0, 1.0, 0.0, 0, &nNodes, &nCells, nullptr, nullptr);
+    std::vector<double> X((size_t)nNodes * 3);
+    std::vector<int> C((size_t)nCells * 4);
+    nuto_mesh_structured(NUTO_TET4, 2, 2, 1, 2.0, 2.0, 1.0, 0.0, 0, &nNodes, &nCells, X.data(), C.data());
+    std::vector<int> ranks((size_t)nCells * 10);
+    int nDofNodes = 0;
+    nuto_dof_layer(NUTO_TET4, nCells, C.data(), NUTO_TET10, ranks.data(), &nDofNodes);
+    DofType disp("Displacements", 3);
+    DofInfo info;
+    info.numIndependent[disp.Id()] = 3 * nDofNodes;
+    CellGroup cells;
+    cells.coordType = NUTO_TET4;
+    cells.coords = X;
+    cells.conn = C;
+    cells.dofNumbering[disp.Id()] = {NUTO_TET10, CellDofs(ranks, 3)};
+    MomentumBalance f{disp, 30000.0, 0.2, 0};
+    unsigned long long seed = 4242;
+    DofVector<double> state;
+    for (int i = 0; i < 3 * nDofNodes; i++)
+        state[disp].push_back(Uniform(seed, -1e-3, 1e-3));
+    SimpleAssembler asmbl(info);
+    DofVector<double> R = asmbl.BuildVector(cells, f, state);
+    DofMatrixSparse<double> K = asmbl.BuildMatrix(cells, f, state);
+    std::vector<double> stress = asmbl.Eval(cells, f, state, IpField::Stress);
+    EXPECT(stress.size() == (size_t)nCells * 4 * 6, "Eval: 4 IPs of the order-2 rule per cell");
+
+    void* om = orc_mesh_from_arrays(NUTO_TET4, nNodes, X.data(), nCells, C.data());
+    const int Ld = orc_mesh_add_dof(om, NUTO_TET10, 3);
+    nuto_params p{NUTO_MOMENTUM_LINEAR_ELASTIC, 3, 30000.0, 0.2, 0, 0, 0, 0, 0, 0, 0};
+    void* P = orc_problem(om, &p, 1, 2, Ld, -1);
+    orc_problem_assemble(P, state[disp].data(), nullptr, nullptr, 1, 1, 1);
+    Compare("ten-node tetrahedra", P, 3 * nDofNodes, R[disp], K);
+    std::vector<double> ref(stress.size());
+    orc_problem_eval_ip(P, state[disp].data(), nullptr, nullptr, NUTO_IP_STRESS, ref.data());
+    double worst = 0.0, scale = 0.0;
+    for (size_t i = 0; i < ref.size(); i++)
+    {
+        worst = std::max(worst, std::fabs(stress[i] - ref[i]));
+        scale = std::max(scale, std::fabs(ref[i]));
+    }
+    EXPECT(worst <= 1e-12 * scale, "Eval stress of ten-node tetrahedra");
     orc_problem_free(P);
     orc_mesh_free(om);
 }
@@ -445,6 +497,7 @@ int main()
         LumpedMassAndEvalCase();
         QuasistaticCase();
         TimeDependentProblemCase();
+        QuadraticTetrahedraCase();
     }
     catch (const std::exception& e)
     {
