@@ -178,6 +178,36 @@ def adjoint_ratio(n, steps, warmup):
             "adjoint_outputs": "K^T lambda (u, eeq) and dR/dtheta^T lambda (3 per cell)"}
 
 
+def pcie_inclusive(ctx, u, e, kap, R, K, reps=3):
+    """The same step for a caller that holds u, eeq and kappa in host memory and wants R and K
+    back there: pinned host buffers, copies in, the assembly, copies out (never the headline value;
+    the boundary takes device pointers and K is meant to stay resident for a device solver)."""
+    import numpy as np
+    import torch
+
+    hu, he, hk = (t.cpu().pin_memory() for t in (u, e, kap))
+    hR = torch.empty(R.numel(), dtype=torch.float64, pin_memory=True)
+    hK = torch.empty(K.numel(), dtype=torch.float64, pin_memory=True)
+    times = []
+    for _ in range(reps + 1):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        u.copy_(hu, non_blocking=True)
+        e.copy_(he, non_blocking=True)
+        kap.copy_(hk, non_blocking=True)
+        ctx.build_vector_matrix(u, e, kap, R, K)
+        hR.copy_(R, non_blocking=True)
+        hK.copy_(K, non_blocking=True)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+    ms = 1000.0 * float(np.median(times[1:]))
+    bytes_h2d = 8 * (u.numel() + e.numel() + kap.numel())
+    bytes_d2h = 8 * (R.numel() + K.numel())
+    return {"ms": ms, "cells_per_s": ctx.n_cells / (ms * 1e-3), "bytes_to_device": bytes_h2d,
+            "bytes_to_host": bytes_d2h, "host_link_GBps": (bytes_h2d + bytes_d2h) / (ms * 1e-3) / 1e9,
+            "note": "pinned host buffers; copies, the assembly and copies back in one timed region"}
+
+
 def batched(n_meshes, steps, warmup):
     """Mixed-type batch: meshes with log-uniform element counts in [1e3, 5e4], tet4 / tet10 / hex8 /
     hex20 drawn uniformly, jittered grids (10 % of the spacing), local isotropic damage. Compares the
@@ -333,6 +363,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=24, help="cells per direction of the CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0, help="OpenMP threads for the CPU baseline (0: auto)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--pcie", action="store_true",
+                    help="also time the step with host-resident inputs and outputs (pinned copies in and out)")
     ap.add_argument("--adjoint-n", type=int, default=64, help="cells per direction of the adjoint workload")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--solve-iters", type=int, default=20,
@@ -433,6 +465,9 @@ def main():
         tflops = flops / (kernel_ms * 1e-3) / 1e12
         rec["fp64"] = {"flops_per_step": flops, "achieved_tflops": tflops, "peak_tflops": FP64_PEAK_TFLOPS,
                        "frac": tflops / FP64_PEAK_TFLOPS}
+    if args.pcie:
+        ctx.n_cells = n_cells
+        rec["pcie_inclusive"] = pcie_inclusive(ctx, u, e, kap, R, K)
     if args.solve_iters > 0:
         rec["solve"] = solve_bench(ctx, X, C, ranks, nn, K, R, args.solve_iters, 100.0 * n / 100)
     if not args.no_adjoint:
