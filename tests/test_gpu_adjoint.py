@@ -151,3 +151,36 @@ def test_vjp_rejects_other_integrands():
     u = to_dev(np.zeros(3 * nn))
     with pytest.raises(N.NutoError):
         ctx.vjp(u, None, None, u)
+
+
+def test_vjp_is_K_transpose_lambda_at_full_size():
+    """The differentiation workload at its full size (64^3 cells of a 64 mm cube, per-cell lognormal
+    E, f_t, G_f, the benchmark's state): the adjoint's d(u, eeq) equals K^T lambda formed on the
+    device from the forward K, within 1e-12 of |K|^T |lambda| (the VJP recomputes every IP and
+    never forms K, so the two roundings are independent)."""
+    n = 64
+    X, C = N.structured_mesh(N.HEX8, n, n, n, (float(n),) * 3)
+    ranks, nn = N.dof_layer(N.HEX8, C, N.HEX8)
+    ctx = N.Context()
+    ctx.set_cells(N.HEX8, X, C)
+    ctx.set_dof_type(0, N.HEX8, 3, 3 * nn, N.cell_dofs(ranks, 3))
+    ctx.set_dof_type(1, N.HEX8, 1, nn, N.cell_dofs(ranks, 1))
+    ctx.set_integrand(N.gradient_damage_params())
+    ctx.set_rule(0, 2)
+    nnz, n_rows = ctx.build_pattern()
+    assert n_rows == 1098500 and nnz == 115024912
+    nc = C.shape[0]
+    ctx.set_material_fields(*S.material_fields(nc))
+    u = torch.from_numpy(S.uniform(S.SEED_U, 3 * nn, -5e-3, 5e-3)).cuda()
+    e = torch.from_numpy(S.uniform(S.SEED_E, nn, 0.0, 5e-4)).cuda()
+    kap = torch.from_numpy(S.uniform(S.SEED_KAPPA, nc * 8, 0.0, 3e-4)).cuda()
+    lam = torch.from_numpy(S.uniform(0x3001, n_rows, -1.0, 1.0)).cuda()
+    _, K = ctx.build_vector_matrix(u, e, kap)
+    d_state, _ = ctx.vjp(u, e, kap, lam, want_theta=False)
+    rp, col = ctx.pattern_device()
+    rows = torch.repeat_interleave(torch.arange(n_rows, device="cuda"), (rp[1:] - rp[:-1]).long())
+    col = col.long()
+    kt = torch.zeros(n_rows, dtype=torch.float64, device="cuda").index_add_(0, col, K * lam[rows])
+    kb = torch.zeros(n_rows, dtype=torch.float64, device="cuda").index_add_(0, col, K.abs() * lam.abs()[rows])
+    err = (d_state - kt).abs()
+    assert bool((err <= 1e-12 * kb).all()), f"max err/bound {float((err / kb).max()):.3e}"
