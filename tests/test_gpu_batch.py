@@ -115,8 +115,8 @@ def test_autotune_picks_a_batched_mode(mixed):
 
 def test_group_limit_splits_types_same_results(mixed):
     """With a small group limit every element type spreads over several groups (super-meshes): each
-    mesh's R and K slices are bit-identical to the one-group-per-type batch, per group (mode 3) and
-    two-phase (mode 5) alike, and graph replay (mode -1) matches."""
+    mesh's R and K slices are bit-identical to the one-group-per-type batch: fused (mode 0), per
+    group (mode 3), two-phase (mode 5), and graph replay (mode -1)."""
     meshes, b, sizes, u, kap = mixed
     b2 = N.Batch()
     b2.set_integrand(N.local_damage_params())
@@ -135,7 +135,7 @@ def test_group_limit_splits_types_same_results(mixed):
         c = np.arange(j["cells"])
         for ip in range(j["nip"]):
             kap2[j["history"] + c + ip * j["history_ld"]] = h[ip * j["cells"] + c]
-    for mode in (3, 5, -1):
+    for mode in (0, 3, 5, -1):
         R, K = (t.cpu().numpy() for t in b.assemble(to_dev(u), None, to_dev(kap), mode=mode if mode >= 0 else 3))
         R2, K2 = (t.cpu().numpy() for t in b2.assemble(to_dev(u2), None, to_dev(kap2), mode=mode))
         for m in range(len(meshes)):
