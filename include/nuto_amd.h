@@ -58,6 +58,13 @@ typedef struct nuto_params
     int plane_state;     /* 0 PLANE_STRESS (reference default for LinearElastic<2>), 1 PLANE_STRAIN */
     double k, kappa0, beta, alpha, c;
     int damage_application; /* 0 FULL, 1 UNILATERAL (LinearElasticDamage.h:14-18) */
+    /* gradient damage only: NonlocalInteraction (GradientDamage.h:150-184), the factor g(omega) of
+     * the nonlocal term c g grad(eeq). 0 Constant (g = 1, the integrand's default), 1 Decreasing
+     * with R = interaction_r, eta = interaction_eta (the reference's defaults are 0.005 and 5).
+     * The node kernel, the adjoint and batches take Constant only; Decreasing runs on the
+     * generic row-owner kernel. */
+    int interaction;
+    double interaction_r, interaction_eta;
 } nuto_params;
 
 /* ---- set-up helpers (host only, no device needed) ------------------------------------------- */

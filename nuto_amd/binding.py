@@ -18,6 +18,7 @@ DIM = {QUAD4: 2, HEX8: 3, TET4: 3, TET10: 3, HEX20: 3}
 MOMENTUM_LINEAR_ELASTIC, GRADIENT_DAMAGE, MOMENTUM_LOCAL_DAMAGE = 0, 1, 2
 PLANE_STRESS, PLANE_STRAIN = 0, 1
 FULL, UNILATERAL = 0, 1
+CONSTANT, DECREASING = 0, 1  # NonlocalInteraction of gradient damage
 # methods of Context.solve (nuto_solve): EigenConjugateGradient, EigenBiCGSTAB
 SOLVER_CG, SOLVER_BICGSTAB = 0, 1
 NO_CONVERGENCE = 6  # NutoError.code of nuto_quasistatic_step without convergence (NewtonRaphson::NoConvergence)
@@ -65,13 +66,18 @@ class Params(ctypes.Structure):
         ("alpha", ctypes.c_double),
         ("c", ctypes.c_double),
         ("damage_application", ctypes.c_int),
+        ("interaction", ctypes.c_int),
+        ("interaction_r", ctypes.c_double),
+        ("interaction_eta", ctypes.c_double),
     ]
 
 
 def gradient_damage_params(E=30000.0, nu=0.2, k=10.0, kappa0=1e-4, beta=100.0, alpha=0.99, c=4.0,
-                           damage_application=FULL):
+                           damage_application=FULL, interaction=CONSTANT, interaction_r=0.005, interaction_eta=5.0):
+    """GradientDamage<3> parameters; interaction DECREASING selects NonlocalInteraction::Decreasing(R, eta)."""
     return Params(kind=GRADIENT_DAMAGE, dim=3, E=E, nu=nu, plane_state=0, k=k, kappa0=kappa0, beta=beta,
-                  alpha=alpha, c=c, damage_application=damage_application)
+                  alpha=alpha, c=c, damage_application=damage_application, interaction=interaction,
+                  interaction_r=interaction_r, interaction_eta=interaction_eta)
 
 
 def linear_elastic_params(dim, E, nu, plane_state=PLANE_STRESS):

@@ -232,7 +232,7 @@ __device__ inline void AssembleRowsBody(const RowArgs& A, int p, int pEnd)
                 double sig[NV];
                 double c1 = 0.0, c2 = 0.0, cm = 0.0;  // uu: c1 ga gb^T + c2 gb ga^T + c2 (ga.gb) I
                 double t6[6] = {0, 0, 0, 0, 0, 0}, m6[6] = {0, 0, 0, 0, 0, 0};
-                double rE = 0.0;
+                double rE = 0.0, cq = 0.0;  // cq: s c dg/dw dw/dk dk/de (grad eeq . g_a), the K_ee term of g'
                 double C11 = 0, C12 = 0, C33 = 0;
                 if constexpr (KIND == kLinearElastic)
                 {
@@ -301,9 +301,16 @@ __device__ inline void AssembleRowsBody(const RowArgs& A, int p, int pEnd)
                         t6[i] = ct * ds.dsdw[i];
                         m6[i] = cmv * mdev[i];
                     }
-                    if constexpr (KIND == kGradientDamage)
-                        rE = s * (Na * (eN - eq) + P.c * (ga[0] * ge[0] + ga[1] * ge[1] + ga[2] * ge[2]));
                     cm = P.c * s;
+                    if constexpr (KIND == kGradientDamage)
+                    {
+                        double gi, dgi;
+                        NonlocalFactor(P, om, gi, dgi);
+                        const double gdot = ga[0] * ge[0] + ga[1] * ge[1] + ga[2] * ge[2];
+                        rE = s * (Na * (eN - eq) + P.c * gi * gdot);
+                        cm *= gi;
+                        cq = s * P.c * gdot * dgi * dom * dk;
+                    }
                 }
 
                 if constexpr (WR)
@@ -379,7 +386,7 @@ __device__ inline void AssembleRowsBody(const RowArgs& A, int p, int pEnd)
                                 kb[3 * 4 + 0] -= Na * wb[0];
                                 kb[3 * 4 + 1] -= Na * wb[1];
                                 kb[3 * 4 + 2] -= Na * wb[2];
-                                kb[15] += s * Na * Nb + cm * dot;
+                                kb[15] += s * Na * Nb + cm * dot + cq * Nb;
                             }
                         }
                     }

@@ -23,7 +23,24 @@ struct LawParams
     int plane_state;         // 0 PLANE_STRESS, 1 PLANE_STRAIN (2D linear elasticity)
     // derived once per launch (see MakeDerived): 3K = cK E, 2G = cG E, modified Mises K1, K2
     double cK, cG, K1, K2;
+    int interaction;         // gradient damage: 0 NonlocalInteraction::Constant, 1 Decreasing
+    double iR, iEta;         // Decreasing(R, eta)
 };
+
+// NonlocalInteraction::Constant / Decreasing (GradientDamage.h:150-184): the factor g(omega) of the
+// nonlocal term and its derivative dg/domega
+__device__ __host__ inline void NonlocalFactor(const LawParams& P, double omega, double& g, double& dg)
+{
+    if (P.interaction == 0)
+    {
+        g = 1.0;
+        dg = 0.0;
+        return;
+    }
+    const double e = exp(-P.iEta * omega), eInf = exp(-P.iEta);
+    g = ((1. - P.iR) * e + P.iR - eInf) / (1. - eInf);
+    dg = ((1. - P.iR) * e) / (1. - eInf) * -P.iEta;
+}
 
 inline LawParams MakeDerived(LawParams p)
 {

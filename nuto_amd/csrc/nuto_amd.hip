@@ -266,7 +266,8 @@ RowArgs MakeRowArgs(nuto_ctx* ctx, const double* u, const double* e, const doubl
     a.tab = ctx->tab.As<double>();
     const nuto_params& p = ctx->params;
     a.P = MakeDerived(LawParams{p.E, p.nu, p.k, p.kappa0, p.beta, p.alpha, p.c, p.damage_application,
-                                p.plane_state, 0.0, 0.0, 0.0, 0.0});
+                                p.plane_state, 0.0, 0.0, 0.0, 0.0, p.interaction, p.interaction_r,
+                                p.interaction_eta});
     a.R = R;
     a.K = K;
     return a;
@@ -506,7 +507,8 @@ void Assemble(nuto_ctx* ctx, const double* u, const double* e, const double* kap
     if (!wr && !wk)
         return;
     PrepareGdm(ctx);
-    if (!ForceGeneric() && ctx->gdm.Usable(ctx->params.kind, ctx->coordType, ctx->f[0].type, ctx->nip))
+    if (!ForceGeneric() && ctx->params.interaction == 0 &&
+        ctx->gdm.Usable(ctx->params.kind, ctx->coordType, ctx->f[0].type, ctx->nip))
     {
         RowArgs a = MakeRowArgs(ctx, u, e, kappa, R, K);
         ctx->gdm.Launch(a, wr, wk, ctx->stream);
@@ -730,6 +732,9 @@ int nuto_set_integrand(nuto_ctx* ctx, const nuto_params* params)
                 "damage integrands are provided in 3D");
         Require(params->damage_application == 0 || params->damage_application == 1, NUTO_ERR_ARGUMENT,
                 "damage application must be FULL (0) or UNILATERAL (1)");
+        Require(params->interaction == 0 || (params->interaction == 1 && params->kind == kGradientDamage &&
+                                             params->interaction_eta != 0.0),
+                NUTO_ERR_ARGUMENT, "interaction must be Constant (0) or, for gradient damage, Decreasing (1) with eta != 0");
         ctx->params = *params;
         ctx->integrandSet = true;
         ctx->gdm.Release();
@@ -1303,6 +1308,8 @@ int nuto_vjp(nuto_ctx* ctx, const double* u, const double* e, const double* kapp
         Require(ctx->params.kind == kGradientDamage, NUTO_ERR_UNSUPPORTED,
                 "the adjoint is provided for the gradient-damage integrand");
         Require(lambda != nullptr && d_state != nullptr, NUTO_ERR_ARGUMENT, "lambda / d_state is NULL");
+        Require(ctx->params.interaction == 0, NUTO_ERR_UNSUPPORTED,
+                "the adjoint is provided for the constant nonlocal interaction");
         if (!ctx->tabValid)
             BuildTables(ctx);
         PrepareGdm(ctx);
@@ -1758,6 +1765,7 @@ int nuto_batch_set_integrand(nuto_batch* b, const nuto_params* params)
     return GuardB(b, [&] {
         Require(params != nullptr, NUTO_ERR_ARGUMENT, "params is NULL");
         Require(params->dim == 3, NUTO_ERR_UNSUPPORTED, "batches hold 3D meshes");
+        Require(params->interaction == 0, NUTO_ERR_UNSUPPORTED, "batches take the constant nonlocal interaction");
         Require(params->kind == kLinearElastic || params->kind == kLocalDamage || params->kind == kGradientDamage,
                 NUTO_ERR_ARGUMENT, "unknown integrand kind");
         ReleaseGroups(b);

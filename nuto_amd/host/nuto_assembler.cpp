@@ -33,6 +33,14 @@ struct DeviceArray
     DeviceArray& operator=(const DeviceArray&) = delete;
     double* d() const { return static_cast<double*>(p); }
 };
+// GradientDamage<3, TInteraction> as the C ABI's parameter block
+nuto_params GdmParams(const GradientDamage& f)
+{
+    nuto_params p{NUTO_GRADIENT_DAMAGE, 3, f.E, f.nu, 0, f.k, f.kappa0, f.beta, f.alpha, f.c, f.damageApplication,
+                  f.interaction, f.interactionR, f.interactionEta};
+    return p;
+}
+
 // The integration rule the group's cells use, as NuTo's tests construct it for each element pair:
 // tetrahedra order 1, order 2 when an interpolation is the ten-node one; tensor-product Gauss with
 // 2 points per direction, 3 when an interpolation is the twenty-node brick. Returns the IPs per cell.
@@ -173,8 +181,7 @@ void SimpleAssembler::Run(const std::vector<DofType>& dofs, const DofVector<doub
 DofVector<double> SimpleAssembler::BuildVector(const CellGroup& cells, const GradientDamage& f,
                                                const DofVector<double>& state)
 {
-    const nuto_params p{NUTO_GRADIENT_DAMAGE, 3, f.E, f.nu, 0, f.k, f.kappa0, f.beta, f.alpha, f.c,
-                        f.damageApplication};
+    const nuto_params p = GdmParams(f);
     Setup(cells, {f.disp, f.eeq}, p, 3);
     DofVector<double> R;
     Run({f.disp, f.eeq}, state, &f.kappas, true, false, &R, nullptr);
@@ -184,8 +191,7 @@ DofVector<double> SimpleAssembler::BuildVector(const CellGroup& cells, const Gra
 DofMatrixSparse<double> SimpleAssembler::BuildMatrix(const CellGroup& cells, const GradientDamage& f,
                                                      const DofVector<double>& state)
 {
-    const nuto_params p{NUTO_GRADIENT_DAMAGE, 3, f.E, f.nu, 0, f.k, f.kappa0, f.beta, f.alpha, f.c,
-                        f.damageApplication};
+    const nuto_params p = GdmParams(f);
     Setup(cells, {f.disp, f.eeq}, p, 3);
     DofMatrixSparse<double> K;
     Run({f.disp, f.eeq}, state, &f.kappas, false, true, nullptr, &K);
@@ -265,16 +271,14 @@ std::vector<double> SimpleAssembler::RunEval(const std::vector<DofType>& dofs, c
 std::vector<double> SimpleAssembler::Eval(const CellGroup& cells, const GradientDamage& f,
                                           const DofVector<double>& state, IpField field)
 {
-    const nuto_params p{NUTO_GRADIENT_DAMAGE, 3, f.E, f.nu, 0, f.k, f.kappa0, f.beta, f.alpha, f.c,
-                        f.damageApplication};
+    const nuto_params p = GdmParams(f);
     Setup(cells, {f.disp, f.eeq}, p, 3);
     return RunEval({f.disp, f.eeq}, state, &f.kappas, cells.NumCells(), 3, field);
 }
 
 void SimpleAssembler::UpdateHistory(const CellGroup& cells, GradientDamage& f, const DofVector<double>& state)
 {
-    const nuto_params p{NUTO_GRADIENT_DAMAGE, 3, f.E, f.nu, 0, f.k, f.kappa0, f.beta, f.alpha, f.c,
-                        f.damageApplication};
+    const nuto_params p = GdmParams(f);
     Setup(cells, {f.disp, f.eeq}, p, 3);
     const int nu = mDofInfo.Total(f.disp), ne = mDofInfo.Total(f.eeq);
     const size_t nk = (size_t)cells.NumCells() * mNip;
@@ -351,8 +355,7 @@ QuasistaticSolver::QuasistaticSolver(const CellGroup& cells, GradientDamage& f, 
     : mDofInfo(std::move(dofInfo))
     , mGdm(&f)
 {
-    const nuto_params p{NUTO_GRADIENT_DAMAGE, 3, f.E, f.nu, 0, f.k, f.kappa0, f.beta, f.alpha, f.c,
-                        f.damageApplication};
+    const nuto_params p = GdmParams(f);
     Init(cells, {f.disp, f.eeq}, p, 3, device);
     const size_t nk = (size_t)cells.NumCells() * mNip;
     if (f.kappas.size() != nk)

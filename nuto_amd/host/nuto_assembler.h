@@ -121,6 +121,9 @@ struct GradientDamage
     double c, E, nu, k, kappa0, beta, alpha;
     int damageApplication = 0;   // 0 FULL, 1 UNILATERAL
     std::vector<double> kappas;  // history, cells x IPs column-major (mKappas, GradientDamage.h:137)
+    // NonlocalInteraction (GradientDamage.h:150-184): 0 Constant (the default), 1 Decreasing(R, eta)
+    int interaction = 0;
+    double interactionR = 0.005, interactionEta = 5.0;
 };
 
 // MomentumBalance<D> with LinearElastic<D> (MomentumBalance.h:16, LinearElastic.h:16)

@@ -41,6 +41,9 @@ class Params(ctypes.Structure):
         ("alpha", ctypes.c_double),
         ("c", ctypes.c_double),
         ("damage_application", ctypes.c_int),
+        ("interaction", ctypes.c_int),
+        ("interaction_r", ctypes.c_double),
+        ("interaction_eta", ctypes.c_double),
     ]
 
 

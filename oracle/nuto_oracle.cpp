@@ -639,7 +639,24 @@ struct Params
     int plane_state;  // 0 PLANE_STRESS, 1 PLANE_STRAIN (LinearElastic<2> only)
     double k, kappa0, beta, alpha, c;
     int damage_application;  // 0 FULL, 1 UNILATERAL
+    int interaction;         // GradientDamage's NonlocalInteraction: 0 Constant, 1 Decreasing
+    double interaction_r, interaction_eta;
 };
+
+// NonlocalInteraction::Constant::Factor/Derivative and Decreasing::Factor/Derivative
+// (GradientDamage.h:153-184)
+static void Interaction(const Params& p, double omega, double& g, double& dgdw)
+{
+    if (p.interaction == 0)
+    {
+        g = 1.;
+        dgdw = 0.;
+        return;
+    }
+    const double R = p.interaction_r, eta = p.interaction_eta;
+    g = ((1. - R) * std::exp(-eta * omega) + R - std::exp(-eta)) / (1. - std::exp(-eta));
+    dgdw = ((1. - R) * std::exp(-eta * omega)) / (1. - std::exp(-eta)) * -eta;
+}
 
 // LinearElastic<D>::CalculateC (LinearElastic.h:50-121)
 static Mat ElasticC(int dim, double E, double nu, int planeState)
@@ -1226,7 +1243,7 @@ static void IntegrateCell(const Problem& P, int c, const double* u, const double
         }
         else
         {
-            // GradientDamage<3>::Gradient/Hessian0 (GradientDamage.h:70-135), Constant interaction
+            // GradientDamage<3>::Gradient/Hessian0 (GradientDamage.h:70-135)
             ElasticDamage3 led(law.E, P.p.nu, P.p.damage_application);
             ModMises3 norm(P.p.nu, P.p.k);
             DamageExp dl{law.kappa0, law.beta, P.p.alpha};
@@ -1245,7 +1262,8 @@ static void IntegrateCell(const Problem& P, int c, const double* u, const double
                 for (int i = 3; i < 6; i++)
                     strainM.v[i] *= 0.5;
             const double* strain = strainM.v.data();
-            const double g = 1., dgdw = 0.;
+            double g, dgdw;
+            Interaction(P.p, omega, g, dgdw);
             Mat strainMag = MulAbs(Bdm, Col(nodeVal[0]));
             LawMag lm = LawMagnitudes(strainMag.v.data(), omega, law.E, P.p.nu, P.p.k, P.p.damage_application,
                                       MisesA(strain, P.p.nu, P.p.k));
@@ -1278,6 +1296,10 @@ static void IntegrateCell(const Problem& P, int c, const double* u, const double
                 mm.v.assign(lm.m, lm.m + 6);
                 Kmag[1][0] = MulAbs(MulAbs(T(Ne), mm), Bdm);
                 Kmag[1][1] = Add(MulAbs(T(Ne), Ne), Scale(MulAbs(T(Bem), Bem), P.p.c));
+                if (dgdw != 0. && dwdk * dkde != 0.)  // shadow of the g' term
+                    Kmag[1][1] = Add(Kmag[1][1], Mul(Scale(MulAbs(T(Bem), MulAbs(Bem, Col(nodeVal[1]))),
+                                                           P.p.c * std::abs(dgdw * dwdk)),
+                                                     Abs(Ne)));
                 Mat dm = Col(std::vector<double>(lm.dsw, lm.dsw + 6));
                 Kmag[0][1] = MulAbs(MulAbs(T(Bdm), Scale(dm, std::abs(dwdk) * dkde)), Ne);
             }

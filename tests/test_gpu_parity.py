@@ -116,6 +116,26 @@ def test_hex8_gradient_damage(app):
     run_pair(p, u, e, kap)
 
 
+@pytest.mark.parametrize("app", [N.FULL, N.UNILATERAL])
+def test_hex8_gradient_damage_decreasing_interaction(app):
+    """NonlocalInteraction::Decreasing(R = 0.005, eta = 5) on a jittered mesh, damaged and undamaged
+    IPs mixed: the generic row-owner kernel (the node kernel takes Constant only) against the
+    oracle; the adjoint and batches refuse it."""
+    X, C = make_mesh(N.HEX8, (4, 3, 3), (4.0, 3.0, 3.0), jitter=0.1)
+    dev = N.gradient_damage_params(damage_application=app, interaction=N.DECREASING, interaction_r=0.005,
+                                   interaction_eta=5.0)
+    orc = O.Params(kind=1, dim=3, E=30000.0, nu=0.2, k=10.0, kappa0=1e-4, beta=100.0, alpha=0.99, c=4.0,
+                   damage_application=app, interaction=1, interaction_r=0.005, interaction_eta=5.0)
+    p = Pair(N.HEX8, N.HEX8, X, C, dev, orc, True)
+    u, e, kap = p.inputs()
+    ctx, R, K, mr, mk = run_pair(p, u, e, kap)
+    with pytest.raises(N.NutoError):
+        ctx.vjp(to_dev(u), to_dev(e), to_dev(kap), to_dev(np.ones(p.n_u + p.n_e)))
+    b = N.Batch()
+    with pytest.raises(N.NutoError):
+        b.set_integrand(dev)
+
+
 def test_hex8_gradient_damage_fixture():
     f = np.load(os.path.join(GOLD, "hex8_gdm_3x3x3.npz"))
     X, C = make_mesh(N.HEX8, (3, 3, 3), (3.0, 3.0, 3.0))

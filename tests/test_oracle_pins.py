@@ -266,21 +266,23 @@ def test_linear_elastic_Ku_equals_R_and_rigid_translation():
     assert np.abs(K - K.T).max() < 1e-11 * np.abs(K).max()
 
 
-def _gdm_problem(n=2, scale=2.0, app=0):
+def _gdm_problem(n=2, scale=2.0, app=0, interaction=0):
     m = O.Mesh.bricks(n, n, n)
     m.scale(scale)
     Ld = m.add_dof(O.HEX8, 3)
     Le = m.add_dof(O.HEX8, 1)
     p = O.Params(kind=1, dim=3, E=30000.0, nu=0.2, k=10.0, kappa0=1e-4, beta=100.0, alpha=0.99, c=4.0,
-                 damage_application=app)
+                 damage_application=app, interaction=interaction, interaction_r=0.005, interaction_eta=5.0)
     return m, O.Problem(m, p, 0, 2, Ld, Le)
 
 
+@pytest.mark.parametrize("interaction", [0, 1])
 @pytest.mark.parametrize("state", ["elastic", "loading", "unloading"])
-def test_gdm_hessian_is_derivative_of_gradient(state):
+def test_gdm_hessian_is_derivative_of_gradient(state, interaction):
     # GradientDamage.cpp:11-122 checks Hessian0 against a forward difference of Gradient in 1D;
-    # here: central differences of the hex8 element vector, all 32 columns.
-    m, P = _gdm_problem(1, 1.0)
+    # here: central differences of the hex8 element vector, all 32 columns, for the Constant and
+    # the Decreasing nonlocal interaction (whose g'(omega) term enters K_ee while loading).
+    m, P = _gdm_problem(1, 1.0, interaction=interaction)
     nd, ne = P.ndof
     rng = np.random.default_rng({"elastic": 1, "loading": 2, "unloading": 3}[state])
     u = rng.uniform(-5e-4, 5e-4, nd)
@@ -305,6 +307,30 @@ def test_gdm_hessian_is_derivative_of_gradient(state):
         Rm, _ = P.cell(0, xm[:nd], xm[nd:], kap, want_k=False)
         fd = (Rp - Rm) / (2 * h)
         np.testing.assert_allclose(K[:, j], fd, rtol=2e-5, atol=2e-5 * np.abs(K).max())
+
+
+def test_decreasing_interaction_is_constant_while_undamaged():
+    """Decreasing::Factor(0) = 1 (GradientDamage.h:174-177) and omega' = 0 below kappa0: an undamaged
+    state assembles to the Constant interaction's R and K (to rounding of g(0)); once damaged,
+    g < 1 scales the nonlocal term and the results differ."""
+    rng = np.random.default_rng(11)
+    outs = {}
+    for interaction in (0, 1):
+        m, P = _gdm_problem(2, 2.0, interaction=interaction)
+        nd, ne = P.ndof
+        u = rng.uniform(-5e-6, 5e-6, nd) if interaction == 0 else outs[0][2]
+        e = rng.uniform(0, 5e-5, ne) if interaction == 0 else outs[0][3]
+        kap = np.full(8 * 8, 9e-5)  # kappa0 = 1e-4: undamaged
+        o = P.assemble(u, e, kap)
+        outs[interaction] = (o["R"], o["val"], u, e)
+    np.testing.assert_allclose(outs[1][0], outs[0][0], rtol=0, atol=1e-15 * np.abs(outs[0][0]).max())
+    np.testing.assert_allclose(outs[1][1], outs[0][1], rtol=0, atol=1e-15 * np.abs(outs[0][1]).max())
+    damaged = {}
+    for interaction in (0, 1):
+        m, P = _gdm_problem(2, 2.0, interaction=interaction)
+        o = P.assemble(outs[0][2], outs[0][3] + 3e-4, np.full(64, 3e-4))
+        damaged[interaction] = o["R"]
+    assert np.abs(damaged[1] - damaged[0]).max() > 1e-6 * np.abs(damaged[0]).max()
 
 
 def test_local_damage_tangent_fd():
