@@ -163,6 +163,46 @@ static void GradientDamageCase()
     EXPECT(outOfRange, "missing DofVector entry throws std::out_of_range");
 }
 
+// GradientDamage with NonlocalInteraction::Decreasing(0.005, 5) through the mirror (the device runs
+// it on the generic row-owner kernel) against the oracle
+static void DecreasingInteractionCase()
+{
+    Mesh m = MakeMesh(NUTO_HEX8, 3, 2, 2);
+    DofType disp("Displacements", 3), eeq("NonlocalEqStrain", 1);
+    DofInfo info;
+    info.numIndependent[disp.Id()] = 3 * m.nDofNodes;
+    info.numIndependent[eeq.Id()] = m.nDofNodes;
+    CellGroup cells;
+    cells.coordType = NUTO_HEX8;
+    cells.coords = m.X;
+    cells.conn = m.C;
+    cells.dofNumbering[disp.Id()] = {NUTO_HEX8, CellDofs(m.ranks, 3)};
+    cells.dofNumbering[eeq.Id()] = {NUTO_HEX8, CellDofs(m.ranks, 1)};
+    GradientDamage f{disp, eeq, 4.0, 30000.0, 0.2, 10.0, 1e-4, 100.0, 0.99, 0, {}};
+    f.interaction = 1;
+    unsigned long long seed = 99;
+    DofVector<double> state;
+    for (int i = 0; i < 3 * m.nDofNodes; i++)
+        state[disp].push_back(Uniform(seed, -5e-3, 5e-3));
+    for (int i = 0; i < m.nDofNodes; i++)
+        state[eeq].push_back(Uniform(seed, 0.0, 5e-4));
+    for (int i = 0; i < m.nCells * 8; i++)
+        f.kappas.push_back(Uniform(seed, 0.0, 3e-4));
+    SimpleAssembler asmbl(info);
+    DofVector<double> R = asmbl.BuildVector(cells, f, state);
+    DofMatrixSparse<double> K = asmbl.BuildMatrix(cells, f, state);
+    std::vector<double> Rall = R[disp];
+    Rall.insert(Rall.end(), R[eeq].begin(), R[eeq].end());
+    void* om = orc_mesh_from_arrays(NUTO_HEX8, m.nNodes, m.X.data(), m.nCells, m.C.data());
+    const int Ld = orc_mesh_add_dof(om, NUTO_HEX8, 3), Le = orc_mesh_add_dof(om, NUTO_HEX8, 1);
+    nuto_params p{NUTO_GRADIENT_DAMAGE, 3, 30000.0, 0.2, 0, 10.0, 1e-4, 100.0, 0.99, 4.0, 0, 1, 0.005, 5.0};
+    void* P = orc_problem(om, &p, 0, 2, Ld, Le);
+    orc_problem_assemble(P, state[disp].data(), state[eeq].data(), f.kappas.data(), 1, 1, 1);
+    Compare("gradient damage, decreasing interaction", P, 4 * m.nDofNodes, Rall, K);
+    orc_problem_free(P);
+    orc_mesh_free(om);
+}
+
 static void MomentumBalanceCase()
 {
     Mesh m = MakeMesh(NUTO_QUAD4, 5, 4, 0);
@@ -493,6 +533,7 @@ int main()
     try
     {
         GradientDamageCase();
+        DecreasingInteractionCase();
         MomentumBalanceCase();
         LumpedMassAndEvalCase();
         QuasistaticCase();
