@@ -205,6 +205,28 @@ def _dev_ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _f64(t, n, name, optional=True):
+    """t, checked to be a float64 tensor of at least n values (None stays None when optional). The C
+    ABI takes bare pointers without lengths, so a float32 or short tensor is refused here, before a
+    kernel could read or write past its end. Every argument of a call is checked this way before any
+    is turned into a device pointer (_ptrs)."""
+    if t is None:
+        if optional:
+            return None
+        raise ValueError(f"{name}: a tensor is required")
+    dt = str(t.dtype)
+    if dt != "torch.float64":
+        raise ValueError(f"{name}: expected float64, got {dt}")
+    if t.numel() < n:
+        raise ValueError(f"{name}: {t.numel()} values, needs {n}")
+    return t
+
+
+def _ptrs(*ts):
+    """Device pointers of checked tensors (contiguous, on the device), in order."""
+    return tuple(_dev_ptr(t) for t in ts)
+
+
 # ---- set-up helpers (host) --------------------------------------------------------------------
 def structured_mesh(coord_type, nx, ny, nz=0, lengths=(1.0, 1.0, 1.0), jitter=0.0, seed=0):
     """Structured mesh: returns (coords n_nodes x dim, conn n_cells x nodes)."""
@@ -419,8 +441,33 @@ class Context:
         self.device = device
         if use_torch_stream:
             self.set_stream(torch.cuda.current_stream(device).cuda_stream)
+        self._init_sizes()
+
+    def _init_sizes(self):
         self.n_rows = 0
         self.nnz = 0
+        self.n_cells = 0
+        self.n_ip = 0
+        self._n_dofs = {}
+        self._n_eq = {}
+        self._history_ld = None
+
+    # lengths of the device arrays the C ABI reads or writes (checked before each call)
+    def _n_u(self):
+        return self._n_dofs.get(0, 0)
+
+    def _n_e(self):
+        return self._n_dofs.get(1, 0)
+
+    def _n_hist(self):
+        return (self.n_cells if self._history_ld is None else self._history_ld) * self.n_ip
+
+    def _n_rhs(self):
+        return sum(self._n_eq.values())
+
+    def _state(self, u, e, kappa):
+        return (_f64(u, self._n_u(), "u", optional=False), _f64(e, self._n_e(), "e"),
+                _f64(kappa, self._n_hist(), "kappa"))
 
     def close(self):
         h, L = getattr(self, "h", None), getattr(self, "_L", None)
@@ -453,6 +500,8 @@ class Context:
     def set_dof_type(self, slot, dof_type, ncomp, n_dofs, cell_dofs_):
         cd = np.ascontiguousarray(cell_dofs_, np.int32)
         self._chk(lib().nuto_set_dof_type(self.h, slot, dof_type, ncomp, n_dofs, _np_ptr(cd)))
+        self._n_dofs[slot] = n_dofs
+        self._n_eq[slot] = 0  # setting the dof type removes the slot's constraint equations
         if slot == 0:
             self.n_dofs0, self.ncomp0 = n_dofs, ncomp
 
@@ -466,6 +515,7 @@ class Context:
 
     def set_history_ld(self, ld):
         self._chk(lib().nuto_set_history_ld(self.h, ld))
+        self._history_ld = ld if ld > 0 else None
 
     def set_material_fields(self, E=None, ft=None, gf=None):
         a = [None if v is None else np.ascontiguousarray(v, np.float64) for v in (E, ft, gf)]
@@ -506,38 +556,38 @@ class Context:
 
     def build_vector(self, u, e=None, kappa=None, out=None):
         R = self._new(self.n_rows) if out is None else out
-        self._chk(lib().nuto_build_vector(self.h, _dev_ptr(u), _dev_ptr(e), _dev_ptr(kappa), _dev_ptr(R)))
+        self._chk(lib().nuto_build_vector(self.h, *_ptrs(*self._state(u, e, kappa), _f64(R, self.n_rows, "out"))))
         return R
 
     def build_matrix(self, u, e=None, kappa=None, out=None):
         K = self._new(self.nnz) if out is None else out
-        self._chk(lib().nuto_build_matrix(self.h, _dev_ptr(u), _dev_ptr(e), _dev_ptr(kappa), _dev_ptr(K)))
+        self._chk(lib().nuto_build_matrix(self.h, *_ptrs(*self._state(u, e, kappa), _f64(K, self.nnz, "out"))))
         return K
 
     def build_vector_matrix(self, u, e=None, kappa=None, R=None, K=None):
         R = self._new(self.n_rows) if R is None else R
         K = self._new(self.nnz) if K is None else K
-        self._chk(lib().nuto_build_vector_matrix(self.h, _dev_ptr(u), _dev_ptr(e), _dev_ptr(kappa), _dev_ptr(R),
-                                                 _dev_ptr(K)))
+        self._chk(lib().nuto_build_vector_matrix(self.h, *_ptrs(*self._state(u, e, kappa), _f64(R, self.n_rows, "R"),
+                                                         _f64(K, self.nnz, "K"))))
         return R, K
 
     def vjp(self, u, e, kappa, lam, want_theta=True):
         """Adjoint of build_vector: (K^T lam, dR/dtheta^T lam as (3, n_cells) or None)."""
+        args = (*self._state(u, e, kappa), _f64(lam, self.n_rows, "lam", optional=False))
         d_state = self._new(self.n_rows)
         d_theta = self._new(3 * self.n_cells) if want_theta else None
-        self._chk(lib().nuto_vjp(self.h, _dev_ptr(u), _dev_ptr(e), _dev_ptr(kappa), _dev_ptr(lam), _dev_ptr(d_state),
-                                 _dev_ptr(d_theta)))
+        self._chk(lib().nuto_vjp(self.h, *_ptrs(*args, d_state, d_theta)))
         return d_state, (None if d_theta is None else d_theta.view(3, self.n_cells))
 
     def update_history(self, u, e, kappa):
-        self._chk(lib().nuto_update_history(self.h, _dev_ptr(u), _dev_ptr(e), _dev_ptr(kappa)))
+        self._chk(lib().nuto_update_history(self.h, *_ptrs(*self._state(u, e, kappa))))
         return kappa
 
     def build_lumped_mass(self, rho, out=None):
         """BuildDiagonallyLumpedMatrix with DynamicMomentumBalance::Hessian2 (N^T N rho): one value
         per dof of slot 0."""
         M = self._new(self.n_dofs0) if out is None else out
-        self._chk(lib().nuto_build_lumped_mass(self.h, float(rho), _dev_ptr(M)))
+        self._chk(lib().nuto_build_lumped_mass(self.h, float(rho), *_ptrs(_f64(M, self._n_u(), "out"))))
         return M
 
     def set_constraints(self, slot, equations):
@@ -555,6 +605,7 @@ class Context:
         tc = np.array(tc, np.float64)
         self._chk(lib().nuto_set_constraints(self.h, slot, len(dep), _np_ptr(dep), _np_ptr(tp), _np_ptr(td),
                                              _np_ptr(tc)))
+        self._n_eq[slot] = len(dep)
 
     def _solve_opts(self, method, tol, max_iter):
         return SolverOptions(method, tol, max_iter)
@@ -564,8 +615,9 @@ class Context:
         x = self._new(self.n_rows) if out is None else out
         info = SolveInfo()
         opt = self._solve_opts(method, tol, max_iter)
-        self._chk(lib().nuto_solve(self.h, _dev_ptr(K), _dev_ptr(f), _dev_ptr(x), ctypes.byref(opt),
-                                   ctypes.byref(info)))
+        self._chk(lib().nuto_solve(self.h, *_ptrs(_f64(K, self.nnz, "K", optional=False),
+                                           _f64(f, self.n_rows, "f", optional=False), _f64(x, self.n_rows, "out")),
+                                   ctypes.byref(opt), ctypes.byref(info)))
         return x, {"iterations": info.iterations, "relative_residual": info.relative_residual,
                    "independent_dofs": info.independent_dofs}
 
@@ -574,9 +626,12 @@ class Context:
         x = self._new(self.n_rows) if out is None else out
         info = SolveInfo()
         opt = self._solve_opts(method, tol, max_iter)
-        self._chk(lib().nuto_solve_trial_state(self.h, _dev_ptr(K), _dev_ptr(f), _dev_ptr(rhs_old),
-                                               _dev_ptr(rhs_new), _dev_ptr(x), ctypes.byref(opt),
-                                               ctypes.byref(info)))
+        self._chk(lib().nuto_solve_trial_state(self.h, *_ptrs(_f64(K, self.nnz, "K", optional=False),
+                                                       _f64(f, self.n_rows, "f", optional=False),
+                                                       _f64(rhs_old, self._n_rhs(), "rhs_old"),
+                                                       _f64(rhs_new, self._n_rhs(), "rhs_new"),
+                                                       _f64(x, self.n_rows, "out")),
+                                               ctypes.byref(opt), ctypes.byref(info)))
         return x, {"iterations": info.iterations, "relative_residual": info.relative_residual,
                    "independent_dofs": info.independent_dofs}
 
@@ -589,8 +644,11 @@ class Context:
         o = NewtonOptions(float(tol), int(max_iter), int(line_search_steps),
                           SolverOptions(int(method), float(linear_tol), int(linear_max_iter)))
         info = NewtonInfo()
-        self._chk(lib().nuto_quasistatic_step(self.h, _dev_ptr(x), _dev_ptr(kappa), _dev_ptr(rhs_old),
-                                              _dev_ptr(rhs_new), ctypes.byref(o), ctypes.byref(info)))
+        self._chk(lib().nuto_quasistatic_step(self.h, *_ptrs(_f64(x, self.n_rows, "x", optional=False),
+                                                      _f64(kappa, self._n_hist(), "kappa"),
+                                                      _f64(rhs_old, self._n_rhs(), "rhs_old"),
+                                                      _f64(rhs_new, self._n_rhs(), "rhs_new")),
+                                              ctypes.byref(o), ctypes.byref(info)))
         return {"iterations": info.iterations, "linear_iterations": info.linear_iterations,
                 "residual_evaluations": info.residual_evaluations, "residual_norm": info.residual_norm}
 
@@ -599,7 +657,8 @@ class Context:
         (cell, IP): a (n_cells, n_ip, n_comp) tensor."""
         nc = (3 if self.dim == 2 else 6) if field in (IP_STRAIN, IP_STRESS) else 1
         o = self._new(self.n_cells * self.n_ip * nc) if out is None else out
-        self._chk(lib().nuto_eval_ip(self.h, _dev_ptr(u), _dev_ptr(e), _dev_ptr(kappa), field, _dev_ptr(o)))
+        pu, pe, pk, po = _ptrs(*self._state(u, e, kappa), _f64(o, self.n_cells * self.n_ip * nc, "out"))
+        self._chk(lib().nuto_eval_ip(self.h, pu, pe, pk, field, po))
         return o.view(self.n_cells, self.n_ip, nc)
 
 
@@ -653,6 +712,13 @@ class Batch:
     def set_integrand(self, params):
         self._chk(lib().nuto_batch_set_integrand(self.h, ctypes.byref(params)))
 
+    def _state(self, u, e, kappa):
+        if self.sizes is None:
+            raise ValueError("build() the batch first")
+        sz = self.sizes
+        return (_f64(u, sz["u"], "u", optional=False), _f64(e, sz["e"], "e"),
+                _f64(kappa, sz["history"], "kappa"))
+
     def add_mesh(self, coord_type, dof_type, coords, conn, ranks, n_dof_nodes):
         coords = np.ascontiguousarray(coords, np.float64)
         conn = np.ascontiguousarray(conn, np.int32)
@@ -698,43 +764,48 @@ class Batch:
         """R and K of every mesh (nuto_batch_assemble). mode -1: the tuned launch replayed from a
         HIP graph; 0..2 fused row-owner launch (64 / 128 / 256 nodes per workgroup); 3 row-owner per
         element type; 4 row-owner per mesh; 5 two-phase per element type; 6 two-phase per mesh."""
-        R = self._new(self.sizes["R"]) if R is None else R
-        K = self._new(self.sizes["K"]) if K is None else K
+        state = self._state(u, e, kappa)
+        R = self._new(self.sizes["R"]) if R is None else _f64(R, self.sizes["R"], "R")
+        K = self._new(self.sizes["K"]) if K is None else _f64(K, self.sizes["K"], "K")
+        args = _ptrs(*state, R, K)
         self._enter()
-        self._chk(lib().nuto_batch_assemble(self.h, _dev_ptr(u), _dev_ptr(e), _dev_ptr(kappa), _dev_ptr(R),
-                                            _dev_ptr(K), mode))
+        self._chk(lib().nuto_batch_assemble(self.h, *args, mode))
         self._leave()
         return R, K
 
     def autotune(self, u, e=None, kappa=None, R=None, K=None):
-        R = self._new(self.sizes["R"]) if R is None else R
-        K = self._new(self.sizes["K"]) if K is None else K
+        state = self._state(u, e, kappa)
+        R = self._new(self.sizes["R"]) if R is None else _f64(R, self.sizes["R"], "R")
+        K = self._new(self.sizes["K"]) if K is None else _f64(K, self.sizes["K"], "K")
+        args = _ptrs(*state, R, K)
         ms = np.zeros(7)
         best = ctypes.c_int()
         self._enter()
-        self._chk(lib().nuto_batch_autotune(self.h, _dev_ptr(u), _dev_ptr(e), _dev_ptr(kappa), _dev_ptr(R),
-                                            _dev_ptr(K), _np_ptr(ms), ctypes.byref(best)))
+        self._chk(lib().nuto_batch_autotune(self.h, *args, _np_ptr(ms), ctypes.byref(best)))
         self._leave()
         return ms, best.value
 
     def vjp(self, u, e, kappa, lam):
+        args = (*self._state(u, e, kappa), _f64(lam, self.sizes["R"], "lam", optional=False))
         d_state = self._new(self.sizes["R"])
         d_theta = self._new(self.sizes["theta"])
+        ptrs = _ptrs(*args, d_state, d_theta)
         self._enter()
-        self._chk(lib().nuto_batch_vjp(self.h, _dev_ptr(u), _dev_ptr(e), _dev_ptr(kappa), _dev_ptr(lam),
-                                       _dev_ptr(d_state), _dev_ptr(d_theta)))
+        self._chk(lib().nuto_batch_vjp(self.h, *ptrs))
         self._leave()
         return d_state, d_theta
 
     def residual_adjoint(self, u, e, kappa, r_obs, R=None, K=None, d_state=None, d_theta=None):
-        R = self._new(self.sizes["R"]) if R is None else R
-        K = self._new(self.sizes["K"]) if K is None else K
-        d_state = self._new(self.sizes["R"]) if d_state is None else d_state
-        d_theta = self._new(self.sizes["theta"]) if d_theta is None else d_theta
+        state = self._state(u, e, kappa)
+        sz = self.sizes
+        r_obs = _f64(r_obs, sz["R"], "r_obs", optional=False)
+        R = self._new(sz["R"]) if R is None else _f64(R, sz["R"], "R")
+        K = self._new(sz["K"]) if K is None else _f64(K, sz["K"], "K")
+        d_state = self._new(sz["R"]) if d_state is None else _f64(d_state, sz["R"], "d_state")
+        d_theta = self._new(sz["theta"]) if d_theta is None else _f64(d_theta, sz["theta"], "d_theta")
+        ptrs = _ptrs(*state, r_obs, R, K, d_state, d_theta)
         self._enter()
-        self._chk(lib().nuto_batch_residual_adjoint(self.h, _dev_ptr(u), _dev_ptr(e), _dev_ptr(kappa),
-                                                    _dev_ptr(r_obs), _dev_ptr(R), _dev_ptr(K), _dev_ptr(d_state),
-                                                    _dev_ptr(d_theta)))
+        self._chk(lib().nuto_batch_residual_adjoint(self.h, *ptrs))
         self._leave()
         return R, K, d_state, d_theta
 

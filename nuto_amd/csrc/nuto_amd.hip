@@ -506,9 +506,11 @@ void Assemble(nuto_ctx* ctx, const double* u, const double* e, const double* kap
     const bool wr = R != nullptr, wk = K != nullptr;
     if (!wr && !wk)
         return;
-    PrepareGdm(ctx);
-    if (!ForceGeneric() && ctx->params.interaction == 0 &&
-        ctx->gdm.Usable(ctx->params.kind, ctx->coordType, ctx->f[0].type, ctx->nip))
+    // the node kernel takes the Constant nonlocal interaction only: no tables for other cases
+    const bool nodeKernel = !ForceGeneric() && ctx->params.interaction == 0;
+    if (nodeKernel)
+        PrepareGdm(ctx);
+    if (nodeKernel && ctx->gdm.Usable(ctx->params.kind, ctx->coordType, ctx->f[0].type, ctx->nip))
     {
         RowArgs a = MakeRowArgs(ctx, u, e, kappa, R, K);
         ctx->gdm.Launch(a, wr, wk, ctx->stream);

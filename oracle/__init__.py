@@ -98,6 +98,8 @@ def lib():
         L.orc_problem_eval_ip.argtypes = [P, P, P, P, ctypes.c_int, P]
         L.orc_mises.restype = ctypes.c_double
         L.orc_damage_exp.restype = ctypes.c_double
+        L.orc_interaction.restype = ctypes.c_double
+        L.orc_interaction.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, P]
         L.orc_mock_assemble.restype = ctypes.c_longlong
         _lib = L
     return _lib
@@ -178,6 +180,13 @@ def damage_exp(k0, beta, alpha, kappa):
     v = lib().orc_damage_exp(ctypes.c_double(k0), ctypes.c_double(beta), ctypes.c_double(alpha),
                              ctypes.c_double(kappa), ctypes.byref(d))
     return v, d.value
+
+
+def interaction(kind, r, eta, omega):
+    """NonlocalInteraction factor g(omega) and derivative g'(omega) (kind 0 Constant, 1 Decreasing)."""
+    d = ctypes.c_double(0.0)
+    g = lib().orc_interaction(int(kind), float(r), float(eta), float(omega), ctypes.byref(d))
+    return g, d.value
 
 
 def elastic_c(dim, E, nu, plane_state=0):

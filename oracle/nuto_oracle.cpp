@@ -1264,6 +1264,9 @@ static void IntegrateCell(const Problem& P, int c, const double* u, const double
             const double* strain = strainM.v.data();
             double g, dgdw;
             Interaction(P.p, omega, g, dgdw);
+            // the nonlocal length c of R_e and K_ee; mutations 7 and 8 (tests only) change it
+            // consistently in both, which the finite-difference check of K cannot see
+            const double cNl = g_mutation == 7 ? P.p.c * P.p.c : (g_mutation == 8 ? -P.p.c : P.p.c);
             Mat strainMag = MulAbs(Bdm, Col(nodeVal[0]));
             LawMag lm = LawMagnitudes(strainMag.v.data(), omega, law.E, P.p.nu, P.p.k, P.p.damage_application,
                                       MisesA(strain, P.p.nu, P.p.k));
@@ -1271,7 +1274,7 @@ static void IntegrateCell(const Problem& P, int c, const double* u, const double
             {
                 Rip[0] = Mul(T(Bd), Col(led.Stress(strain, omega))).v;
                 Mat r1 = Scale(T(Ne), eeq - norm.Value(strain));
-                Mat r2 = Mul(T(Be), Scale(eeqGrad, (g_mutation == 5 ? -1.0 : 1.0) * P.p.c * g));
+                Mat r2 = Mul(T(Be), Scale(eeqGrad, (g_mutation == 5 ? -1.0 : 1.0) * cNl * g));
                 Rip[1] = Add(r1, r2).v;
                 Rmag[0] = MulAbs(T(Bdm), Col(std::vector<double>(lm.sig, lm.sig + 6))).v;
                 Mat m1 = Scale(Abs(T(Ne)), std::abs(eeq) + lm.eeq);
@@ -1286,8 +1289,8 @@ static void IntegrateCell(const Problem& P, int c, const double* u, const double
                 Mat dn(1, 6);
                 dn.v = norm.Derivative(strain);
                 Kip[1][0] = Mul(Mul(Scale(T(Ne), g_mutation == 2 ? 1. : -1.), dn), Bd);
-                Mat ee = g_mutation == 1 ? Mul(T(Ne), Ne) : Add(Mul(T(Ne), Ne), Scale(Mul(T(Be), Be), P.p.c * g));
-                Mat ee3 = Mul(Scale(Scale(Mul(T(Be), Scale(eeqGrad, P.p.c)), dgdw), dwdk * dkde), Ne);
+                Mat ee = g_mutation == 1 ? Mul(T(Ne), Ne) : Add(Mul(T(Ne), Ne), Scale(Mul(T(Be), Be), cNl * g));
+                Mat ee3 = Mul(Scale(Scale(Mul(T(Be), Scale(eeqGrad, cNl)), dgdw), dwdk * dkde), Ne);
                 Kip[1][1] = Add(ee, ee3);
                 Mat dsw = Col(led.DstressDomega(strain));
                 Kip[0][1] = Mul(Mul(T(Bd), Scale(Scale(dsw, dwdk), dkde)), Ne);
@@ -1826,6 +1829,20 @@ double orc_mises(double nu, double k, const double* strain, double* deriv)
         std::copy(d.begin(), d.end(), deriv);
     }
     return m.Value(strain);
+}
+
+// NonlocalInteraction Factor (return value) and Derivative (*deriv) at omega
+double orc_interaction(int interaction, double r, double eta, double omega, double* deriv)
+{
+    Params p{};
+    p.interaction = interaction;
+    p.interaction_r = r;
+    p.interaction_eta = eta;
+    double g, dg;
+    Interaction(p, omega, g, dg);
+    if (deriv)
+        *deriv = dg;
+    return g;
 }
 
 double orc_damage_exp(double k0, double beta, double alpha, double kappa, double* deriv)

@@ -90,3 +90,44 @@ def test_linear_patch(big):
     inner = ((xn > 0.5) & (xn < 99.5)).all(dim=1)
     rin = ru.view(nn, 3)[inner].abs().max()
     assert float(rin) <= 1e-12 * float(mu.view(nn, 3)[inner].max())
+
+
+def test_nonlocal_term_closed_form(big):
+    """GradientDamage's nonlocal term at u = 0 and a linear eeq = a + b.x below kappa0 (omega = 0):
+    e.R_e = e.K_ee.e = int eeq^2 + c int |grad eeq|^2 over the 100 mm cube, and the rows of K_ee sum
+    to int N_a = (incident cells) h^3 / 8 (the c-part of each row cancels). R_u and K_ue vanish.
+    Tolerances: 1e-11 relative on the sums of 1e6 cell terms; row sums to 1e-12 of sum |K_ij|."""
+    ctx, nn, nnz, n_rows, rp, col, Xd, nc = big
+    c = 4.0
+    a, b = 2.5e-5, torch.tensor([3e-7, -2e-7, 4e-7], dtype=torch.float64, device="cuda")
+    e = (a + Xd @ b).contiguous()
+    assert float(e.max()) < 1e-4
+    u = torch.zeros(3 * nn, dtype=torch.float64, device="cuda")
+    kap = torch.zeros(nc * 8, dtype=torch.float64, device="cuda")
+    R, K = ctx.build_vector_matrix(u, e, kap)
+    L = 100.0
+    V = L ** 3
+    beta = (L * b).cpu().numpy()
+    S, Q = beta.sum(), (beta ** 2).sum()
+    energy = V * (a * a + a * S + S * S / 4 + Q / 12) + c * V * float(b @ b)
+    assert float(R[:3 * nn].abs().max()) == 0.0
+    eR = float(e @ R[3 * nn:])
+    assert abs(eR - energy) <= 1e-11 * energy, (eR, energy)
+    # K_ee e and the row sums of K_ee over its e columns (columns >= 3 nn)
+    rows = torch.repeat_interleave(torch.arange(n_rows, device="cuda"), (rp[1:] - rp[:-1]).long())
+    ecol = col.long() >= 3 * nn
+    erow = rows >= 3 * nn
+    assert float(K[(~erow) & ecol].abs().max()) == 0.0  # K_ue
+    sel = erow & ecol
+    Kee, r, cc = K[sel], rows[sel] - 3 * nn, col.long()[sel] - 3 * nn
+    y = torch.zeros(nn, dtype=torch.float64, device="cuda").index_add_(0, r, Kee * e[cc])
+    assert abs(float(e @ y) - energy) <= 1e-11 * energy
+    rs = torch.zeros(nn, dtype=torch.float64, device="cuda").index_add_(0, r, Kee)
+    mag = torch.zeros(nn, dtype=torch.float64, device="cuda").index_add_(0, r, Kee.abs())
+    h = L / n
+    inc = torch.ones(nn, dtype=torch.float64, device="cuda")
+    for d in range(3):
+        on_face = (Xd[:, d] < 0.5 * h) | (Xd[:, d] > L - 0.5 * h)
+        inc = inc * torch.where(on_face, 1.0, 2.0)
+    lumped = inc * h ** 3 / 8
+    assert bool(((rs - lumped).abs() <= 1e-12 * mag).all())

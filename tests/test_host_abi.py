@@ -62,3 +62,70 @@ def test_tet_split_is_conforming_and_positive():
     faces = np.sort(np.concatenate([C[:, [0, 1, 2]], C[:, [0, 1, 3]], C[:, [0, 2, 3]], C[:, [1, 2, 3]]]), axis=1)
     _, cnt = np.unique(faces, axis=0, return_counts=True)
     assert cnt.max() == 2 and (cnt == 1).sum() == 6 * 9 * 2
+
+
+# ---- the binding refuses tensors the C ABI would overrun (checked before any library call) ------
+def _ctx_without_device(n_u=30, n_e=10, n_cells=5, n_ip=8, nnz=100):
+    """A Context whose sizes are set as after set_cells / set_dof_type / set_rule / build_pattern,
+    with no device handle: every refusal below must happen before the C ABI is reached."""
+    ctx = object.__new__(N.Context)
+    ctx._init_sizes()
+    ctx.h = None
+    ctx._n_dofs = {0: n_u, 1: n_e}
+    ctx.n_rows, ctx.nnz, ctx.n_cells, ctx.n_ip = n_u + n_e, nnz, n_cells, n_ip
+    ctx.n_dofs0 = n_u
+    ctx.dim = 3
+    return ctx
+
+
+def test_binding_refuses_wrong_dtype_and_short_tensors():
+    import torch
+
+    ctx = _ctx_without_device()
+    f64 = lambda n: torch.zeros(n, dtype=torch.float64)  # noqa: E731
+    u, e, kap, R, K = f64(30), f64(10), f64(40), f64(40), f64(100)
+    cases = [
+        (lambda: ctx.build_vector(u.float(), e, kap, out=R), "expected float64"),
+        (lambda: ctx.build_vector(f64(29), e, kap, out=R), "needs 30"),
+        (lambda: ctx.build_vector(u, f64(9), kap, out=R), "needs 10"),
+        (lambda: ctx.build_vector(u, e, f64(39), out=R), "needs 40"),
+        (lambda: ctx.build_vector(u, e, kap, out=f64(39)), "needs 40"),
+        (lambda: ctx.build_matrix(u, e, kap, out=f64(99)), "needs 100"),
+        (lambda: ctx.build_vector_matrix(u, e, kap, R=R, K=K.float()), "expected float64"),
+        (lambda: ctx.vjp(u, e, kap, f64(39)), "needs 40"),
+        (lambda: ctx.update_history(u, e, f64(7)), "needs 40"),
+        (lambda: ctx.build_lumped_mass(1.0, out=f64(29)), "needs 30"),
+        (lambda: ctx.solve(f64(99), f64(40), out=f64(40)), "needs 100"),
+        (lambda: ctx.eval_ip(0, u, e, kap, out=f64(5 * 8 * 6 - 1)), "needs 240"),
+        (lambda: ctx.quasistatic_step(f64(39), kap), "needs 40"),
+    ]
+    for call, msg in cases:
+        with pytest.raises(ValueError, match=msg):
+            call()
+    # a wider history (set_history_ld) needs a longer kappa
+    ctx._history_ld = 7
+    with pytest.raises(ValueError, match="needs 56"):
+        ctx.build_vector(u, e, kap, out=R)
+    ctx._history_ld = None
+    # correct dtypes and lengths get as far as the device check
+    with pytest.raises(ValueError, match="device tensor"):
+        ctx.build_vector_matrix(u, e, kap, R=R, K=K)
+
+
+def test_batch_binding_refuses_short_tensors():
+    import torch
+
+    b = object.__new__(N.Batch)
+    b.h = None
+    b.sizes = dict(u=30, e=10, R=40, K=100, history=16, theta=6)
+    f64 = lambda n: torch.zeros(n, dtype=torch.float64)  # noqa: E731
+    u, e, kap = f64(30), f64(10), f64(16)
+    with pytest.raises(ValueError, match="needs 40"):
+        b.residual_adjoint(u, e, kap, f64(39), R=f64(40), K=f64(100), d_state=f64(40), d_theta=f64(6))
+    with pytest.raises(ValueError, match="needs 6"):
+        b.residual_adjoint(u, e, kap, f64(40), R=f64(40), K=f64(100), d_state=f64(40), d_theta=f64(5))
+    with pytest.raises(ValueError, match="needs 16"):
+        b.assemble(u, e, f64(15), R=f64(40), K=f64(100))
+    b.sizes = None
+    with pytest.raises(ValueError, match="build"):
+        b.assemble(u, e, kap, R=f64(40), K=f64(100))

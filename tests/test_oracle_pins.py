@@ -333,6 +333,163 @@ def test_decreasing_interaction_is_constant_while_undamaged():
     assert np.abs(damaged[1] - damaged[0]).max() > 1e-6 * np.abs(damaged[0]).max()
 
 
+# ---------------------------------------------------------------------------------------------
+# the nonlocal term of GradientDamage (GradientDamage.h:85-87, 112-114) in closed form
+#
+# On a mesh of affine cells (a sheared, unequal-sided parallelepiped) at u = 0 with a linear
+# nonlocal strain eeq(x) = a + b.x below kappa0: eps = 0 and eps_eq(0) = 0, omega = omega' = 0, so
+#   R_e[a] = int N_a eeq + c g grad N_a . grad eeq,     K_ee = int N_a N_b + c g grad N_a . grad N_b,
+# and with the interpolated field (linear fields are reproduced exactly)
+#   e.R_e = e.K_ee.e = int eeq^2 + c g int |grad eeq|^2 = V [alpha^2 + alpha S + S^2/4 + Q/12] + c g V |b|^2
+# (alpha = a + b.t, beta = A^T b, S = sum beta, Q = sum beta^2: the moments of the unit cube under
+# x = A xi + t), both exact under 2x2x2 Gauss (degree <= 2 per direction). Rows of K_ee sum to
+# int N_a (sum_b N_b = 1, sum_b grad N_b = 0): the c-part of each row cancels. A consistent error
+# in c (c^2, -c, a dropped factor) leaves R and K mutually consistent, so the finite-difference
+# check above passes it; these closed forms do not.
+# ---------------------------------------------------------------------------------------------
+_A = np.array([[1.0, 0.25, -0.1], [0.0, 1.4, 0.3], [0.0, 0.0, 0.8]])
+_T = np.array([0.5, -1.0, 2.0])
+
+
+def _affine_gdm(n, c, interaction=0, scale=1.0):
+    m0 = O.Mesh.bricks(n, n, n)  # the unit cube
+    X = m0.coords() @ (scale * _A).T + _T
+    m = O.Mesh.from_arrays(O.HEX8, X, m0.conn())
+    Ld, Le = m.add_dof(O.HEX8, 3), m.add_dof(O.HEX8, 1)
+    p = O.Params(kind=1, dim=3, E=30000.0, nu=0.2, k=10.0, kappa0=1e-4, beta=100.0, alpha=0.99, c=c,
+                 interaction=interaction, interaction_r=0.005, interaction_eta=5.0)
+    P = O.Problem(m, p, 0, 2, Ld, Le)
+    # coordinates of the dof nodes: a cell's local node j is coordinate node cconn[c, j] and dof
+    # node dconn[c, j]
+    lay = m.dof_layer(Le)
+    xd = np.zeros((lay["n_nodes"], 3))
+    xd[lay["numbers"][lay["conn"].ravel(), 0]] = X[m0.conn().ravel()]
+    incident = np.bincount(lay["numbers"][lay["conn"].ravel(), 0], minlength=lay["n_nodes"])
+    return P, xd, incident, abs(np.linalg.det(scale * _A))
+
+
+def _linear_energy(V, a, b, A, t, c, g):
+    alpha = a + b @ t
+    beta = A.T @ b
+    S, Q = beta.sum(), (beta ** 2).sum()
+    return V * (alpha ** 2 + alpha * S + S * S / 4 + Q / 12), c * g * V * (b @ b)
+
+
+def _dense(out, n):
+    return O.csr_to_dense(out["row_ptr"], out["col"], out["val"], n)
+
+
+def _nonlocal_closed_form_errors(c, mutation=0, interaction=0, kappa_hist=0.0, g=None):
+    """Relative deviations of e.R_e, e.K_ee.e and the K_ee row sums from their closed forms (g: the
+    interaction factor the closed form assumes; default: the oracle's factor at the history's omega)."""
+    n = 3
+    P, xd, incident, V = _affine_gdm(n, c, interaction)
+    nd, ne = P.ndof
+    a, b = 1e-5, np.array([1.5e-5, -0.8e-5, 1.1e-5])
+    e = a + xd @ b
+    assert e.max() < 1e-4 and e.min() >= 0.0  # below kappa0
+    u = np.zeros(nd)
+    kap = np.full(P.ncells * 8, kappa_hist)
+    O.set_mutation(mutation)
+    try:
+        out = P.assemble(u, e, kap)
+    finally:
+        O.set_mutation(0)
+    if g is None:
+        g = 1.0
+        if kappa_hist > 0.0:
+            omega = O.damage_exp(1e-4, 100.0, 0.99, kappa_hist)[0]
+            g = O.interaction(interaction, 0.005, 5.0, omega)[0]
+    e2, grad = _linear_energy(V, a, b, _A, _T, c, g)
+    K = _dense(out, nd + ne)
+    Kee = K[nd:, nd:]
+    eR = e @ out["R"][nd:]
+    eKe = e @ Kee @ e
+    rows = Kee.sum(axis=1)
+    lumped = incident * (V / n ** 3) / 8.0
+    return (abs(eR - (e2 + grad)) / (e2 + grad), abs(eKe - (e2 + grad)) / (e2 + grad),
+            np.abs(rows - lumped).max() / lumped.max(), out, K, nd, xd)
+
+
+@pytest.mark.parametrize("c", [4.0, 1.0, 0.25])
+def test_gdm_nonlocal_term_closed_form(c):
+    eR, eKe, rows, out, K, nd, xd = _nonlocal_closed_form_errors(c)
+    assert eR < 1e-12 and eKe < 1e-12, (eR, eKe)
+    assert rows < 1e-12, rows
+    # eps = 0: no stress, no residual on the displacement rows; omega' = 0: K_ue vanishes
+    assert np.abs(out["R"][:nd]).max() == 0.0
+    assert np.abs(K[:nd, nd:]).max() == 0.0
+
+
+def test_gdm_eu_block_closed_form():
+    """K_eu = -int N_a (d eps_eq / d eps) B_b (GradientDamage.h:105-107); at eps = 0 the modified
+    Mises derivative is K1 [1, 1, 1, 0, 0, 0] (its A == 0 branch, ModifiedMisesStrainNorm.h:242-250),
+    so for an affine displacement u = G x: K_eu u = -K1 tr(G) int N_a, K1 = (k - 1) / (2 k (1 - 2 nu))."""
+    _, _, _, out, K, nd, xd = _nonlocal_closed_form_errors(4.0)
+    G = np.array([[1.0, 0.2, -0.3], [0.4, -2.0, 0.1], [0.0, 0.5, 3.0]])
+    ulin = (xd @ G.T).ravel()
+    k, nu = 10.0, 0.2
+    K1 = (k - 1) / (2 * k * (1 - 2 * nu))
+    P, xd2, incident, V = _affine_gdm(3, 4.0)
+    lumped = incident * (V / 27) / 8.0
+    got = K[nd:, :nd] @ ulin
+    want = -K1 * np.trace(G) * lumped
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * np.abs(K[nd:, :nd]).sum(axis=1).max() * np.abs(ulin).max())
+
+
+@pytest.mark.parametrize("mutation", [7, 8])
+def test_consistent_nonlocal_mutations_fail_the_closed_form(mutation):
+    """c -> c^2 (7) and c -> -c (8), planted in R_e and K_ee alike: the closed forms reject them,
+    while the finite-difference Hessian check (consistency of K with R) still passes."""
+    eR, eKe, rows, *_ = _nonlocal_closed_form_errors(4.0, mutation)
+    assert eR > 1e-3 and eKe > 1e-3
+    m, P = _gdm_problem(1, 1.0)
+    nd, ne = P.ndof
+    rng = np.random.default_rng(2)
+    u, e, kap = rng.uniform(-5e-4, 5e-4, nd), rng.uniform(2e-4, 5e-4, ne), np.full(8, 1e-4)
+    O.set_mutation(mutation)
+    try:
+        _, Kc = P.cell(0, u, e, kap)
+        x0 = np.concatenate([u, e])
+        j = nd + 3
+        xp, xm = x0.copy(), x0.copy()
+        xp[j] += 1e-9
+        xm[j] -= 1e-9
+        fd = (P.cell(0, xp[:nd], xp[nd:], kap, want_k=False)[0] - P.cell(0, xm[:nd], xm[nd:], kap, want_k=False)[0]) / 2e-9
+    finally:
+        O.set_mutation(0)
+    np.testing.assert_allclose(Kc[:, j], fd, rtol=2e-5, atol=2e-5 * np.abs(Kc).max())
+
+
+def test_decreasing_interaction_endpoints_and_derivative():
+    """Decreasing::Factor (GradientDamage.h:170-184): g(0) = 1, g(1) = R, g decreasing in omega, and
+    Derivative is the derivative of Factor (central differences)."""
+    R, eta = 0.005, 5.0
+    assert O.interaction(1, R, eta, 0.0)[0] == pytest.approx(1.0, abs=1e-15)
+    assert O.interaction(1, R, eta, 1.0)[0] == pytest.approx(R, abs=1e-15)
+    w = np.linspace(0.0, 1.0, 11)
+    g = np.array([O.interaction(1, R, eta, x)[0] for x in w])
+    assert (np.diff(g) < 0).all()
+    for x in (0.1, 0.5, 0.9):
+        h = 1e-6
+        fd = (O.interaction(1, R, eta, x + h)[0] - O.interaction(1, R, eta, x - h)[0]) / (2 * h)
+        assert O.interaction(1, R, eta, x)[1] == pytest.approx(fd, rel=1e-8)
+    assert O.interaction(0, R, eta, 0.7) == (1.0, 0.0)
+
+
+def test_decreasing_interaction_scales_the_nonlocal_term():
+    """With a damaged history kappa_hist > kappa0 above eeq everywhere, omega = omega(kappa_hist) is
+    uniform and d kappa / d eeq = 0, so e.R_e = int eeq^2 + c g(omega) int |grad eeq|^2 and K_ee has
+    no g' term: the Decreasing factor multiplies exactly the gradient term."""
+    eR, eKe, rows, *_ = _nonlocal_closed_form_errors(4.0, interaction=1, kappa_hist=3e-4)
+    assert eR < 1e-12 and eKe < 1e-12 and rows < 1e-12
+    # Constant keeps g = 1 at the same damaged state; Decreasing does not
+    eRc, *_ = _nonlocal_closed_form_errors(4.0, interaction=0, kappa_hist=3e-4)
+    assert eRc < 1e-12
+    eRw, eKw, *_ = _nonlocal_closed_form_errors(4.0, interaction=1, kappa_hist=3e-4, g=1.0)
+    assert eRw > 1e-3 and eKw > 1e-3
+
+
 def test_local_damage_tangent_fd():
     m = O.Mesh.bricks(1, 1, 1)
     L = m.add_dof(O.HEX8, 3)

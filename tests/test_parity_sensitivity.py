@@ -1,5 +1,6 @@
 """The parity rule is not vacuous: plausible mistakes planted in the oracle (a dropped term, a
-flipped sign, a wrong shear factor, a dropped history max, a transposed operand) change R or K by
+flipped sign, a wrong shear factor, a dropped history max, a transposed operand, the nonlocal length
+squared or negated in R and K alike) change R or K by
 more than 1e-12 of the contribution magnitudes on the committed 3x3x3 gradient-damage fixture, so
 the same rule applied to the device path would reject each of them.
 """
@@ -46,7 +47,7 @@ def test_unmutated_oracle_passes_its_own_rule(fix):
     assert not violates(a, b)
 
 
-@pytest.mark.parametrize("mutation", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("mutation", [1, 2, 3, 4, 5, 7, 8])
 def test_planted_mistake_is_rejected(fix, mutation):
     ref = assemble(fix, 0)
     assert violates(ref, assemble(fix, mutation)), f"mutation {mutation} passed the parity rule"
