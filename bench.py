@@ -315,24 +315,29 @@ def batched_adjoint(n_meshes, steps, warmup):
             "meshes": len(meshes), "meshes_dropped_for_memory": dropped}
 
 
-def cpu_baseline(n, threads, budget_s=30.0):
+def cpu_baseline(n, threads, n_one_core=24, budget_s=30.0):
     """Times the oracle (NuTo's algorithm restated: per-thread triplet lists, critical splice,
-    setFromTriplets) on an n^3 sample of the same workload."""
+    setFromTriplets) on an n^3 sample of the same workload with `threads` OpenMP threads (R+K, and
+    BuildVector and BuildMatrix alone), and on an n_one_core^3 sample with one thread (cells/s is
+    intensive in the mesh size)."""
     import numpy as np
 
     import oracle as O
     from nuto_amd import synthetic as S
 
-    m = O.Mesh.bricks(n, n, n)
-    m.scale(float(n))  # 1 mm spacing, as the 100 mm / 100 cells workload
-    Ld = m.add_dof(O.HEX8, 3)
-    Le = m.add_dof(O.HEX8, 1)
-    p = O.Params(kind=1, dim=3, E=30000.0, nu=0.2, k=10.0, kappa0=1e-4, beta=100.0, alpha=0.99, c=4.0)
-    P = O.Problem(m, p, 0, 2, Ld, Le)
-    u = S.uniform(S.SEED_U, P.ndof[0], -5e-3, 5e-3)
-    e = S.uniform(S.SEED_E, P.ndof[1], 0.0, 5e-4)
-    k = S.uniform(S.SEED_KAPPA, P.ncells * 8, 0.0, 3e-4)
-    def median_s(want_r, want_k, nth, runs=5, budget=budget_s):
+    def problem(nc):
+        m = O.Mesh.bricks(nc, nc, nc)
+        m.scale(float(nc))  # 1 mm spacing, as the 100 mm / 100 cells workload
+        Ld = m.add_dof(O.HEX8, 3)
+        Le = m.add_dof(O.HEX8, 1)
+        p = O.Params(kind=1, dim=3, E=30000.0, nu=0.2, k=10.0, kappa0=1e-4, beta=100.0, alpha=0.99, c=4.0)
+        P = O.Problem(m, p, 0, 2, Ld, Le)
+        u = S.uniform(S.SEED_U, P.ndof[0], -5e-3, 5e-3)
+        e = S.uniform(S.SEED_E, P.ndof[1], 0.0, 5e-4)
+        k = S.uniform(S.SEED_KAPPA, P.ncells * 8, 0.0, 3e-4)
+        return m, P, u, e, k
+
+    def median_s(P, u, e, k, want_r, want_k, nth, runs, budget=budget_s):
         times = []
         t_start = time.time()
         while len(times) < runs and (time.time() - t_start) < budget:
@@ -341,27 +346,43 @@ def cpu_baseline(n, threads, budget_s=30.0):
             times.append(time.perf_counter() - t0)
         return float(np.median(times)), len(times)
 
-    t, runs = median_s(True, True, threads)
-    # BuildVector and BuildMatrix separately (the reference times them as two calls), and one core
-    tv, _ = median_s(True, False, threads, 3, budget_s / 4)
-    tm, _ = median_s(False, True, threads, 3, budget_s / 4)
-    t1, _ = median_s(True, True, 1, 1, budget_s)
-    return {"value": P.ncells / t, "unit": "cells/s", "cores": threads, "kind": "port",
-            "sample": f"hex8 gradient damage {n}x{n}x{n} cells (1 mm spacing), R+K, median of {runs} runs "
-                      f"of the oracle with {threads} OpenMP threads",
-            "build_vector_cells_per_s": P.ncells / tv, "build_matrix_cells_per_s": P.ncells / tm,
-            "one_core_cells_per_s": P.ncells / t1, "cpu_model": _cpu_model()}
+    m, P, u, e, k = problem(n)
+    P.assemble(u, e, k, want_r=True, want_k=True, nthreads=threads)  # warm-up (page faults, thread start)
+    t, runs = median_s(P, u, e, k, True, True, threads, 3)
+    # BuildVector and BuildMatrix separately (the reference times them as two calls)
+    tv, _ = median_s(P, u, e, k, True, False, threads, 3, budget_s / 3)
+    tm, _ = median_s(P, u, e, k, False, True, threads, 1, budget_s / 3)
+    cells = P.ncells
+    del P, m
+    _, P1, u1, e1, k1 = problem(n_one_core)
+    t1, _ = median_s(P1, u1, e1, k1, True, True, 1, 1)
+    return {"value": cells / t, "unit": "cells/s", "cores": threads, "kind": "port",
+            "sample": f"hex8 gradient damage {n}x{n}x{n} cells (1 mm spacing, {cells} cells), R+K, median of "
+                      f"{runs} runs of the oracle with {threads} OpenMP threads (triplet lists, critical splice, "
+                      f"setFromTriplets); one core: {n_one_core}x{n_one_core}x{n_one_core} cells",
+            "build_vector_cells_per_s": cells / tv, "build_matrix_cells_per_s": cells / tm,
+            "one_core_cells_per_s": n_one_core ** 3 / t1, "host": _host_cpu()}
 
 
-def _cpu_model():
+def _host_cpu():
+    """lscpu's model, sockets and cores, and the CPUs this process may use."""
+    info = {}
     try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
-    except OSError:
+        import subprocess
+
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            key, _, val = line.partition(":")
+            if key.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info[key.strip()] = val.strip()
+    except Exception:  # noqa: BLE001 - reported, never fatal
         pass
-    return "unknown"
+    try:
+        info["usable_cpus"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        info["usable_cpus"] = os.cpu_count()
+    info["OMP_NUM_THREADS"] = os.environ.get("OMP_NUM_THREADS")
+    return info
 
 
 def main():
@@ -370,8 +391,9 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=100, help="cells per direction (100 = the headline workload)")
-    ap.add_argument("--cpu-sample", type=int, default=24, help="cells per direction of the CPU baseline sample")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="OpenMP threads for the CPU baseline (0: auto)")
+    ap.add_argument("--cpu-sample", type=int, default=50, help="cells per direction of the CPU baseline sample")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="OpenMP threads for the CPU baseline (0: OMP_NUM_THREADS, else the usable CPUs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--pcie", action="store_true",
                     help="also time the step with host-resident inputs and outputs (pinned copies in and out)")
@@ -492,9 +514,15 @@ def main():
         rec["batched_adjoint"] = batched_adjoint(args.batch_gdm_meshes, min(args.steps, 10), args.warmup)
         torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        # all the cores the process is given (the GPU host allots a share per GPU: OMP_NUM_THREADS)
+        omp = os.environ.get("OMP_NUM_THREADS", "")
+        threads = args.cpu_threads or (int(omp) if omp.isdigit() and int(omp) > 0 else _host_cpu()["usable_cpus"])
         try:
-            rec["cpu_baseline"] = cpu_baseline(args.cpu_sample, threads)
+            cb = cpu_baseline(args.cpu_sample, threads)
+            # context only: vs_baseline stays null (no published number exists for this metric)
+            cb["gpu_over_cpu"] = value / cb["value"]
+            cb["gpu_over_one_core"] = value / cb["one_core_cells_per_s"]
+            rec["cpu_baseline"] = cb
         except Exception as ex:  # the baseline is reported, never the measured value
             rec["cpu_baseline"] = {"error": str(ex)}
     print(json.dumps(rec), flush=True)

@@ -133,6 +133,20 @@ int nuto_build_pattern(nuto_ctx* ctx, long long* nnz, int* n_rows);
 int nuto_copy_pattern(nuto_ctx* ctx, int* row_ptr, int* col);
 int nuto_pattern_device(nuto_ctx* ctx, const int** row_ptr, const int** col);
 
+/* Block (slot_i, slot_j) of the matrix: K(dofI, dofJ) of NuTo's DofMatrixSparse, a map of one sparse
+ * matrix per dof type pair (DofMatrixContainer.h:16-24; BuildMatrix fills it per (dofI, dofJ),
+ * SimpleAssembler.cpp:105-155), whose monolithic export is ToEigen (DofMatrixSparseConvertEigen.h:
+ * 40-67). The block is CSR over the rows of slot_i (dof numbers of that type), columns numbered
+ * within slot_j. In every row the block's entries are one run of the monolithic row (slot 0's
+ * columns come before slot 1's): block entry k of row r is K[run_begin[r] + k - row_ptr[r]].
+ * nnz: the block's nonzeros; row_ptr (rows of slot_i + 1), col (nnz) and run_begin (rows of slot_i):
+ * host arrays, each may be NULL. After nuto_build_pattern. */
+int nuto_block_pattern(nuto_ctx* ctx, int slot_i, int slot_j, long long* nnz, int* row_ptr, int* col,
+                       long long* run_begin);
+/* The values of block (slot_i, slot_j) gathered from K (as nuto_build_matrix writes it) into
+ * `values` in the block's CSR order; device arrays, values: the block's nonzeros. */
+int nuto_block_values(nuto_ctx* ctx, int slot_i, int slot_j, const double* K, double* values);
+
 /* SimpleAssembler::BuildVector (SimpleAssembler.cpp:38-65): R (length n_rows) from the dof vectors
  * u (slot 0) and e (slot 1, NULL for single-field integrands) and history kappa (NULL if unused). */
 int nuto_build_vector(nuto_ctx* ctx, const double* u, const double* e, const double* kappa, double* R);

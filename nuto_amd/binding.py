@@ -146,6 +146,8 @@ def lib():
         L.nuto_batch_residual_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.nuto_build_pattern.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ip]
         L.nuto_copy_pattern.argtypes = [vp, vp, vp]
+        L.nuto_block_pattern.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), vp, vp, vp]
+        L.nuto_block_values.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp]
         L.nuto_pattern_device.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp)]
         for f in ("nuto_build_vector", "nuto_build_matrix"):
             getattr(L, f).argtypes = [vp, vp, vp, vp, vp]
@@ -544,6 +546,30 @@ class Context:
         ci = np.zeros(self.nnz, np.int32)
         self._chk(lib().nuto_copy_pattern(self.h, _np_ptr(rp), _np_ptr(ci)))
         return rp, ci
+
+    def block_pattern(self, slot_i, slot_j):
+        """Block (slot_i, slot_j) of the matrix, K(dofI, dofJ) of NuTo's DofMatrixSparse: (row_ptr, col,
+        run_begin) as numpy arrays; columns numbered within slot_j; run_begin[r] is the block row's first
+        entry in the monolithic values (nuto_block_pattern)."""
+        nnz = ctypes.c_longlong()
+        self._chk(lib().nuto_block_pattern(self.h, slot_i, slot_j, ctypes.byref(nnz), None, None, None))
+        rows = self._n_dofs.get(slot_i, 0)
+        rp = np.zeros(rows + 1, np.int32)
+        ci = np.zeros(nnz.value, np.int32)
+        rb = np.zeros(rows, np.int64)
+        self._chk(lib().nuto_block_pattern(self.h, slot_i, slot_j, ctypes.byref(nnz), _np_ptr(rp), _np_ptr(ci),
+                                           _np_ptr(rb)))
+        return rp, ci, rb
+
+    def block_values(self, slot_i, slot_j, K, out=None):
+        """The values of block (slot_i, slot_j) of K (a device tensor as build_matrix returns it), in the
+        block's CSR order (nuto_block_values)."""
+        nnz = ctypes.c_longlong()
+        self._chk(lib().nuto_block_pattern(self.h, slot_i, slot_j, ctypes.byref(nnz), None, None, None))
+        kp = _f64(K, self.nnz, "K", optional=False)
+        out = self._new(nnz.value) if out is None else _f64(out, nnz.value, "out")
+        self._chk(lib().nuto_block_values(self.h, slot_i, slot_j, *_ptrs(kp, out)))
+        return out
 
     def pattern_device(self):
         """(row_ptr, col) as torch int32 tensors on the device (copied through the host)."""

@@ -347,8 +347,11 @@ struct NodeFetch
     double kh;  // history kappa of cell r at IP j
 };
 
-// GEO0: the cell's geometry record (IP 0's) in every lane of the group (affine cells)
-template <bool PRE, bool GEO0 = false>
+// GEO0: the cell's geometry record (IP 0's) in every lane of the group (affine cells).
+// CLAMP: lanes past the node's last incident cell (f.cell < 0) load cell 0's records instead of
+// skipping the loads, so that every lane runs the same straight-line code (their results are
+// never used: the slot map of the reduction leaves absent cells out).
+template <bool PRE, bool GEO0 = false, bool CLAMP = false>
 __device__ inline void FetchNode(const RowArgs& A, const GdmHex8Tables& T, int pos, int r, int j, NodeFetch& f)
 {
     const size_t pr = (size_t)pos * 8 + r;
@@ -357,10 +360,11 @@ __device__ inline void FetchNode(const RowArgs& A, const GdmHex8Tables& T, int p
     f.slot = T.nodeSlot[pr * 8 + j];
     f.kh = 0.0;
     f.q = 0;
-    if (f.cell >= 0)
+    if (CLAMP || f.cell >= 0)
     {
+        const int cl = CLAMP && f.cell < 0 ? 0 : f.cell;
         const double2* gr = reinterpret_cast<const double2*>(
-            GEO0 ? T.geoCell + (size_t)f.cell * gdm8::kGeo : T.geo + ((size_t)f.cell * 8 + j) * gdm8::kGeo);
+            GEO0 ? T.geoCell + (size_t)cl * gdm8::kGeo : T.geo + ((size_t)cl * 8 + j) * gdm8::kGeo);
 #pragma unroll
         for (int k = 0; k < gdm8::kGeo / 2; k++)
         {
@@ -370,7 +374,7 @@ __device__ inline void FetchNode(const RowArgs& A, const GdmHex8Tables& T, int p
         }
         if constexpr (PRE)
         {
-            const double2* sp = reinterpret_cast<const double2*>(T.ipState + ((size_t)f.cell * 8 + j) * gdm8::kSr);
+            const double2* sp = reinterpret_cast<const double2*>(T.ipState + ((size_t)cl * 8 + j) * gdm8::kSr);
 #pragma unroll
             for (int k = 0; k < gdm8::kSr / 2; k++)
             {
@@ -392,9 +396,9 @@ __device__ inline void FetchNode(const RowArgs& A, const GdmHex8Tables& T, int p
         }
         if (A.thE)
         {
-            f.th[0] = A.thE[f.cell];
-            f.th[1] = A.thFt[f.cell];
-            f.th[2] = A.thGf[f.cell];
+            f.th[0] = A.thE[cl];
+            f.th[1] = A.thFt[cl];
+            f.th[2] = A.thGf[cl];
         }
     }
 }
@@ -665,7 +669,7 @@ __device__ inline void NodeBody(const RowArgs& A, const GdmHex8Tables& T, int po
     const int p = head.x, ru0 = head.y, L = head.z, re0 = head.w;
     const int nA = L >> 2;
     NodeFetch f;
-    FetchNode<PRE, AFF>(A, T, pos, r, j, f);
+    FetchNode<PRE, AFF, AFF>(A, T, pos, r, j, f);
     const bool valid = f.cell >= 0;  // r < number of cells incident to p
     const int la = f.la;
 
@@ -686,20 +690,19 @@ __device__ inline void NodeBody(const RowArgs& A, const GdmHex8Tables& T, int po
         acc[i] = 0.0;
     if constexpr (AFF)
     {
+        // every lane, absent cells included (their records and blocks are never read; their R terms
+        // are zeroed)
         double rec[kSt];
-        if (valid)
-            PreRecord<FULLDMG, kSt, false>(f.sr, f.g, la, j, P, rec, Rv);
+        PreRecord<FULLDMG, kSt, false>(f.sr, f.g, la, j, P, rec, Rv);
+        if (!valid)
+            Rv[0] = Rv[1] = Rv[2] = Rv[3] = 0.0;
         if (WK)
         {
-            if (valid)
-                StoreRecord<kSt>(state + (r * 8 + j) * kSt, rec);
+            StoreRecord<kSt>(state + (r * 8 + j) * kSt, rec);
             __syncthreads();
-            if (valid)
-            {
-                // lane (r, b) works on cell r, whose J^-1 the group holds
-                const double inv[3][3] = {{f.g[0], f.g[1], f.g[2]}, {f.g[3], f.g[4], f.g[5]}, {f.g[6], f.g[7], f.g[8]}};
-                PhaseBAffine<FULLDMG, kSt>(state, r, j, inv, acc);
-            }
+            // lane (r, b) works on cell r, whose J^-1 the group holds
+            const double inv[3][3] = {{f.g[0], f.g[1], f.g[2]}, {f.g[3], f.g[4], f.g[5]}, {f.g[6], f.g[7], f.g[8]}};
+            PhaseBAffine<FULLDMG, kSt>(state, r, j, inv, acc);
         }
     }
     else if constexpr (PRE)
@@ -804,7 +807,7 @@ __device__ inline void NodeBody(const RowArgs& A, const GdmHex8Tables& T, int po
         if (WK && valid)
             PhaseB<FULLDMG, kSt>(state, r, j, 0, 8, acc);
     }
-    if (WK && valid && FULLDMG)
+    if (WK && (valid || AFF) && FULLDMG)
     {
         // uu = lambda' S + mu' S^T + mu' tr(S) I
         const double lam = (m3K - m2G) * kThird, mu = 0.5 * m2G;

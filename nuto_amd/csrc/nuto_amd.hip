@@ -84,6 +84,16 @@ struct nuto_ctx
     bool consBuilt = false;
     int nInd = 0, nEq = 0;
     DevBuf cRow, cCol, cVal, ctRow, ctCol, ctVal, consDep;
+    // per-block views of the monolithic matrix (nuto_block_pattern / nuto_block_values), built on
+    // first use per (slot i, slot j): the block's CSR row pointers and, per row, where the block's
+    // run starts in the monolithic row
+    struct BlockLayout
+    {
+        bool valid = false;
+        int rows = 0, row0 = 0;
+        long long nnz = 0;
+        DevBuf rowPtr, runBegin;
+    } blocks[2][2];
     // staged cell matrices and vectors of the two-phase generic assembly
     DevBuf keStage, reStage;
     int maxAdj = -1;  // largest node neighbourhood (LDS rows of the wave gather), from the node graph
@@ -169,6 +179,13 @@ void Upload(nuto_ctx* ctx, DevBuf& b, const T* src, size_t n, const char* what)
 void InvalidatePattern(nuto_ctx* ctx)
 {
     ctx->patternBuilt = false;
+    for (auto& row : ctx->blocks)
+        for (auto& b : row)
+        {
+            b.valid = false;
+            b.rowPtr.Release();
+            b.runBegin.Release();
+        }
     ctx->gdm.Release();
     ctx->gdmTried = false;
     ctx->consBuilt = false;
@@ -493,6 +510,78 @@ void PrepareGdm(nuto_ctx* ctx)
                      ctx->rowPtr.As<int>(), ctx->nRows, ctx->stream, err);
     Check(err, "hex8 node tables");
     ctx->gdmTried = true;
+}
+
+// Block (slot i, slot j) of the monolithic matrix, row by row: the monolithic row of dof row0 + r
+// holds slot 0's columns [0, colSplit) before slot 1's (columns ascending), so the block's entries
+// are one run. count[r] = its length, runBegin[r] = its first monolithic entry.
+__global__ void __launch_bounds__(256) kBlockRuns(const int* __restrict__ rowPtr, const int* __restrict__ col,
+                                                  int row0, int rows, int colSplit, int slotJ,
+                                                  int* __restrict__ count, long long* __restrict__ runBegin)
+{
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= rows)
+        return;
+    const int b = rowPtr[row0 + r], e = rowPtr[row0 + r + 1];
+    int lo = b, hi = e;  // first entry with col >= colSplit
+    while (lo < hi)
+    {
+        const int m = (lo + hi) >> 1;
+        if (col[m] < colSplit)
+            lo = m + 1;
+        else
+            hi = m;
+    }
+    count[r] = slotJ == 0 ? lo - b : e - lo;
+    runBegin[r] = slotJ == 0 ? b : lo;
+}
+
+// block columns (colShift subtracted) or values, one wave per block row
+template <class T>
+__global__ void __launch_bounds__(256) kBlockGather(const int* __restrict__ blockRowPtr,
+                                                    const long long* __restrict__ runBegin, int rows,
+                                                    const T* __restrict__ src, T shift, T* __restrict__ dst)
+{
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= rows)
+        return;
+    const int lane = threadIdx.x & 63;
+    const int b = blockRowPtr[r], n = blockRowPtr[r + 1] - b;
+    const T* s = src + runBegin[r];
+    for (int k = lane; k < n; k += 64)
+        dst[b + k] = s[k] - shift;
+}
+
+nuto_ctx::BlockLayout& BlockOf(nuto_ctx* ctx, int si, int sj)
+{
+    Require(ctx->patternBuilt, NUTO_ERR_STATE, "pattern not built");
+    Require(si >= 0 && si < 2 && sj >= 0 && sj < 2, NUTO_ERR_ARGUMENT, "dof type slots are 0 and 1");
+    Require(ctx->f[si].set && ctx->f[sj].set, NUTO_ERR_STATE, "no dof numbering for that slot");
+    nuto_ctx::BlockLayout& L = ctx->blocks[si][sj];
+    if (L.valid)
+        return L;
+    const int n0 = ctx->f[0].nDofs;
+    L.row0 = si == 0 ? 0 : n0;
+    L.rows = ctx->f[si].nDofs;
+    DevBuf cnt;
+    Check(cnt.Alloc((size_t)L.rows * sizeof(int)), "block counts");
+    Check(L.runBegin.Alloc((size_t)L.rows * sizeof(long long)), "block runs");
+    if (L.rows > 0)
+    {
+        hipLaunchKernelGGL(kBlockRuns, dim3((L.rows + 255) / 256), dim3(256), 0, ctx->stream, ctx->rowPtr.As<int>(),
+                           ctx->col.As<int>(), L.row0, L.rows, n0, sj, cnt.As<int>(), L.runBegin.As<long long>());
+        Check(hipGetLastError(), "block runs");
+    }
+    std::vector<int> c(L.rows), rp(L.rows + 1, 0);
+    if (L.rows > 0)
+        Check(hipMemcpyAsync(c.data(), cnt.p, c.size() * sizeof(int), hipMemcpyDeviceToHost, ctx->stream), "block counts");
+    Check(hipStreamSynchronize(ctx->stream), "block counts");
+    for (int r = 0; r < L.rows; r++)
+        rp[r + 1] = rp[r] + c[r];  // a block has at most the monolithic nonzeros (int32 CSR)
+    L.nnz = rp[L.rows];
+    Check(L.rowPtr.Upload(rp.data(), rp.size() * sizeof(int), ctx->stream), "block row pointers");
+    L.valid = true;
+    return L;
 }
 
 void Assemble(nuto_ctx* ctx, const double* u, const double* e, const double* kappa, double* R, double* K,
@@ -863,6 +952,50 @@ int nuto_copy_pattern(nuto_ctx* ctx, int* row_ptr, int* col)
                   "copy row_ptr");
         if (col)
             Check(hipMemcpy(col, ctx->col.p, (size_t)ctx->nnz * sizeof(int), hipMemcpyDeviceToHost), "copy col");
+    });
+}
+
+int nuto_block_pattern(nuto_ctx* ctx, int slot_i, int slot_j, long long* nnz, int* row_ptr, int* col,
+                       long long* run_begin)
+{
+    return Guard(ctx, [&] {
+        nuto_ctx::BlockLayout& L = BlockOf(ctx, slot_i, slot_j);
+        if (nnz)
+            *nnz = L.nnz;
+        if (row_ptr)
+            Check(hipMemcpy(row_ptr, L.rowPtr.p, (size_t)(L.rows + 1) * sizeof(int), hipMemcpyDeviceToHost),
+                  "copy block row_ptr");
+        if (run_begin && L.rows > 0)
+            Check(hipMemcpy(run_begin, L.runBegin.p, (size_t)L.rows * sizeof(long long), hipMemcpyDeviceToHost),
+                  "copy block runs");
+        if (col && L.nnz > 0)
+        {
+            DevBuf c;
+            Check(c.Alloc((size_t)L.nnz * sizeof(int)), "block columns");
+            const int shift = slot_j == 0 ? 0 : ctx->f[0].nDofs;
+            hipLaunchKernelGGL(kBlockGather<int>, dim3((L.rows + 3) / 4), dim3(256), 0, ctx->stream,
+                               L.rowPtr.As<int>(), L.runBegin.As<long long>(), L.rows, ctx->col.As<int>(), shift,
+                               c.As<int>());
+            Check(hipGetLastError(), "block columns");
+            Check(hipMemcpyAsync(col, c.p, (size_t)L.nnz * sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
+                  "copy block columns");
+            Check(hipStreamSynchronize(ctx->stream), "copy block columns");
+        }
+    });
+}
+
+int nuto_block_values(nuto_ctx* ctx, int slot_i, int slot_j, const double* K, double* values)
+{
+    return Guard(ctx, [&] {
+        Require(K != nullptr && values != nullptr, NUTO_ERR_ARGUMENT, "K or values is NULL");
+        nuto_ctx::BlockLayout& L = BlockOf(ctx, slot_i, slot_j);
+        if (L.rows > 0 && L.nnz > 0)
+        {
+            hipLaunchKernelGGL(kBlockGather<double>, dim3((L.rows + 3) / 4), dim3(256), 0, ctx->stream,
+                               L.rowPtr.As<int>(), L.runBegin.As<long long>(), L.rows, K, 0.0, values);
+            Check(hipGetLastError(), "block values");
+        }
+        SyncCheck(ctx, "block values");
     });
 }
 

@@ -170,11 +170,24 @@ void SimpleAssembler::Run(const std::vector<DofType>& dofs, const DofVector<doub
     }
     if (wantK)
     {
-        K->rowPtr.resize(mRows + 1);
-        K->col.resize(mNnz);
-        K->val.resize(mNnz);
-        Check(nuto_copy_pattern(mCtx, K->rowPtr.data(), K->col.data()), "BuildMatrix");
-        Check(nuto_copy_to_host(mCtx, K->val.data(), kv.p, mNnz * sizeof(double)), "BuildMatrix");
+        // BuildMatrix fills one sparse matrix per (dofI, dofJ) (SimpleAssembler.cpp:105-155): the
+        // blocks of the device's monolithic matrix
+        for (size_t si = 0; si < dofs.size(); si++)
+            for (size_t sj = 0; sj < dofs.size(); sj++)
+            {
+                long long nb = 0;
+                Check(nuto_block_pattern(mCtx, (int)si, (int)sj, &nb, nullptr, nullptr, nullptr), "BuildMatrix");
+                SparseMatrix<double>& B = (*K)(dofs[si], dofs[sj]);
+                B = SparseMatrix<double>(n[si], n[sj]);
+                B.col.resize(nb);
+                B.val.resize(nb);
+                Check(nuto_block_pattern(mCtx, (int)si, (int)sj, &nb, B.rowPtr.data(), B.col.data(), nullptr),
+                      "BuildMatrix");
+                DeviceArray bv(mCtx, (size_t)std::max(nb, 1LL) * sizeof(double));
+                Check(nuto_block_values(mCtx, (int)si, (int)sj, kv.d(), bv.d()), "BuildMatrix");
+                Check(nuto_synchronize(mCtx), "BuildMatrix");
+                Check(nuto_copy_to_host(mCtx, B.val.data(), bv.p, (size_t)nb * sizeof(double)), "BuildMatrix");
+            }
     }
 }
 
@@ -532,41 +545,6 @@ CellGroup MeshGmsh::Cells(int coordType, int physicalGroupId) const
     return g;
 }
 
-void AddTo(DofMatrixSparse<double>& a, const DofMatrixSparse<double>& b)
-{
-    const int ra = a.rowPtr.empty() ? 0 : a.Rows(), rb = b.rowPtr.empty() ? 0 : b.Rows();
-    const int rows = std::max(ra, rb);
-    DofMatrixSparse<double> s;
-    s.rowPtr.assign(1, 0);
-    s.col.reserve(a.col.size() + b.col.size());
-    s.val.reserve(a.col.size() + b.col.size());
-    for (int i = 0; i < rows; i++)
-    {
-        int p = i < ra ? a.rowPtr[i] : 0, pe = i < ra ? a.rowPtr[i + 1] : 0;
-        int q = i < rb ? b.rowPtr[i] : 0, qe = i < rb ? b.rowPtr[i + 1] : 0;
-        while (p < pe || q < qe)  // both rows have ascending columns
-        {
-            if (q == qe || (p < pe && a.col[p] < b.col[q]))
-            {
-                s.col.push_back(a.col[p]);
-                s.val.push_back(a.val[p++]);
-            }
-            else if (p == pe || b.col[q] < a.col[p])
-            {
-                s.col.push_back(b.col[q]);
-                s.val.push_back(b.val[q++]);
-            }
-            else
-            {
-                s.col.push_back(a.col[p]);
-                s.val.push_back(a.val[p++] + b.val[q++]);
-            }
-        }
-        s.rowPtr.push_back((int)s.col.size());
-    }
-    a = std::move(s);
-}
-
 void TimeDependentProblem::AddGradientFunction(const CellGroup& group, const GradientDamage& f)
 {
     mGradient.push_back({&group, &f, nullptr, nullptr});
@@ -629,16 +607,12 @@ DofVector<double> TimeDependentProblem::Gradient(const DofVector<double>& dofVal
 DofMatrixSparse<double> TimeDependentProblem::Hessian0(const DofVector<double>& dofValues,
                                                        std::vector<DofType> dofs, double /*t*/, double /*dt*/)
 {
-    int rows = 0;
-    for (const DofType& d : dofs)
-        rows += mDofInfo.Total(d);
     DofMatrixSparse<double> hessian0;
-    hessian0.rowPtr.assign(rows + 1, 0);
     for (const Term& term : mHessian0)
     {
         CheckDofs(term, dofs);
-        AddTo(hessian0, term.gdm ? mAssembler.BuildMatrix(*term.group, *term.gdm, dofValues)
-                                 : mAssembler.BuildMatrix(*term.group, *term.mb, dofValues));
+        hessian0 += term.gdm ? mAssembler.BuildMatrix(*term.group, *term.gdm, dofValues)
+                             : mAssembler.BuildMatrix(*term.group, *term.mb, dofValues);
     }
     return hessian0;
 }

@@ -4,8 +4,11 @@
 //   Exception          NuTo::Exception (nuto/base/Exception.h:9-26), thrown for every failing call
 //   DofType            NuTo::DofType (nuto/mechanics/dofs/DofType.h:8-50): name, components, unique id
 //   DofVector<T>       NuTo::DofVector (DofVector.h:11-149): one value vector per DofType
-//   DofMatrixSparse<T> NuTo::DofMatrixSparse (DofMatrixSparse.h:9-10) exported monolithically as
-//                      ToEigen(K, {d0, d1}) does (DofMatrixSparseConvertEigen.h:40-67): CSR here
+//   SparseMatrix<T>    the blocks' Eigen::SparseMatrix<T> (a CSR here: columns ascending per row,
+//                      explicit zeros kept as setFromTriplets keeps them)
+//   DofMatrixSparse<T> NuTo::DofMatrixSparse (DofMatrixSparse.h:9-10): one SparseMatrix per dof type
+//                      pair, K(dofI, dofJ) (DofMatrixContainer.h:16-24); ToEigen(K, dofs) exports the
+//                      blocks of any dof list monolithically (DofMatrixSparseConvertEigen.h:40-67)
 //   DofInfo            NuTo::DofInfo (DofInfo.h:7-17): numIndependent / numDependent per DofType
 //   SimpleAssembler    BuildVector / BuildMatrix (SimpleAssembler.h:14-21, SimpleAssembler.cpp:38-156)
 //   Constraints        Constraint::Constraints / Equation (Constraints.h:47-130, Equation.h:12-66): per DofType,
@@ -75,24 +78,186 @@ const std::vector<T>& DofVector<T>::operator[](const DofType& d) const
     return it->second;
 }
 
-// Monolithic CSR of the blocks (d0, d0) (d0, d1) (d1, d0) (d1, d1): rows [d0 ; d1], columns [d0, d1].
+// One block: rows x cols, CSR with columns ascending within each row; explicit zeros are entries.
 template <typename T>
-struct DofMatrixSparse
+struct SparseMatrix
 {
-    std::vector<int> rowPtr, col;
+    int rows = 0, cols = 0;
+    std::vector<int> rowPtr{0}, col;
     std::vector<T> val;
-    int Rows() const { return (int)rowPtr.size() - 1; }
+    SparseMatrix() = default;
+    SparseMatrix(int r, int c)
+        : rows(r)
+        , cols(c)
+        , rowPtr(r + 1, 0)
+    {
+    }
+    int Rows() const { return rows; }
+    int Cols() const { return cols; }
     long long NonZeros() const { return (long long)val.size(); }
     T Coeff(int row, int column) const;  // 0 outside the pattern
+    T& CoeffRef(int row, int column);    // inserts a zero entry when absent, as Eigen's coeffRef does
+    SparseMatrix& operator+=(const SparseMatrix& b);  // Eigen's sparse sum: the union of both patterns
 };
 
 template <typename T>
-T DofMatrixSparse<T>::Coeff(int row, int column) const
+T SparseMatrix<T>::Coeff(int row, int column) const
 {
     for (int k = rowPtr[row]; k < rowPtr[row + 1]; k++)
         if (col[k] == column)
             return val[k];
     return T(0);
+}
+
+template <typename T>
+T& SparseMatrix<T>::CoeffRef(int row, int column)
+{
+    if (row < 0 || row >= rows || column < 0 || column >= cols)
+        throw std::out_of_range("SparseMatrix::CoeffRef: index outside the matrix");
+    int k = rowPtr[row];
+    while (k < rowPtr[row + 1] && col[k] < column)
+        k++;
+    if (k < rowPtr[row + 1] && col[k] == column)
+        return val[k];
+    col.insert(col.begin() + k, column);
+    val.insert(val.begin() + k, T(0));
+    for (int r = row + 1; r <= rows; r++)
+        rowPtr[r]++;
+    return val[k];
+}
+
+template <typename T>
+SparseMatrix<T>& SparseMatrix<T>::operator+=(const SparseMatrix<T>& b)
+{
+    if (b.rows != rows || b.cols != cols)
+        throw std::invalid_argument("SparseMatrix +=: the blocks differ in size");
+    SparseMatrix<T> s(rows, cols);
+    s.col.reserve(col.size() + b.col.size());
+    s.val.reserve(col.size() + b.col.size());
+    for (int i = 0; i < rows; i++)
+    {
+        int p = rowPtr[i], pe = rowPtr[i + 1], q = b.rowPtr[i], qe = b.rowPtr[i + 1];
+        while (p < pe || q < qe)  // both rows have ascending columns
+        {
+            if (q == qe || (p < pe && col[p] < b.col[q]))
+            {
+                s.col.push_back(col[p]);
+                s.val.push_back(val[p++]);
+            }
+            else if (p == pe || b.col[q] < col[p])
+            {
+                s.col.push_back(b.col[q]);
+                s.val.push_back(b.val[q++]);
+            }
+            else
+            {
+                s.col.push_back(col[p]);
+                s.val.push_back(val[p++] + b.val[q++]);
+            }
+        }
+        s.rowPtr[i + 1] = (int)s.col.size();
+    }
+    *this = std::move(s);
+    return *this;
+}
+
+// DofMatrixContainer<SparseMatrix<T>> (DofMatrixContainer.h:12-122): K(dofI, dofJ) creates the block
+// when absent (std::map::operator[]); the const access throws std::out_of_range (std::map::at).
+template <typename T>
+class DofMatrixSparse
+{
+public:
+    SparseMatrix<T>& operator()(const DofType& d0, const DofType& d1) { return mData[{d0.Id(), d1.Id()}]; }
+    const SparseMatrix<T>& operator()(const DofType& d0, const DofType& d1) const
+    {
+        auto it = mData.find({d0.Id(), d1.Id()});
+        if (it == mData.end())
+            throw std::out_of_range("DofMatrixSparse: no block (" + d0.GetName() + ", " + d1.GetName() + ")");
+        return it->second;
+    }
+    bool Has(const DofType& d0, const DofType& d1) const { return mData.count({d0.Id(), d1.Id()}) != 0; }
+    // DofMatrixContainer::operator+= (DofMatrixContainer.h:27-37): blocks present in both are summed,
+    // blocks only in rhs are copied
+    DofMatrixSparse& operator+=(const DofMatrixSparse& rhs)
+    {
+        for (const auto& entry : rhs.mData)
+        {
+            auto it = mData.find(entry.first);
+            if (it == mData.end())
+                mData[entry.first] = entry.second;
+            else
+                it->second += entry.second;
+        }
+        return *this;
+    }
+
+private:
+    std::map<std::pair<int, int>, SparseMatrix<T>> mData;
+};
+
+// TotalRows / TotalCols / TotalNonZeros (DofMatrixSparseConvertEigen.h:7-30)
+template <typename T>
+int TotalRows(const DofMatrixSparse<T>& v, const std::vector<DofType>& dofs)
+{
+    int total = 0;
+    for (const DofType& d : dofs)
+        total += v(d, d).Rows();
+    return total;
+}
+
+template <typename T>
+int TotalCols(const DofMatrixSparse<T>& v, const std::vector<DofType>& dofs)
+{
+    int total = 0;
+    for (const DofType& d : dofs)
+        total += v(d, d).Cols();
+    return total;
+}
+
+template <typename T>
+long long TotalNonZeros(const DofMatrixSparse<T>& v, const std::vector<DofType>& dofs)
+{
+    long long total = 0;
+    for (const DofType& d0 : dofs)
+        for (const DofType& d1 : dofs)
+            total += v(d0, d1).NonZeros();
+    return total;
+}
+
+// ToEigen (DofMatrixSparseConvertEigen.h:40-67): the blocks of `dofs` as one matrix, rows and columns
+// in the order of `dofs` (row block of dofRow offset by the rows of the dof types before it, column
+// block by the columns of the blocks before it in that row). One dof type: its block.
+template <typename T>
+SparseMatrix<T> ToEigen(const DofMatrixSparse<T>& v, std::vector<DofType> dofs)
+{
+    if (dofs.size() == 1)
+        return v(dofs.front(), dofs.front());
+    SparseMatrix<T> out(TotalRows(v, dofs), TotalCols(v, dofs));
+    out.col.reserve((size_t)TotalNonZeros(v, dofs));
+    out.val.reserve((size_t)TotalNonZeros(v, dofs));
+    int startRow = 0;
+    for (const DofType& dofRow : dofs)
+    {
+        const int nr = v(dofRow, dofRow).Rows();
+        for (int i = 0; i < nr; i++)
+        {
+            int startCol = 0;
+            for (const DofType& dofCol : dofs)
+            {
+                const SparseMatrix<T>& m = v(dofRow, dofCol);
+                if (i < m.Rows())
+                    for (int k = m.rowPtr[i]; k < m.rowPtr[i + 1]; k++)
+                    {
+                        out.col.push_back(m.col[k] + startCol);
+                        out.val.push_back(m.val[k]);
+                    }
+                startCol += m.Cols();
+            }
+            out.rowPtr[startRow + i + 1] = (int)out.col.size();
+        }
+        startRow += nr;
+    }
+    return out;
 }
 
 struct DofInfo
@@ -299,14 +464,9 @@ private:
     int mNip = 0;  // IPs per cell of the rule of the last Setup (tetrahedron or tensor-product Gauss)
 };
 
-// DofMatrixSparse += (DofMatrixSparse.h:9-10 over Eigen's sparse sum): the union of both patterns,
-// entries present in both added, explicit zeros kept; a matrix with fewer rows (a leading part of
-// the dof types, e.g. {displacements} of {displacements, eeq}) adds to the leading rows.
-void AddTo(DofMatrixSparse<double>& a, const DofMatrixSparse<double>& b);
-
 // TimeDependentProblem (TimeDependentProblem.h:20-90, TimeDependentProblem.cpp:34-85): (cell group,
 // integrand) pairs. Gradient and Hessian0 assemble every pair with SimpleAssembler and sum the
-// results in the order the pairs were added (DofVector +=, DofMatrixSparse += as AddTo);
+// results in the order the pairs were added (DofVector +=, DofMatrixSparse += per block);
 // UpdateHistory updates every pair's history in place. Groups and integrands are held by
 // reference: the caller keeps them alive, as NuTo's Bind captures the integrand by reference
 // (Bind.h:14,21). The node values are the dofValues argument (NuTo merges them into its nodes

@@ -78,11 +78,14 @@ static std::vector<int> CellDofs(const std::vector<int>& ranks, int ncomp)
     return d;
 }
 
+// R and the monolithic export of K (ToEigen over the integrand's dof types, rows [d0; d1]) against
+// the oracle's assembly
 static void Compare(const char* what, void* P, int nRows, const std::vector<double>& R,
-                    const DofMatrixSparse<double>& K)
+                    const SparseMatrix<double>& K)
 {
     std::vector<double> rR(nRows), rB(nRows);
     orc_problem_R(P, rR.data(), rB.data());
+    EXPECT(K.Rows() == nRows && K.Cols() == nRows, (std::string(what) + " size").c_str());
     for (int i = 0; i < nRows; i++)
         EXPECT(std::fabs(R[i] - rR[i]) <= 1e-12 * rB[i], (std::string(what) + " R").c_str());
     std::vector<int> rp(nRows + 1), ci(K.col.size());
@@ -96,6 +99,97 @@ static void Compare(const char* what, void* P, int nRows, const std::vector<doub
             EXPECT(false, (std::string(what) + " K").c_str());
             break;
         }
+}
+
+// The blocks of a two-field matrix: sizes, the export in the other dof order (rows and columns
+// [eeq; disp]) as a permutation of [disp; eeq], and the const access to a missing block.
+static void BlockChecks(const DofMatrixSparse<double>& K, const DofType& disp, const DofType& eeq)
+{
+    const int nu = K(disp, disp).Rows(), ne = K(eeq, eeq).Rows();
+    EXPECT(K(disp, disp).Cols() == nu && K(disp, eeq).Rows() == nu && K(disp, eeq).Cols() == ne &&
+               K(eeq, disp).Rows() == ne && K(eeq, disp).Cols() == nu && K(eeq, eeq).Cols() == ne,
+           "block sizes");
+    EXPECT(K(disp, eeq).NonZeros() == K(eeq, disp).NonZeros(), "ue and eu blocks have transposed patterns");
+    const SparseMatrix<double> a = ToEigen(K, {disp, eeq}), b = ToEigen(K, {eeq, disp});
+    EXPECT(a.NonZeros() == b.NonZeros() && TotalNonZeros(K, {disp, eeq}) == a.NonZeros(), "export nonzeros");
+    bool same = true;
+    for (int i = 0; i < a.Rows() && same; i++)
+        for (int k = a.rowPtr[i]; k < a.rowPtr[i + 1]; k++)
+        {
+            const int i2 = i < nu ? i + ne : i - nu, j2 = a.col[k] < nu ? a.col[k] + ne : a.col[k] - nu;
+            same = same && b.Coeff(i2, j2) == a.val[k];
+        }
+    EXPECT(same, "ToEigen in the order {eeq, disp} permutes the {disp, eeq} export");
+    bool threw = false;
+    try
+    {
+        DofType other("Temperature", 1);
+        (void)K(disp, other);
+    }
+    catch (const std::out_of_range&)
+    {
+        threw = true;
+    }
+    EXPECT(threw, "missing DofMatrixSparse block throws std::out_of_range");
+}
+
+// The reference's container tests (test/mechanics/dofs/DofMatrixSparseConvertEigen.cpp:4-49) and the
+// block sum: host code only, no device.
+static void ContainerCases()
+{
+    DofType dof0("foo", 1), dof1("bar", 1);
+    {
+        DofMatrixSparse<double> m;
+        m(dof0, dof0) = SparseMatrix<double>(2, 2);
+        m(dof0, dof1) = SparseMatrix<double>(2, 4);
+        m(dof1, dof0) = SparseMatrix<double>(4, 2);
+        m(dof1, dof1) = SparseMatrix<double>(4, 4);
+        double e[6][6] = {};
+        m(dof0, dof0).CoeffRef(0, 1) = 1;
+        m(dof0, dof0).CoeffRef(1, 0) = 10;
+        e[0][1] = 1;
+        e[1][0] = 10;
+        m(dof0, dof1).CoeffRef(0, 3) = 4;
+        e[0][3 + 2] = 4;
+        m(dof1, dof0).CoeffRef(2, 1) = 5;
+        e[2 + 2][1] = 5;
+        m(dof1, dof1).CoeffRef(2, 3) = 6;
+        e[2 + 2][3 + 2] = 6;
+        const SparseMatrix<double> x = ToEigen(m, {dof0, dof1});
+        double err = 0.0;
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 6; j++)
+                err += (x.Coeff(i, j) - e[i][j]) * (x.Coeff(i, j) - e[i][j]);
+        EXPECT(x.Rows() == 6 && x.Cols() == 6 && std::sqrt(err) < 1e-10, "DofMatrixExportEigen");
+        // the other order: row/column blocks swap
+        const SparseMatrix<double> y = ToEigen(m, {dof1, dof0});
+        EXPECT(y.Coeff(2, 1 + 4) == 5 && y.Coeff(0 + 4, 3) == 4 && y.Coeff(2, 3) == 6 && y.Coeff(4, 5) == 1 &&
+                   y.Coeff(5, 4) == 10 && y.NonZeros() == 5,
+               "ToEigen in the order {dof1, dof0}");
+        // DofMatrixContainer::operator+=: summed blocks take the union of the patterns
+        DofMatrixSparse<double> s;
+        s += m;
+        s += m;
+        EXPECT(s(dof0, dof1).Coeff(0, 3) == 8 && s(dof1, dof1).NonZeros() == 1, "DofMatrixSparse +=");
+    }
+    {
+        DofMatrixSparse<double> m;
+        m(dof0, dof0) = SparseMatrix<double>(2, 8);
+        m(dof0, dof1) = SparseMatrix<double>(0, 0);
+        m(dof1, dof0) = SparseMatrix<double>(0, 0);
+        m(dof1, dof1) = SparseMatrix<double>(0, 10);
+        bool ok = true;
+        SparseMatrix<double> e;
+        try
+        {
+            e = ToEigen(m, {dof0, dof1});
+        }
+        catch (...)
+        {
+            ok = false;
+        }
+        EXPECT(ok && e.Rows() == 2 && e.Cols() == 18, "DofMatrixZeros");
+    }
 }
 
 static void GradientDamageCase()
@@ -146,7 +240,8 @@ static void GradientDamageCase()
     nuto_params p{NUTO_GRADIENT_DAMAGE, 3, 30000.0, 0.2, 0, 10.0, 1e-4, 100.0, 0.99, 4.0, 0};
     void* P = orc_problem(om, &p, 0, 2, Ld, Le);
     orc_problem_assemble(P, state[disp].data(), state[eeq].data(), f.kappas.data(), 1, 1, 1);
-    Compare("gradient damage", P, 4 * m.nDofNodes, Rall, K);
+    Compare("gradient damage", P, 4 * m.nDofNodes, Rall, ToEigen(K, {disp, eeq}));
+    BlockChecks(K, disp, eeq);
     orc_problem_free(P);
     orc_mesh_free(om);
 
@@ -198,7 +293,7 @@ static void DecreasingInteractionCase()
     nuto_params p{NUTO_GRADIENT_DAMAGE, 3, 30000.0, 0.2, 0, 10.0, 1e-4, 100.0, 0.99, 4.0, 0, 1, 0.005, 5.0};
     void* P = orc_problem(om, &p, 0, 2, Ld, Le);
     orc_problem_assemble(P, state[disp].data(), state[eeq].data(), f.kappas.data(), 1, 1, 1);
-    Compare("gradient damage, decreasing interaction", P, 4 * m.nDofNodes, Rall, K);
+    Compare("gradient damage, decreasing interaction", P, 4 * m.nDofNodes, Rall, ToEigen(K, {disp, eeq}));
     orc_problem_free(P);
     orc_mesh_free(om);
 }
@@ -227,7 +322,8 @@ static void MomentumBalanceCase()
     nuto_params p{NUTO_MOMENTUM_LINEAR_ELASTIC, 2, 30000.0, 0.2, 0, 0, 0, 0, 0, 0, 0};
     void* P = orc_problem(om, &p, 0, 2, Ld, -1);
     orc_problem_assemble(P, state[disp].data(), nullptr, nullptr, 1, 1, 1);
-    Compare("momentum balance", P, 2 * m.nDofNodes, R[disp], K);
+    // one dof type: hessian(d, d) is the whole matrix (test/mechanics/cell/SimpleAssembler.cpp:113)
+    Compare("momentum balance", P, 2 * m.nDofNodes, R[disp], K(disp, disp));
     orc_problem_free(P);
     orc_mesh_free(om);
 }
@@ -269,7 +365,7 @@ static void QuadraticTetrahedraCase()
     nuto_params p{NUTO_MOMENTUM_LINEAR_ELASTIC, 3, 30000.0, 0.2, 0, 0, 0, 0, 0, 0, 0};
     void* P = orc_problem(om, &p, 1, 2, Ld, -1);
     orc_problem_assemble(P, state[disp].data(), nullptr, nullptr, 1, 1, 1);
-    Compare("ten-node tetrahedra", P, 3 * nDofNodes, R[disp], K);
+    Compare("ten-node tetrahedra", P, 3 * nDofNodes, R[disp], ToEigen(K, {disp}));
     std::vector<double> ref(stress.size());
     orc_problem_eval_ip(P, state[disp].data(), nullptr, nullptr, NUTO_IP_STRESS, ref.data());
     double worst = 0.0, scale = 0.0;
@@ -493,7 +589,7 @@ static void TimeDependentProblemCase()
     void* P = orc_problem(om, &p, 0, 2, Ld, Le);
     orc_problem_set_theta(P, E.data(), ft.data(), gf.data());
     orc_problem_assemble(P, state[disp].data(), state[eeq].data(), kap.data(), 1, 1, 1);
-    Compare("time dependent problem, two groups", P, 4 * m.nDofNodes, Rall, K);
+    Compare("time dependent problem, two groups", P, 4 * m.nDofNodes, Rall, ToEigen(K, {disp, eeq}));
     orc_problem_free(P);
     orc_mesh_free(om);
 
@@ -528,10 +624,18 @@ static void TimeDependentProblemCase()
     EXPECT(threw, "dof types that do not fit the integrand throw Exception");
 }
 
-int main()
+int main(int argc, char** argv)
 {
+    // "containers": the host-only container cases (no device needed)
+    const bool containersOnly = argc > 1 && std::string(argv[1]) == "containers";
     try
     {
+        ContainerCases();
+        if (containersOnly)
+        {
+            std::printf(gFailures ? "FAILED (%d)\n" : "ok\n", gFailures);
+            return gFailures ? 1 : 0;
+        }
         GradientDamageCase();
         DecreasingInteractionCase();
         MomentumBalanceCase();

@@ -129,3 +129,15 @@ def test_batch_binding_refuses_short_tensors():
     b.sizes = None
     with pytest.raises(ValueError, match="build"):
         b.assemble(u, e, kap, R=f64(40), K=f64(100))
+
+
+def test_cpp_mirror_containers():
+    """The C++ mirror's DofMatrixSparse blocks and ToEigen on the reference's container cases
+    (test/mechanics/dofs/DofMatrixSparseConvertEigen.cpp:4-49): host code, run without a device."""
+    import subprocess
+
+    exe = os.path.join(ROOT, "tests", "cpp", "test_host_shim")
+    if not os.path.exists(exe):
+        pytest.skip("tests/cpp not built (__graft_entry__.build())")
+    r = subprocess.run([exe, "containers"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stdout + r.stderr
