@@ -255,8 +255,15 @@ def batched(n_meshes, steps, warmup):
         k = {(N.TET4, N.TET4): "tet4", (N.TET4, N.TET10): "tet10", (N.HEX8, N.HEX8): "hex8",
              (N.HEX8, N.HEX20): "hex20"}[(ct, dt)]
         counts[k] = counts.get(k, 0) + 1
+    names = {N.TET4: "tet4", N.TET10: "tet10", N.HEX8: "hex8", N.HEX20: "hex20"}
+    kernels = {}
+    for g in range(b.group_info(0)["groups"]):
+        gi = b.group_info(g)
+        kernels[names[gi["dof_type"]]] = {1: "gram", 0: "columns", -1: "default"}[gi["cell_kernel"]]
+    out_bytes = 8 * (sizes["K"] + sizes["R"])
     return {"workload": f"{n_meshes} meshes, cells log-uniform in [1e3, 5e4], types {counts}, jitter 10 %, "
                         f"local isotropic damage R+K", "cells": cells, "nnz": sizes["K"],
+            "output_GBps": out_bytes / (graph_ms * 1e-3) / 1e9, "cell_kernel": kernels,
             "batched_ms": graph_ms, "per_mesh_launch_ms": per_mesh_ms, "speedup": per_mesh_ms / graph_ms,
             "cells_per_s": cells / (graph_ms * 1e-3), "tuned_mode": best,
             "per_mesh_mode": per_mesh_mode,

@@ -315,6 +315,10 @@ int nuto_batch_build(nuto_batch* batch); /* patterns of all meshes, once */
 int nuto_batch_sizes(nuto_batch* batch, long long* sizes);
 int nuto_batch_mesh_info(nuto_batch* batch, int mesh, long long* info /* 17 */);
 int nuto_batch_copy_pattern(nuto_batch* batch, int mesh, int* row_ptr, int* col);
+/* info[6] of a group: dof element type, meshes, dof nodes, cells, cell-matrix kernel of its
+ * two-phase launch (1 Gram form, 0 by columns, -1 the element's default until nuto_batch_autotune
+ * times both), number of groups */
+int nuto_batch_group_info(nuto_batch* batch, int group, long long* info /* 6 */);
 /* per-cell (E, f_t, G_f) over all cells, meshes in the order they were added (host arrays) */
 int nuto_batch_set_material_fields(nuto_batch* batch, const double* E, const double* ft, const double* gf);
 /* R and K of every mesh. mode -1: the tuned launch (mode 3 until tuned) replayed from a HIP graph; 0..2: one fused launch

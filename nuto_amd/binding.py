@@ -138,6 +138,7 @@ def lib():
         L.nuto_batch_build.argtypes = [vp]
         L.nuto_batch_sizes.argtypes = [vp, vp]
         L.nuto_batch_mesh_info.argtypes = [vp, ctypes.c_int, vp]
+        L.nuto_batch_group_info.argtypes = [vp, ctypes.c_int, vp]
         L.nuto_batch_copy_pattern.argtypes = [vp, ctypes.c_int, vp, vp]
         L.nuto_batch_set_material_fields.argtypes = [vp, vp, vp, vp]
         L.nuto_batch_assemble.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_int]
@@ -771,6 +772,14 @@ class Batch:
         info = np.zeros(17, np.int64)
         self._chk(lib().nuto_batch_mesh_info(self.h, m, _np_ptr(info)))
         return dict(zip(self.MESH_INFO, (int(v) for v in info)))
+
+    def group_info(self, g):
+        """A group's dof element type, meshes, dof nodes, cells, cell-matrix kernel (1 Gram form,
+        0 by columns, -1 the element's default) and the number of groups."""
+        info = np.zeros(6, np.int64)
+        self._chk(lib().nuto_batch_group_info(self.h, g, _np_ptr(info)))
+        return dict(zip(("dof_type", "meshes", "dof_nodes", "cells", "cell_kernel", "groups"),
+                        (int(v) for v in info)))
 
     def pattern(self, m):
         i = self.mesh_info(m)

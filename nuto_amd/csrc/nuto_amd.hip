@@ -16,6 +16,7 @@
 #include "gdm_hex8_vjp.hpp"
 #include "batch.hpp"
 #include "cell_fields.hpp"
+#include "cell_gram.hpp"
 #include "cell_matrices.hpp"
 #include "solve.hpp"
 #include "mesh.hpp"
@@ -97,6 +98,7 @@ struct nuto_ctx
     // staged cell matrices and vectors of the two-phase generic assembly
     DevBuf keStage, reStage;
     int maxAdj = -1;  // largest node neighbourhood (LDS rows of the wave gather), from the node graph
+    int cellGram = -1;  // cell-matrix kernel of the two-phase assembly: 1 Gram form, 0 by columns, -1 per element
 };
 
 namespace
@@ -324,17 +326,73 @@ bool TryStage(nuto_ctx* ctx, int nd)
 
 // Two-phase assembly of the cells [c0, c1) and the rows of the dof nodes [p0, p1) (the rows'
 // incident cells must lie in [c0, c1): a whole mesh of a batch, or the whole group).
+// NUTO_AMD_CELL_GRAM=0 computes the cell matrices column by column (kCellMatrices) instead of in
+// Gram form (kCellGram)
+bool CellGramOff()
+{
+    const char* v = std::getenv("NUTO_AMD_CELL_GRAM");
+    return v && v[0] == '0';
+}
+
+// kCellGram's shape per element: cells per workgroup, threads, IPs per chunk, unroll of the IP loop
+// of the blocks (3 only for hex20, whose long chunk hides the LDS latency; the others lose occupancy)
+template <int NND>
+struct GramShape;
+template <>
+struct GramShape<4>  // tet4: 8 lanes per cell
+{
+    static constexpr int CPW = 8, NT = 64, ICH = 1, UNR = 1;
+};
+template <>
+struct GramShape<10>  // tet10: 50 lanes per cell
+{
+    static constexpr int CPW = 2, NT = 128, ICH = 4, UNR = 1;
+};
+template <>
+struct GramShape<8>  // hex8: 32 lanes per cell
+{
+    static constexpr int CPW = 2, NT = 64, ICH = 4, UNR = 1;
+};
+template <>
+struct GramShape<20>  // hex20: 200 lanes per cell
+{
+    static constexpr int CPW = 1, NT = 256, ICH = 9, UNR = 3;
+};
+
+template <int KIND, int NNC, int NND, int NIP>
+void LaunchCellMatrices(const RowArgs& a, bool gram, int c0, int c1, double* ke, double* re, hipStream_t stream)
+{
+    if (!gram || CellGramOff())
+    {
+        constexpr int CPW = 64 / (NND * 3);
+        hipLaunchKernelGGL((kCellMatrices<KIND, NNC, NND, NIP>), dim3((unsigned)((c1 - c0 + CPW - 1) / CPW)), dim3(64),
+                           0, stream, a, c0, c1, ke, re);
+        return;
+    }
+    using Sh = GramShape<NND>;
+    const dim3 grid((unsigned)((c1 - c0 + Sh::CPW - 1) / Sh::CPW));
+    // one Gram unless UNILATERAL damage application makes c1 / c2 vary between IPs
+    if (KIND == kLocalDamage && a.P.damage_application != 0)
+        hipLaunchKernelGGL((kCellGram<KIND, NNC, NND, NIP, Sh::CPW, Sh::NT, Sh::ICH, Sh::UNR, true>), grid, dim3(Sh::NT), 0,
+                           stream, a, c0, c1, ke, re);
+    else
+        hipLaunchKernelGGL((kCellGram<KIND, NNC, NND, NIP, Sh::CPW, Sh::NT, Sh::ICH, Sh::UNR, false>), grid, dim3(Sh::NT), 0,
+                           stream, a, c0, c1, ke, re);
+}
+
 template <int KIND, int NNC, int NND, int NIP>
 void LaunchTwoPhaseT(nuto_ctx* ctx, const RowArgs& a, bool wr, int c0, int c1, int p0, int p1, hipStream_t stream)
 {
-    constexpr int ND = NND * 3, CPW = 64 / ND;
+    constexpr int ND = NND * 3;
     if (c1 <= c0 || p1 <= p0)
         return;
     EnsureStage(ctx, ND);
     double* ke = ctx->keStage.As<double>();
     double* re = ctx->reStage.As<double>();
-    hipLaunchKernelGGL((kCellMatrices<KIND, NNC, NND, NIP>), dim3((unsigned)((c1 - c0 + CPW - 1) / CPW)), dim3(64), 0,
-                       stream, a, c0, c1, ke, re);
+    // the Gram form wins on the hexahedra (32 and 200 lanes per cell); the column kernel is as fast or
+    // faster on the tetrahedra, whose cell matrices are small (a batch's autotune measures both)
+    const bool gram = ctx->cellGram >= 0 ? ctx->cellGram == 1 : NND >= 8;
+    LaunchCellMatrices<KIND, NNC, NND, NIP>(a, gram, c0, c1, ke, re, stream);
     if (ctx->maxAdj < 0)
     {
         int m = 0;
@@ -2090,6 +2148,21 @@ int nuto_batch_sizes(nuto_batch* b, long long* sizes)
     });
 }
 
+int nuto_batch_group_info(nuto_batch* b, int group, long long* info)
+{
+    return GuardB(b, [&] {
+        Require(b->built, NUTO_ERR_STATE, "batch not built (nuto_batch_build)");
+        Require(group >= 0 && group < (int)b->groups.size() && info, NUTO_ERR_ARGUMENT, "bad group index / info");
+        const BatchGroup& g = b->groups[group];
+        info[0] = g.dt;
+        info[1] = (long long)g.meshes.size();
+        info[2] = g.nNodes;
+        info[3] = g.nCells;
+        info[4] = g.ctx->cellGram;
+        info[5] = (long long)b->groups.size();
+    });
+}
+
 int nuto_batch_mesh_info(nuto_batch* b, int mesh, long long* info)
 {
     return GuardB(b, [&] {
@@ -2224,34 +2297,56 @@ int nuto_batch_autotune(nuto_batch* b, const double* u, const double* e, const d
         hipEvent_t t0, t1;
         Check(hipEventCreate(&t0), "event");
         Check(hipEventCreate(&t1), "event");
+        // median of 5 timed runs after one warm-up
+        auto median = [&](auto&& launch) {
+            launch();
+            std::vector<float> rep;
+            for (int it = 0; it < 5; it++)
+            {
+                Check(hipEventRecord(t0, b->stream), "event");
+                launch();
+                Check(hipEventRecord(t1, b->stream), "event");
+                Check(hipEventSynchronize(t1), "event");
+                float x = 0.0f;
+                Check(hipEventElapsedTime(&x, t0, t1), "event");
+                rep.push_back(x);
+            }
+            std::sort(rep.begin(), rep.end());
+            return (double)rep[rep.size() / 2];
+        };
+        // the two-phase modes need the staging of every group; without it they are not candidates
+        bool staged = !IsGdm(b);
+        for (BatchGroup& g : b->groups)
+            staged = staged && TryStage(g.ctx, 3 * NodesOf(g.dt));
+        if (staged)
+        {
+            // each group's cell-matrix kernel (Gram form or by columns), timed on the group's two-phase launch
+            for (BatchGroup& g : b->groups)
+            {
+                const RowArgs a = GroupArgs(b, g, u, e, kappa, R, K);
+                double t[2];
+                for (int gram = 0; gram < 2; gram++)
+                {
+                    g.ctx->cellGram = gram;
+                    t[gram] = median([&] {
+                        if (b->params.kind == kLinearElastic)
+                            LaunchTwoPhaseType<kLinearElastic>(g.ctx, a, g.type, 0, g.nCells, 0, g.nNodes, b->stream);
+                        else
+                            LaunchTwoPhaseType<kLocalDamage>(g.ctx, a, g.type, 0, g.nCells, 0, g.nNodes, b->stream);
+                    });
+                }
+                g.ctx->cellGram = t[1] <= t[0] ? 1 : 0;
+            }
+        }
         double bestMs = 1e300;
         int bestMode = 3;
         for (int mode = 0; mode < kBatchModes; mode++)
         {
             double v = -1.0;
-            bool usable = !(IsGdm(b) && (mode < 3 || mode >= 5));
-            if (usable && mode >= 5)
-            {
-                // the two-phase modes need the staging of every group; without it they are not candidates
-                for (BatchGroup& g : b->groups)
-                    usable = usable && TryStage(g.ctx, 3 * NodesOf(g.dt));
-            }
+            const bool usable = !(IsGdm(b) && (mode < 3 || mode >= 5)) && (mode < 5 || staged);
             if (usable)
             {
-                LaunchBatch(b, mode, u, e, kappa, R, K);  // warm-up
-                std::vector<float> rep;
-                for (int it = 0; it < 5; it++)
-                {
-                    Check(hipEventRecord(t0, b->stream), "event");
-                    LaunchBatch(b, mode, u, e, kappa, R, K);
-                    Check(hipEventRecord(t1, b->stream), "event");
-                    Check(hipEventSynchronize(t1), "event");
-                    float x = 0.0f;
-                    Check(hipEventElapsedTime(&x, t0, t1), "event");
-                    rep.push_back(x);
-                }
-                std::sort(rep.begin(), rep.end());
-                v = rep[rep.size() / 2];
+                v = median([&] { LaunchBatch(b, mode, u, e, kappa, R, K); });
                 if (mode != 4 && mode != 6 && v < bestMs)  // per-mesh launches are the reference, not candidates
                 {
                     bestMs = v;
