@@ -161,6 +161,17 @@ int nuto_build_matrix(nuto_ctx* ctx, const double* u, const double* e, const dou
 int nuto_build_vector_matrix(nuto_ctx* ctx, const double* u, const double* e, const double* kappa, double* R,
                              double* K);
 
+/* No counterpart in the reference: nuto_build_vector_matrix for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc). It calls no
+ * allocating routine: the hex8 gradient-damage node kernel's pre-pass records go to a static 32 MiB
+ * device scratch array (one per device), so it takes at most 29127 cells (8 IPs each; more:
+ * NUTO_ERR_ARGUMENT). Needs the node tables from set-up: the integrand and the rule set before
+ * nuto_build_pattern, or one nuto_build_vector_matrix call (NUTO_ERR_STATE otherwise); other
+ * integrands and elements: NUTO_ERR_UNSUPPORTED. Every context on one device shares that device's
+ * scratch: calls that overlap in time on one device (other streams, other host threads) race. */
+int nuto_build_vector_matrix_noalloc(nuto_ctx* ctx, const double* u, const double* e, const double* kappa,
+                                     double* R, double* K);
+
 /* TimeDependentProblem::UpdateHistory -> GradientDamage::Update (TimeDependentProblem.cpp:78-85,
  * GradientDamage.h:122-125) or EvolutionImplicit::Update (LocalIsotropicDamage.h:130-133):
  * kappa(cell, ip) = max(kappa(cell, ip), eeq(ip)), in place. */

@@ -162,6 +162,7 @@ def lib():
         L.nuto_solve_trial_state.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.POINTER(SolverOptions),
                                               ctypes.POINTER(SolveInfo)]
         L.nuto_build_vector_matrix.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.nuto_build_vector_matrix_noalloc.argtypes = [vp, vp, vp, vp, vp, vp]
         L.nuto_device_alloc.argtypes = [vp, ctypes.POINTER(vp), ctypes.c_ulonglong]
         L.nuto_device_free.argtypes = [vp, vp]
         L.nuto_copy_to_device.argtypes = [vp, vp, vp, ctypes.c_ulonglong]
@@ -596,6 +597,14 @@ class Context:
         K = self._new(self.nnz) if K is None else K
         self._chk(lib().nuto_build_vector_matrix(self.h, *_ptrs(*self._state(u, e, kappa), _f64(R, self.n_rows, "R"),
                                                          _f64(K, self.nnz, "K"))))
+        return R, K
+
+    def build_vector_matrix_noalloc(self, u, e, kappa, R, K):
+        """build_vector_matrix into the caller's R and K without any device allocation (the records go
+        to the library's per-device scratch; at most 29127 hex8 gradient-damage cells)."""
+        self._chk(lib().nuto_build_vector_matrix_noalloc(
+            self.h, *_ptrs(*self._state(u, e, kappa), _f64(R, self.n_rows, "R", optional=False),
+                           _f64(K, self.nnz, "K", optional=False))))
         return R, K
 
     def vjp(self, u, e, kappa, lam, want_theta=True):

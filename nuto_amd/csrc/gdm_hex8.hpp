@@ -1315,25 +1315,30 @@ struct GdmHex8Plan
             hipLaunchKernelGGL((kGdmHex8Node<false, true, FULLDMG, MODE>), dim3(grid_), dim3(64), 0, stream, a, t);
     }
 
-    template <bool FULLDMG>
-    void LaunchT(const RowArgs& a, bool wr, bool wk, hipStream_t stream)
+    // doubles of the pre-pass state records (written in full by kGdmHex8Ip before the node kernel reads them)
+    size_t StateDoubles() const
     {
-        const GdmHex8Tables t = Tables();
+        return prepass ? (size_t)nCells * 8 * gdm8::kSr : 0;
+    }
+
+    template <bool FULLDMG>
+    void LaunchT(const RowArgs& a, bool wr, bool wk, double* state, hipStream_t stream)
+    {
+        GdmHex8Tables t = Tables();
         if (prepass)
         {
+            t.ipState = state;
             const dim3 ipGrid((unsigned)(((long long)nCells * 8 + 63) / 64));
             if (affine)
             {
                 if (nCells > 0)
-                    hipLaunchKernelGGL((kGdmHex8Ip<FULLDMG, true>), ipGrid, dim3(64), 0, stream, a, t,
-                                       ipState.As<double>());
+                    hipLaunchKernelGGL((kGdmHex8Ip<FULLDMG, true>), ipGrid, dim3(64), 0, stream, a, t, state);
                 LaunchNodes<FULLDMG, 2>(a, t, wr, wk, grid, stream);
             }
             else
             {
                 if (nCells > 0)
-                    hipLaunchKernelGGL((kGdmHex8Ip<FULLDMG, false>), ipGrid, dim3(64), 0, stream, a, t,
-                                       ipState.As<double>());
+                    hipLaunchKernelGGL((kGdmHex8Ip<FULLDMG, false>), ipGrid, dim3(64), 0, stream, a, t, state);
                 LaunchNodes<FULLDMG, 1>(a, t, wr, wk, grid, stream);
             }
         }
@@ -1341,14 +1346,17 @@ struct GdmHex8Plan
             LaunchNodes<FULLDMG, 0>(a, t, wr, wk, grid, stream);
     }
 
-    void Launch(const RowArgs& a, bool wr, bool wk, hipStream_t stream)
+    // state: device memory of StateDoubles() doubles for the pre-pass records (nullptr: the plan's own)
+    void Launch(const RowArgs& a, bool wr, bool wk, hipStream_t stream, double* state = nullptr)
     {
         if (nNodes == 0)
             return;
+        if (!state && ipState.p)
+            state = ipState.As<double>();
         if (a.P.damage_application == 0)
-            LaunchT<true>(a, wr, wk, stream);
+            LaunchT<true>(a, wr, wk, state, stream);
         else
-            LaunchT<false>(a, wr, wk, stream);
+            LaunchT<false>(a, wr, wk, state, stream);
     }
 
     // R + K of the nodes [first, end) only (one mesh of a batch), plain block order

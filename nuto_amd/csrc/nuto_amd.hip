@@ -5,6 +5,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -22,6 +23,15 @@
 #include "mesh.hpp"
 
 using namespace nuto_amd;
+
+// Device scratch of nuto_build_vector_matrix_noalloc: the hex8 node kernel's pre-pass state records
+// (gdm8::kSr doubles per cell and IP). Each device holds its own copy from the code object's load to
+// the end of the process; host code reaches it only through the address that hipGetSymbolAddress
+// returns for the current device (ScratchOf).
+constexpr size_t kScratchBytes = size_t(32) << 20;
+constexpr size_t kScratchDoubles = kScratchBytes / sizeof(double);
+constexpr int kScratchMaxCells = (int)(kScratchDoubles / (8 * gdm8::kSr));  // 29127 cells of 8 IPs
+static __device__ __align__(256) double scratch[kScratchDoubles];
 
 namespace
 {
@@ -115,6 +125,32 @@ void Require(bool ok, int code, const std::string& msg)
 {
     if (!ok)
         throw Failure{code, msg};
+}
+
+// Device address of `scratch` per device, resolved once per device at context creation with that
+// device current (ResolveScratch); nullptr until then.
+std::mutex gScratchMu;
+std::vector<double*> gScratch;
+
+hipError_t ResolveScratch(int device)
+{
+    std::lock_guard<std::mutex> lock(gScratchMu);
+    if (device < (int)gScratch.size() && gScratch[device])
+        return hipSuccess;
+    double* p = nullptr;
+    const hipError_t err = hipGetSymbolAddress(reinterpret_cast<void**>(&p), HIP_SYMBOL(scratch));
+    if (err != hipSuccess)
+        return err;
+    if (device >= (int)gScratch.size())
+        gScratch.resize(device + 1, nullptr);
+    gScratch[device] = p;
+    return hipSuccess;
+}
+
+double* ScratchOf(int device)
+{
+    std::lock_guard<std::mutex> lock(gScratchMu);
+    return device < (int)gScratch.size() ? gScratch[device] : nullptr;
 }
 
 template <class F>
@@ -741,7 +777,8 @@ int nuto_create(nuto_ctx** out, int device)
     if (!ctx)
         return NUTO_ERR_MEMORY;
     ctx->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking) != hipSuccess)
+    if (hipSetDevice(device) != hipSuccess || ResolveScratch(device) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking) != hipSuccess)
     {
         delete ctx;
         return NUTO_ERR_DEVICE;
@@ -994,6 +1031,15 @@ int nuto_build_pattern(nuto_ctx* ctx, long long* nnz, int* n_rows)
         ctx->nnz = nz;
         ctx->nRows = (int)(rp.size() - 1);
         ctx->patternBuilt = true;
+        // gradient damage with the integrand and the rule already set: the IP tables and the node
+        // kernel's tables now, so that assemblies (nuto_build_vector_matrix_noalloc) allocate nothing
+        if (ctx->integrandSet && ctx->ruleSet && ctx->params.kind == kGradientDamage &&
+            ctx->params.interaction == 0 && !ForceGeneric())
+        {
+            if (!ctx->tabValid)
+                BuildTables(ctx);
+            PrepareGdm(ctx);
+        }
         if (nnz)
             *nnz = nz;
         if (n_rows)
@@ -1087,6 +1133,34 @@ int nuto_build_vector_matrix(nuto_ctx* ctx, const double* u, const double* e, co
 {
     return Guard(ctx, [&] {
         Assemble(ctx, u, e, kappa, R, K, true, true);
+    });
+}
+
+int nuto_build_vector_matrix_noalloc(nuto_ctx* ctx, const double* u, const double* e, const double* kappa,
+                                     double* R, double* K)
+{
+    return Guard(ctx, [&] {
+        CheckReady(ctx, u, e, kappa);
+        Require(R != nullptr && K != nullptr, NUTO_ERR_ARGUMENT, "R / K is NULL");
+        Require(ctx->params.kind == kGradientDamage && ctx->params.interaction == 0 && !ForceGeneric(),
+                NUTO_ERR_UNSUPPORTED, "the allocation-free assembly is provided for the hex8 gradient-damage node "
+                                      "kernel (constant nonlocal interaction)");
+        Require(ctx->tabValid && ctx->gdmTried, NUTO_ERR_STATE,
+                "node tables not prepared: set the integrand and the rule before nuto_build_pattern, or call "
+                "nuto_build_vector_matrix once");
+        Require(ctx->gdm.Usable(ctx->params.kind, ctx->coordType, ctx->f[0].type, ctx->nip), NUTO_ERR_UNSUPPORTED,
+                "the allocation-free assembly needs hex8 cells and dofs, 2x2x2 Gauss points and at most 8 cells "
+                "and 32 neighbours per node");
+        Require(ctx->gdm.StateDoubles() <= kScratchDoubles, NUTO_ERR_ARGUMENT,
+                "the pre-pass records of " + std::to_string(ctx->nCells) + " cells exceed the device scratch of " +
+                    std::to_string(kScratchBytes >> 20) + " MiB (at most " + std::to_string(kScratchMaxCells) +
+                    " cells)");
+        double* state = ScratchOf(ctx->device);
+        Require(state != nullptr, NUTO_ERR_DEVICE, "device scratch address not resolved");
+        RowArgs a = MakeRowArgs(ctx, u, e, kappa, R, K);
+        ctx->gdm.Launch(a, true, true, ctx->stream, state);
+        Check(hipGetLastError(), "gdm hex8 launch");
+        SyncCheck(ctx, "gdm hex8 kernel");
     });
 }
 
