@@ -259,11 +259,12 @@ def batched(n_meshes, steps, warmup):
     kernels = {}
     for g in range(b.group_info(0)["groups"]):
         gi = b.group_info(g)
-        kernels[names[gi["dof_type"]]] = {1: "gram", 0: "columns", -1: "default"}[gi["cell_kernel"]]
+        kernels[names[gi["dof_type"]]] = {"cell_kernel": {1: "gram", 0: "columns", -1: "default"}[gi["cell_kernel"]],
+                                          "cells": gi["cells"], "stage_cells": gi["stage_cells"]}
     out_bytes = 8 * (sizes["K"] + sizes["R"])
     return {"workload": f"{n_meshes} meshes, cells log-uniform in [1e3, 5e4], types {counts}, jitter 10 %, "
                         f"local isotropic damage R+K", "cells": cells, "nnz": sizes["K"],
-            "output_GBps": out_bytes / (graph_ms * 1e-3) / 1e9, "cell_kernel": kernels,
+            "output_GBps": out_bytes / (graph_ms * 1e-3) / 1e9, "groups": kernels,
             "batched_ms": graph_ms, "per_mesh_launch_ms": per_mesh_ms, "speedup": per_mesh_ms / graph_ms,
             "cells_per_s": cells / (graph_ms * 1e-3), "tuned_mode": best,
             "per_mesh_mode": per_mesh_mode,

@@ -321,15 +321,22 @@ int nuto_batch_add_mesh(nuto_batch* batch, int coord_type, int dof_type, int n_c
  * (2^31 - 1) / (16 * 32) nodes so that each group's CSR stays within int32 offsets (the node kernel
  * takes at most 32 neighbours per node). Takes effect at the next nuto_batch_build. */
 int nuto_batch_set_group_limit(nuto_batch* batch, long long max_dof_nodes);
+/* Staging of the two-phase launches (mode 5; 3D momentum balance; mode 6 stages one mesh at a
+ * time): each group stages the cell
+ * matrices of consecutive meshes up to `bytes` at a time (8 (nd^2 + nd) bytes per cell, nd = 3 x
+ * nodes per cell; a larger mesh alone), so that any batch size runs in bounded memory. 0: the whole
+ * group at once. Default: the whole group when it can be allocated, else 1 GiB; nuto_batch_autotune
+ * chooses per group among the whole group, 1 GiB, 256 MiB and 64 MiB. After nuto_batch_build. */
+int nuto_batch_set_stage_bytes(nuto_batch* batch, long long bytes);
 int nuto_batch_build(nuto_batch* batch); /* patterns of all meshes, once */
 /* sizes[6]: lengths of u, eeq, R, K, history and d_theta */
 int nuto_batch_sizes(nuto_batch* batch, long long* sizes);
 int nuto_batch_mesh_info(nuto_batch* batch, int mesh, long long* info /* 17 */);
 int nuto_batch_copy_pattern(nuto_batch* batch, int mesh, int* row_ptr, int* col);
-/* info[6] of a group: dof element type, meshes, dof nodes, cells, cell-matrix kernel of its
+/* info[7] of a group: dof element type, meshes, dof nodes, cells, cell-matrix kernel of its
  * two-phase launch (1 Gram form, 0 by columns, -1 the element's default until nuto_batch_autotune
- * times both), number of groups */
-int nuto_batch_group_info(nuto_batch* batch, int group, long long* info /* 6 */);
+ * times both), number of groups, cells per staging chunk (0: the whole group, -1: not chosen yet) */
+int nuto_batch_group_info(nuto_batch* batch, int group, long long* info /* 7 */);
 /* per-cell (E, f_t, G_f) over all cells, meshes in the order they were added (host arrays) */
 int nuto_batch_set_material_fields(nuto_batch* batch, const double* E, const double* ft, const double* gf);
 /* R and K of every mesh. mode -1: the tuned launch (mode 3 until tuned) replayed from a HIP graph; 0..2: one fused launch

@@ -135,6 +135,7 @@ def lib():
         L.nuto_batch_add_mesh.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp,
                                           ctypes.c_int, vp, ip]
         L.nuto_batch_set_group_limit.argtypes = [vp, ctypes.c_longlong]
+        L.nuto_batch_set_stage_bytes.argtypes = [vp, ctypes.c_longlong]
         L.nuto_batch_build.argtypes = [vp]
         L.nuto_batch_sizes.argtypes = [vp, vp]
         L.nuto_batch_mesh_info.argtypes = [vp, ctypes.c_int, vp]
@@ -770,6 +771,10 @@ class Batch:
         """Dof nodes per group (super-mesh) of one element pair (nuto_batch_set_group_limit; 0: default)."""
         self._chk(lib().nuto_batch_set_group_limit(self.h, int(max_dof_nodes)))
 
+    def set_stage_bytes(self, nbytes):
+        """Staging budget of each group's two-phase launches (nuto_batch_set_stage_bytes; 0: whole group)."""
+        self._chk(lib().nuto_batch_set_stage_bytes(self.h, int(nbytes)))
+
     def build(self):
         self._chk(lib().nuto_batch_build(self.h))
         sz = np.zeros(6, np.int64)
@@ -784,10 +789,11 @@ class Batch:
 
     def group_info(self, g):
         """A group's dof element type, meshes, dof nodes, cells, cell-matrix kernel (1 Gram form,
-        0 by columns, -1 the element's default) and the number of groups."""
-        info = np.zeros(6, np.int64)
+        0 by columns, -1 the element's default), the number of groups and the cells per staging
+        chunk (0: the whole group, -1: not chosen yet)."""
+        info = np.zeros(7, np.int64)
         self._chk(lib().nuto_batch_group_info(self.h, g, _np_ptr(info)))
-        return dict(zip(("dof_type", "meshes", "dof_nodes", "cells", "cell_kernel", "groups"),
+        return dict(zip(("dof_type", "meshes", "dof_nodes", "cells", "cell_kernel", "groups", "stage_cells"),
                         (int(v) for v in info)))
 
     def pattern(self, m):

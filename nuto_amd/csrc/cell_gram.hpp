@@ -347,7 +347,7 @@ __global__ void __launch_bounds__(NT) kCellGram(const RowArgs A, int c0, int c1,
         {
             const int b = p + q * HALF;
             const double tr = G[q][0] + G[q][4] + G[q][8];
-            double* dst = Ke + ((size_t)cell * ND + 3 * a) * ND + 3 * b;
+            double* dst = Ke + ((size_t)(cell - c0) * ND + 3 * a) * ND + 3 * b;
 #pragma unroll
             for (int i = 0; i < 3; i++)
 #pragma unroll
@@ -366,7 +366,7 @@ __global__ void __launch_bounds__(NT) kCellGram(const RowArgs A, int c0, int c1,
     {
         const int cl = k / (NND * 3), rr = k - cl * NND * 3;
         if (cellBase + cl < c1)
-            Re[(size_t)(cellBase + cl) * ND + rr] = sR[cl][rr];
+            Re[(size_t)(cellBase + cl - c0) * ND + rr] = sR[cl][rr];
     }
 }
 

@@ -239,16 +239,16 @@ __global__ void __launch_bounds__(64) kCellMatrices(const RowArgs A, int c0, int
 
     if (act)
     {
-        double* dst = Ke + (size_t)cell * ND * ND + j;
+        double* dst = Ke + (size_t)(cell - c0) * ND * ND + j;
 #pragma unroll
         for (int r = 0; r < ND; r++)
             dst[(size_t)r * ND] = kcol[r];
-        Re[(size_t)cell * ND + j] = rj;
+        Re[(size_t)(cell - c0) * ND + j] = rj;
     }
 }
 
 template <int NND, bool WR>
-__global__ void __launch_bounds__(64) kGatherRows(const RowArgs A, int p0, int p1, const double* __restrict__ Ke,
+__global__ void __launch_bounds__(64) kGatherRows(const RowArgs A, int p0, int p1, int s0, const double* __restrict__ Ke,
                                                   const double* __restrict__ Re)
 {
     constexpr int ND = NND * 3;
@@ -265,10 +265,10 @@ __global__ void __launch_bounds__(64) kGatherRows(const RowArgs A, int p0, int p
         const int cell = A.incCell[r];
         const int la = A.incLocal[r];
         const unsigned short* sl = A.slot + (size_t)r * NND;
-        const double* src = Ke + ((size_t)cell * ND + la * 3) * ND;
+        const double* src = Ke + ((size_t)(cell - s0) * ND + la * 3) * ND;
         if constexpr (WR)
         {
-            const double* rs = Re + (size_t)cell * ND + la * 3;
+            const double* rs = Re + (size_t)(cell - s0) * ND + la * 3;
             racc[0] += rs[0];
             racc[1] += rs[1];
             racc[2] += rs[2];
@@ -307,7 +307,7 @@ __global__ void __launch_bounds__(64) kGatherRows(const RowArgs A, int p0, int p
 // read with coalesced loads and added at their slots in cell order, then the rows are written with
 // coalesced stores (the rows of p are contiguous in the CSR).
 template <int NND, bool WR>
-__global__ void __launch_bounds__(64) kGatherRowsWave(const RowArgs A, int p0, int p1, const double* __restrict__ Ke,
+__global__ void __launch_bounds__(64) kGatherRowsWave(const RowArgs A, int p0, int p1, int s0, const double* __restrict__ Ke,
                                                       const double* __restrict__ Re)
 {
     constexpr int ND = NND * 3;
@@ -328,7 +328,7 @@ __global__ void __launch_bounds__(64) kGatherRowsWave(const RowArgs A, int p0, i
         const int cell = A.incCell[r];
         const int la = A.incLocal[r];
         const unsigned short* sl = A.slot + (size_t)r * NND;
-        const double* src = Ke + ((size_t)cell * ND + la * 3) * ND;
+        const double* src = Ke + ((size_t)(cell - s0) * ND + la * 3) * ND;
         for (int f = lane; f < 3 * ND; f += 64)
         {
             const int i = f / ND, rem = f - i * ND;
@@ -337,7 +337,7 @@ __global__ void __launch_bounds__(64) kGatherRowsWave(const RowArgs A, int p0, i
             rows[i * L + 3 * q + jj] += src[f];
         }
         if (WR && lane < 3)
-            racc += Re[(size_t)cell * ND + la * 3 + lane];
+            racc += Re[(size_t)(cell - s0) * ND + la * 3 + lane];
         __syncthreads();
     }
     double* dst = A.K + ru0;

@@ -80,7 +80,7 @@ constexpr int sMn = 15;   // -N_a s d eps_eq / d eps (6)
 constexpr int sCg = 21;   // c s g_a (3)
 constexpr int sSn = 24;   // s N_a (1)
 constexpr int sKa = 25;   // UNILATERAL: c2 g_a (3)
-constexpr int kRed = 1160;  // doubles after phase B: partial blocks 64 x kPs, R sums 32, zero block 8, slot map 32
+constexpr int kRed = 1160;  // doubles after phase B: partial blocks 64 x kPs, R sums 32, 8 unused, slot map 32
 
 // value of lane (lane & ~7) + n within each group of 8 lanes (ds_swizzle bit mode:
 // and-mask 0x18 keeps the group, or-mask n selects the lane)
@@ -652,7 +652,7 @@ struct NodeLds
 };
 
 // The rows of node p (one 64-lane workgroup); `state` is the workgroup's LDS (NodeLds::kState doubles):
-// records [i][r][kSt]; after phase B: partial blocks, R sums, zero block and the slot map.
+// records [i][r][kSt]; after phase B: partial blocks, R sums and the slot map.
 template <bool WR, bool WK, bool FULLDMG, int MODE>
 __device__ inline void NodeBody(const RowArgs& A, const GdmHex8Tables& T, int pos, double* state)
 {
@@ -836,17 +836,14 @@ __device__ inline void NodeBody(const RowArgs& A, const GdmHex8Tables& T, int po
     // ---- partial blocks, R sums and the slot map to LDS (the records are dead now) ------------------
     double* part = state;               // [r][b][kPs]
     double* rcell = state + 8 * 8 * kPs;  // [4][r]: R terms summed over the IPs of cell r
-    double* zero = rcell + 32;          // 8 zeros: the block of an absent (cell, slot) pair
     // slotOf[q][r] = local index b of neighbour q in cell r, 0xFF if q is not a node of cell r
-    unsigned char(*slotOf)[kMaxInc] = reinterpret_cast<unsigned char(*)[kMaxInc]>(zero + 8);
+    unsigned char(*slotOf)[kMaxInc] = reinterpret_cast<unsigned char(*)[kMaxInc]>(rcell + 40);
     if (WK)
     {
         reinterpret_cast<unsigned int*>(slotOf)[lane] = 0xFFFFFFFFu;  // 64 x 4 bytes = the whole map
 #pragma unroll
         for (int i = 0; i < 16; i++)
             part[lane * kPs + i] = acc[i];
-        if (lane < 8)
-            zero[lane] = 0.0;
     }
     if (WR && j == 0)
     {
@@ -860,12 +857,16 @@ __device__ inline void NodeBody(const RowArgs& A, const GdmHex8Tables& T, int po
     __syncthreads();
 
     // ---- reduce: lane (q, half) sums its 8 block entries over the cells in order and writes them ----
-    // (absent cells add the zero block, so every lane runs the same 8 steps). Row layout: the three
+    // (only the cells that hold q are read: the lanes of absent (cell, q) pairs sit out that step, so
+    // the LDS reads are the 64 partial blocks, not 8 per neighbour). Row layout: the three
     // displacement rows of p are consecutive in the CSR (4 nA columns each), then the eeq row.
     if (WK)
     {
         const int q = lane >> 1, h = lane & 1;
         double sum[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            sum[k] = 0.0;
         if (q < nA)
         {
             const unsigned long long row = *reinterpret_cast<const unsigned long long*>(&slotOf[q][0]);
@@ -873,10 +874,13 @@ __device__ inline void NodeBody(const RowArgs& A, const GdmHex8Tables& T, int po
             for (int rr = 0; rr < 8; rr++)
             {
                 const unsigned b = (unsigned)(row >> (8 * rr)) & 0xFFu;
-                const double* src = b == 0xFFu ? zero : part + (rr * 8 + (int)b) * kPs + h * 8;
+                if (b != 0xFFu)
+                {
+                    const double* src = part + (rr * 8 + (int)b) * kPs + h * 8;
 #pragma unroll
-                for (int k = 0; k < 8; k++)
-                    sum[k] = rr == 0 ? src[k] : sum[k] + src[k];
+                    for (int k = 0; k < 8; k++)
+                        sum[k] += src[k];
+                }
             }
         }
         __syncthreads();  // the rows below overwrite the partial blocks

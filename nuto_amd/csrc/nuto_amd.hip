@@ -336,9 +336,10 @@ bool TwoPhaseOff()
 }
 
 // staging for the cell matrices of cells [0, nCells) (grown on demand, kept with the context)
-void EnsureStage(nuto_ctx* ctx, int nd)
+// staging of `cells` cell matrices (nd x nd) and vectors; it only grows
+void EnsureStage(nuto_ctx* ctx, int nd, long long cells)
 {
-    const size_t ke = (size_t)ctx->nCells * nd * nd * sizeof(double), re = (size_t)ctx->nCells * nd * sizeof(double);
+    const size_t ke = (size_t)cells * nd * nd * sizeof(double), re = (size_t)cells * nd * sizeof(double);
     if (ctx->keStage.bytes < ke)
         Check(ctx->keStage.Alloc(ke), "cell matrix staging");
     if (ctx->reStage.bytes < re)
@@ -346,11 +347,11 @@ void EnsureStage(nuto_ctx* ctx, int nd)
 }
 
 // the staging is there or could be allocated (a failed allocation clears the error state)
-bool TryStage(nuto_ctx* ctx, int nd)
+bool TryStage(nuto_ctx* ctx, int nd, long long cells)
 {
     try
     {
-        EnsureStage(ctx, nd);
+        EnsureStage(ctx, nd, cells);
         return true;
     }
     catch (const Failure&)
@@ -422,7 +423,7 @@ void LaunchTwoPhaseT(nuto_ctx* ctx, const RowArgs& a, bool wr, int c0, int c1, i
     constexpr int ND = NND * 3;
     if (c1 <= c0 || p1 <= p0)
         return;
-    EnsureStage(ctx, ND);
+    EnsureStage(ctx, ND, c1 - c0);  // cell c's matrix at staging position c - c0
     double* ke = ctx->keStage.As<double>();
     double* re = ctx->reStage.As<double>();
     // the Gram form wins on the hexahedra (32 and 200 lanes per cell); the column kernel is as fast or
@@ -443,18 +444,18 @@ void LaunchTwoPhaseT(nuto_ctx* ctx, const RowArgs& a, bool wr, int c0, int c1, i
         // one wave per node, rows summed in LDS
         if (wr)
             hipLaunchKernelGGL((kGatherRowsWave<NND, true>), dim3((unsigned)(p1 - p0)), dim3(64), lds, stream, a, p0, p1,
-                               ke, re);
+                               c0, ke, re);
         else
             hipLaunchKernelGGL((kGatherRowsWave<NND, false>), dim3((unsigned)(p1 - p0)), dim3(64), lds, stream, a, p0,
-                               p1, ke, re);
+                               p1, c0, ke, re);
     }
     else
     {
         const dim3 grid((unsigned)((p1 - p0 + 63) / 64));
         if (wr)
-            hipLaunchKernelGGL((kGatherRows<NND, true>), grid, dim3(64), 0, stream, a, p0, p1, ke, re);
+            hipLaunchKernelGGL((kGatherRows<NND, true>), grid, dim3(64), 0, stream, a, p0, p1, c0, ke, re);
         else
-            hipLaunchKernelGGL((kGatherRows<NND, false>), grid, dim3(64), 0, stream, a, p0, p1, ke, re);
+            hipLaunchKernelGGL((kGatherRows<NND, false>), grid, dim3(64), 0, stream, a, p0, p1, c0, ke, re);
     }
     Check(hipGetLastError(), "two-phase assembly launch");
 }
@@ -468,7 +469,7 @@ void LaunchRowsT(nuto_ctx* ctx, const RowArgs& a, bool wr, bool wk)
     if constexpr (DIM == 3 && KIND != kGradientDamage && NND * 3 <= 64)
     {
         // the single-pass kernel serves when the staging does not fit in device memory
-        if (wk && !TwoPhaseOff() && TryStage(ctx, NND * 3))
+        if (wk && !TwoPhaseOff() && TryStage(ctx, NND * 3, a.nCells))
         {
             LaunchTwoPhaseT<KIND, NNC, NND, NIP>(ctx, a, wr, 0, a.nCells, 0, n, ctx->stream);
             SyncCheck(ctx, "assembly kernel");
@@ -1641,6 +1642,7 @@ struct BatchGroup
     int nNodes = 0, nCells = 0, nCoord = 0, nip = 0, nRows = 0;
     long long nnz = 0;
     long long uOff = 0, eOff = 0, ROff = 0, KOff = 0, hOff = 0, tOff = 0;  // offsets into the batch arrays
+    long long stageCells = -1;  // two-phase chunk: cells whose matrices are staged at once (0: all; -1: default)
 };
 
 // default rule of each (coordinate, dof) element pair: tensor Gauss 2 / 3, tetrahedron order 1 / 2
@@ -1836,6 +1838,69 @@ bool IsGdm(const nuto_batch* b)
     return b->params.kind == kGradientDamage;
 }
 
+// cells of a two-phase chunk of `bytes` of staging (matrix and vector per cell)
+long long StageCellsOf(const BatchGroup& g, size_t bytes)
+{
+    const size_t nd = 3 * (size_t)NodesOf(g.dt);
+    return std::max<long long>(1, (long long)(bytes / ((nd * nd + nd) * sizeof(double))));
+}
+
+// default chunk: the whole group when its staging can be allocated, else 1 GiB of staging
+long long DefaultStageCells(BatchGroup& g)
+{
+    if (TryStage(g.ctx, 3 * NodesOf(g.dt), g.nCells))
+        return 0;
+    return StageCellsOf(g, size_t(1) << 30);
+}
+
+// cells of the largest chunk LaunchTwoPhaseGroup forms with the group's stageCells
+long long MaxChunkCells(const nuto_batch* b, const BatchGroup& g)
+{
+    if (g.stageCells <= 0 || g.nCells <= g.stageCells)
+        return g.nCells;
+    long long best = 0;
+    size_t i = 0;
+    while (i < g.meshes.size())
+    {
+        long long cells = b->meshes[g.meshes[i]].nCells;
+        size_t j = i + 1;
+        while (j < g.meshes.size() && cells + b->meshes[g.meshes[j]].nCells <= g.stageCells)
+            cells += b->meshes[g.meshes[j++]].nCells;
+        best = std::max(best, cells);
+        i = j;
+    }
+    return best;
+}
+
+// Two-phase launches of a group in chunks of consecutive meshes whose cells fit stageCells (0: the
+// whole group at once; a mesh larger than the chunk is a chunk of its own). The rows of a mesh only
+// touch its own cells, so each chunk's staging is consumed by its gather before the next chunk
+// rewrites it (same stream).
+template <int KIND>
+void LaunchTwoPhaseGroup(nuto_batch* b, BatchGroup& g, const RowArgs& a)
+{
+    if (g.stageCells < 0)
+        g.stageCells = DefaultStageCells(g);
+    if (g.stageCells == 0 || g.nCells <= g.stageCells)
+    {
+        LaunchTwoPhaseType<KIND>(g.ctx, a, g.type, 0, g.nCells, 0, g.nNodes, b->stream);
+        return;
+    }
+    size_t i = 0;
+    while (i < g.meshes.size())
+    {
+        const BatchMesh& first = b->meshes[g.meshes[i]];
+        long long cells = first.nCells;
+        size_t j = i + 1;
+        while (j < g.meshes.size() && cells + b->meshes[g.meshes[j]].nCells <= g.stageCells)
+            cells += b->meshes[g.meshes[j++]].nCells;
+        const BatchMesh& last = b->meshes[g.meshes[j - 1]];
+        LaunchTwoPhaseType<KIND>(g.ctx, a, g.type, first.cellOff, last.cellOff + last.nCells, first.nodeOff,
+                                 last.nodeOff + last.nDof, b->stream);
+        i = j;
+    }
+}
+
 // The kernels of one batched R + K assembly in launch mode `mode` (no synchronisation).
 void LaunchBatch(nuto_batch* b, int mode, const double* u, const double* e, const double* kappa, double* R, double* K)
 {
@@ -1863,9 +1928,9 @@ void LaunchBatch(nuto_batch* b, int mode, const double* u, const double* e, cons
                 if (mode == 5)
                 {
                     if (kind == kLinearElastic)
-                        LaunchTwoPhaseType<kLinearElastic>(g.ctx, a, g.type, 0, g.nCells, 0, g.nNodes, b->stream);
+                        LaunchTwoPhaseGroup<kLinearElastic>(b, g, a);
                     else
-                        LaunchTwoPhaseType<kLocalDamage>(g.ctx, a, g.type, 0, g.nCells, 0, g.nNodes, b->stream);
+                        LaunchTwoPhaseGroup<kLocalDamage>(b, g, a);
                     continue;
                 }
                 for (int m : g.meshes)
@@ -2080,6 +2145,17 @@ int nuto_batch_set_group_limit(nuto_batch* b, long long max_dof_nodes)
     });
 }
 
+int nuto_batch_set_stage_bytes(nuto_batch* b, long long bytes)
+{
+    return GuardB(b, [&] {
+        Require(b->built, NUTO_ERR_STATE, "batch not built (nuto_batch_build)");
+        Require(bytes >= 0, NUTO_ERR_ARGUMENT, "negative staging budget");
+        for (BatchGroup& g : b->groups)
+            g.stageCells = bytes == 0 ? 0 : StageCellsOf(g, (size_t)bytes);
+        DropGraph(b);
+    });
+}
+
 int nuto_batch_build(nuto_batch* b)
 {
     return GuardB(b, [&] {
@@ -2234,6 +2310,7 @@ int nuto_batch_group_info(nuto_batch* b, int group, long long* info)
         info[3] = g.nCells;
         info[4] = g.ctx->cellGram;
         info[5] = (long long)b->groups.size();
+        info[6] = g.stageCells;
     });
 }
 
@@ -2388,28 +2465,59 @@ int nuto_batch_autotune(nuto_batch* b, const double* u, const double* e, const d
             std::sort(rep.begin(), rep.end());
             return (double)rep[rep.size() / 2];
         };
-        // the two-phase modes need the staging of every group; without it they are not candidates
-        bool staged = !IsGdm(b);
-        for (BatchGroup& g : b->groups)
-            staged = staged && TryStage(g.ctx, 3 * NodesOf(g.dt));
+        // the two-phase modes (not for gradient damage): each group's cell-matrix kernel (Gram form or by
+        // columns) and staging chunk (the whole group when it fits, 1 GiB, 256 MiB, 64 MiB of staging),
+        // timed on the group's two-phase launch
+        const bool staged = !IsGdm(b);
         if (staged)
         {
-            // each group's cell-matrix kernel (Gram form or by columns), timed on the group's two-phase launch
             for (BatchGroup& g : b->groups)
             {
                 const RowArgs a = GroupArgs(b, g, u, e, kappa, R, K);
-                double t[2];
-                for (int gram = 0; gram < 2; gram++)
+                const int nd = 3 * NodesOf(g.dt);
+                std::vector<long long> chunks;
+                if (TryStage(g.ctx, nd, g.nCells))
+                    chunks.push_back(0);
+                for (size_t mib : {1024, 256, 64})
                 {
-                    g.ctx->cellGram = gram;
-                    t[gram] = median([&] {
-                        if (b->params.kind == kLinearElastic)
-                            LaunchTwoPhaseType<kLinearElastic>(g.ctx, a, g.type, 0, g.nCells, 0, g.nNodes, b->stream);
-                        else
-                            LaunchTwoPhaseType<kLocalDamage>(g.ctx, a, g.type, 0, g.nCells, 0, g.nNodes, b->stream);
-                    });
+                    const long long c = StageCellsOf(g, mib << 20);
+                    if (c < g.nCells && TryStage(g.ctx, nd, c))
+                        chunks.push_back(c);
                 }
-                g.ctx->cellGram = t[1] <= t[0] ? 1 : 0;
+                Require(!chunks.empty(), NUTO_ERR_MEMORY, "no staging for the two-phase assembly");
+                double bestT = 1e300;
+                int bestGram = 1;
+                long long bestChunk = chunks[0];
+                for (int gram = 0; gram < 2; gram++)
+                    for (long long c : chunks)
+                    {
+                        g.ctx->cellGram = gram;
+                        g.stageCells = c;
+                        const double t = median([&] {
+                            if (b->params.kind == kLinearElastic)
+                                LaunchTwoPhaseGroup<kLinearElastic>(b, g, a);
+                            else
+                                LaunchTwoPhaseGroup<kLocalDamage>(b, g, a);
+                        });
+                        if (t < bestT)
+                        {
+                            bestT = t;
+                            bestGram = gram;
+                            bestChunk = c;
+                        }
+                    }
+                g.ctx->cellGram = bestGram;
+                g.stageCells = bestChunk;
+                // staging of the chosen chunk only, allocated here: a graph capture of the tuned launch
+                // must not allocate
+                const long long keep = MaxChunkCells(b, g);
+                if (g.ctx->keStage.bytes > (size_t)keep * nd * nd * sizeof(double))
+                {
+                    Check(hipStreamSynchronize(b->stream), "staging release");
+                    g.ctx->keStage.Release();
+                    g.ctx->reStage.Release();
+                }
+                EnsureStage(g.ctx, nd, keep);
             }
         }
         double bestMs = 1e300;

@@ -104,13 +104,34 @@ def test_graph_replays_match_eager(mixed):
 
 
 def test_autotune_picks_a_batched_mode(mixed):
-    _, b, sizes, u, kap = mixed
+    """A batch of its own (tuning changes each group's cell kernel and staging chunk): the tuned graph
+    replays the chosen mode bit for bit, and every group reports its choices."""
+    meshes, _, _, u, kap = mixed
+    b, sizes = build(meshes, N.local_damage_params())
     ms, best = b.autotune(to_dev(u), None, to_dev(kap))
     assert best in (0, 1, 2, 3, 5) and all(v > 0 for v in ms)
+    for g in range(b.group_info(0)["groups"]):
+        gi = b.group_info(g)
+        assert gi["cell_kernel"] in (0, 1) and gi["stage_cells"] >= 0
     for rep in range(3):
         R, K = b.assemble(to_dev(u), None, to_dev(kap), mode=-1)
         ref = b.assemble(to_dev(u), None, to_dev(kap), mode=best)
         assert torch.equal(R, ref[0]) and torch.equal(K, ref[1])
+
+
+def test_staging_chunks_same_results(mixed):
+    """Two-phase launches over chunks of meshes (bounded staging) reproduce the whole-group launch
+    bit for bit: every mesh alone (1 byte), a few meshes per chunk, and the whole group."""
+    meshes, _, _, u, kap = mixed
+    b, sizes = build(meshes, N.local_damage_params())
+    b.set_stage_bytes(0)
+    ref = [t.cpu().numpy() for t in b.assemble(to_dev(u), None, to_dev(kap), mode=5)]
+    for nbytes in (1, 200_000, 2_000_000, 0):
+        b.set_stage_bytes(nbytes)
+        chunks = {b.group_info(g)["stage_cells"] for g in range(b.group_info(0)["groups"])}
+        assert (chunks == {0}) == (nbytes == 0)
+        R, K = (t.cpu().numpy() for t in b.assemble(to_dev(u), None, to_dev(kap), mode=5))
+        assert np.array_equal(R, ref[0]) and np.array_equal(K, ref[1]), nbytes
 
 
 def test_group_limit_splits_types_same_results(mixed):
