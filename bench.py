@@ -259,7 +259,8 @@ def batched(n_meshes, steps, warmup):
     kernels = {}
     for g in range(b.group_info(0)["groups"]):
         gi = b.group_info(g)
-        kernels[names[gi["dof_type"]]] = {"cell_kernel": {1: "gram", 0: "columns", -1: "default"}[gi["cell_kernel"]],
+        kernels[names[gi["dof_type"]]] = {"cell_kernel": {0: "columns", 1: "gram, 1 row node per lane", 2: "gram, 2 row nodes per lane",
+                                                         -1: "default"}[gi["cell_kernel"]],
                                           "cells": gi["cells"], "stage_cells": gi["stage_cells"]}
     out_bytes = 8 * (sizes["K"] + sizes["R"])
     return {"workload": f"{n_meshes} meshes, cells log-uniform in [1e3, 5e4], types {counts}, jitter 10 %, "

@@ -334,8 +334,9 @@ int nuto_batch_sizes(nuto_batch* batch, long long* sizes);
 int nuto_batch_mesh_info(nuto_batch* batch, int mesh, long long* info /* 17 */);
 int nuto_batch_copy_pattern(nuto_batch* batch, int mesh, int* row_ptr, int* col);
 /* info[7] of a group: dof element type, meshes, dof nodes, cells, cell-matrix kernel of its
- * two-phase launch (1 Gram form, 0 by columns, -1 the element's default until nuto_batch_autotune
- * times both), number of groups, cells per staging chunk (0: the whole group, -1: not chosen yet) */
+ * two-phase launch (0 by columns, 1 / 2 Gram form with one / two row nodes per lane, -1 the
+ * element's default until nuto_batch_autotune times each), number of groups, cells per staging
+ * chunk (0: the whole group, -1: not chosen yet) */
 int nuto_batch_group_info(nuto_batch* batch, int group, long long* info /* 7 */);
 /* per-cell (E, f_t, G_f) over all cells, meshes in the order they were added (host arrays) */
 int nuto_batch_set_material_fields(nuto_batch* batch, const double* E, const double* ft, const double* gf);

@@ -34,13 +34,17 @@ namespace gram
 constexpr int kIpRec = 30;  // per IP: J^-1 (9), s, c1, c2, sigma (6), t (6), d eps_eq / d eps (6)
 }
 
-// CPW cells per workgroup of NT threads; lanes [0, CPW * NND * NND / 2) own blocks
-template <int KIND, int NNC, int NND, int NIP, int CPW, int NT, int ICH, int UNR, bool TWO>
+// CPW cells per workgroup of NT threads; lanes [0, CPW * LPC) own blocks. AP = 1: a lane owns row
+// node a and column nodes p, p + NND/2 (LPC = NND^2 / 2 lanes per cell); AP = 2: row nodes a, a +
+// NND/2 as well (LPC = NND^2 / 4), four blocks per lane from the same g and u, v reads (fewer LDS
+// reads per block, more registers)
+template <int KIND, int NNC, int NND, int NIP, int CPW, int NT, int ICH, int UNR, int AP, bool TWO>
 __global__ void __launch_bounds__(NT) kCellGram(const RowArgs A, int c0, int c1, double* __restrict__ Ke,
                                                 double* __restrict__ Re)
 {
     using namespace gram;
-    constexpr int ND = 3 * NND, HALF = NND / 2, LPC = NND * HALF;
+    constexpr int ND = 3 * NND, HALF = NND / 2, LPC = (AP == 2 ? HALF : NND) * HALF;
+    static_assert(AP == 1 || AP == 2, "one or two row nodes per lane");
     constexpr int S = 1 + NNC * 3 + NND + NND * 3;  // IP table row: w, dNc, N, dNd
     constexpr bool LD = KIND == kLocalDamage;
     static_assert(NND % 2 == 0, "column nodes come in pairs");
@@ -50,8 +54,10 @@ __global__ void __launch_bounds__(NT) kCellGram(const RowArgs A, int c0, int c1,
     __shared__ double sU[CPW][NND * 3];
     __shared__ double sJG[CPW][NIP][18];
     __shared__ double sIp[CPW][NIP][kIpRec];
-    __shared__ __align__(16) double sG[CPW][ICH][NND][4];
-    __shared__ __align__(16) double sUV[CPW][ICH][NND][LD ? 8 : 1];
+    // per (node, IP): g (0..2), u (3..5), v (6..8); the odd record length (3 or 9 doubles) puts the
+    // records that the lanes of a wave read at once (consecutive nodes) on distinct LDS banks
+    constexpr int NR = LD ? 9 : 3;
+    __shared__ double sGU[CPW][ICH][NND][NR];
     __shared__ double sRb[CPW][ICH][NND][3];
     __shared__ double sR[CPW][NND * 3];
 
@@ -185,25 +191,27 @@ __global__ void __launch_bounds__(NT) kCellGram(const RowArgs A, int c0, int c1,
         }
     }
 
-    // block lane: cell bc, row node a, column nodes b1 = p, b2 = p + HALF
+    // block lane: cell bc, row nodes a (+ HALF when AP = 2), column nodes b1 = p, b2 = p + HALF
     const bool blk = t < CPW * LPC;
     const int bc = blk ? t / LPC : 0;
     const int a = blk ? (t - bc * LPC) / HALF : 0;
     const int p = blk ? t - bc * LPC - a * HALF : 0;
-    double G[2][9], G1[2][TWO ? 9 : 1], R1[2][LD ? 9 : 1];
+    double G[AP][2][9], G1[AP][2][TWO ? 9 : 1], R1[AP][2][LD ? 9 : 1];
 #pragma unroll
-    for (int q = 0; q < 2; q++)
-    {
+    for (int sa = 0; sa < AP; sa++)
 #pragma unroll
-        for (int k = 0; k < 9; k++)
+        for (int q = 0; q < 2; q++)
         {
-            G[q][k] = 0.0;
-            if constexpr (TWO)
-                G1[q][k] = 0.0;
-            if constexpr (LD)
-                R1[q][k] = 0.0;
+#pragma unroll
+            for (int k = 0; k < 9; k++)
+            {
+                G[sa][q][k] = 0.0;
+                if constexpr (TWO)
+                    G1[sa][q][k] = 0.0;
+                if constexpr (LD)
+                    R1[sa][q][k] = 0.0;
+            }
         }
-    }
 
 #pragma unroll 1
     for (int ip0 = 0; ip0 < NIP; ip0 += ICH)
@@ -222,7 +230,7 @@ __global__ void __launch_bounds__(NT) kCellGram(const RowArgs A, int c0, int c1,
 #pragma unroll
             for (int c = 0; c < 3; c++)
                 g[c] = dN[0] * rec[c] + dN[1] * rec[3 + c] + dN[2] * rec[6 + c];
-            double* gd = sG[cl][i][n];
+            double* gd = sGU[cl][i][n];
             gd[0] = g[0];
             gd[1] = g[1];
             gd[2] = g[2];
@@ -233,16 +241,15 @@ __global__ void __launch_bounds__(NT) kCellGram(const RowArgs A, int c0, int c1,
             sRb[cl][i][n][2] = rec[9] * bt[2];
             if constexpr (LD)
             {
-                double* uv = sUV[cl][i][n];
                 double w[3];
                 BTv3(g, rec + 18, w);
-                uv[0] = w[0];
-                uv[1] = w[1];
-                uv[2] = w[2];
+                gd[3] = w[0];
+                gd[4] = w[1];
+                gd[5] = w[2];
                 BTv3(g, rec + 24, w);
-                uv[4] = w[0];
-                uv[5] = w[1];
-                uv[6] = w[2];
+                gd[6] = w[0];
+                gd[7] = w[1];
+                gd[8] = w[2];
             }
         }
         __syncthreads();
@@ -257,7 +264,7 @@ __global__ void __launch_bounds__(NT) kCellGram(const RowArgs A, int c0, int c1,
                     acc += sRb[cl][i][r / 3][r % 3];
             sR[cl][r] = acc;
         }
-        // ---- blocks (a, b1), (a, b2) over the chunk's IPs -------------------------------------------
+        // ---- blocks (a, b1), (a, b2) [and (a + HALF, b1), (a + HALF, b2)] over the chunk's IPs --------
         if (blk)
         {
 #pragma unroll UNR
@@ -267,56 +274,67 @@ __global__ void __launch_bounds__(NT) kCellGram(const RowArgs A, int c0, int c1,
                     break;
                 const double* rec = sIp[bc][ip0 + i];
                 const double cw1 = rec[10], cw2 = rec[11];
-                const double* ga = sG[bc][i][a];
-                const double ha[3] = {cw2 * ga[0], cw2 * ga[1], cw2 * ga[2]};
-                double h1[3];
-                if constexpr (TWO)
+                double ha[AP][3], h1[AP][TWO ? 3 : 1], ua[AP][LD ? 3 : 1];
+#pragma unroll
+                for (int sa = 0; sa < AP; sa++)
                 {
-                    h1[0] = cw1 * ga[0];
-                    h1[1] = cw1 * ga[1];
-                    h1[2] = cw1 * ga[2];
-                }
-                double ua[3];
-                if constexpr (LD)
-                {
-                    const double* uv = sUV[bc][i][a];
-                    ua[0] = uv[0];
-                    ua[1] = uv[1];
-                    ua[2] = uv[2];
+                    const double* ga = sGU[bc][i][a + sa * HALF];
+#pragma unroll
+                    for (int r = 0; r < 3; r++)
+                    {
+                        ha[sa][r] = cw2 * ga[r];
+                        if constexpr (TWO)
+                            h1[sa][r] = cw1 * ga[r];
+                    }
+                    if constexpr (LD)
+                    {
+                        ua[sa][0] = ga[3];
+                        ua[sa][1] = ga[4];
+                        ua[sa][2] = ga[5];
+                    }
                 }
 #pragma unroll
                 for (int q = 0; q < 2; q++)
                 {
                     const int b = p + q * HALF;
-                    const double* gb = sG[bc][i][b];
+                    const double* gb = sGU[bc][i][b];
                     const double g0 = gb[0], g1 = gb[1], g2 = gb[2];
-#pragma unroll
-                    for (int r = 0; r < 3; r++)
-                    {
-                        G[q][3 * r + 0] = fma(ha[r], g0, G[q][3 * r + 0]);
-                        G[q][3 * r + 1] = fma(ha[r], g1, G[q][3 * r + 1]);
-                        G[q][3 * r + 2] = fma(ha[r], g2, G[q][3 * r + 2]);
-                    }
-                    if constexpr (TWO)
-                    {
-#pragma unroll
-                        for (int r = 0; r < 3; r++)
-                        {
-                            G1[q][3 * r + 0] = fma(h1[r], g0, G1[q][3 * r + 0]);
-                            G1[q][3 * r + 1] = fma(h1[r], g1, G1[q][3 * r + 1]);
-                            G1[q][3 * r + 2] = fma(h1[r], g2, G1[q][3 * r + 2]);
-                        }
-                    }
+                    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
                     if constexpr (LD)
                     {
-                        const double* vb = sUV[bc][i][b] + 4;
-                        const double v0 = vb[0], v1 = vb[1], v2 = vb[2];
+                        v0 = gb[6];
+                        v1 = gb[7];
+                        v2 = gb[8];
+                    }
+#pragma unroll
+                    for (int sa = 0; sa < AP; sa++)
+                    {
 #pragma unroll
                         for (int r = 0; r < 3; r++)
                         {
-                            R1[q][3 * r + 0] = fma(ua[r], v0, R1[q][3 * r + 0]);
-                            R1[q][3 * r + 1] = fma(ua[r], v1, R1[q][3 * r + 1]);
-                            R1[q][3 * r + 2] = fma(ua[r], v2, R1[q][3 * r + 2]);
+                            G[sa][q][3 * r + 0] = fma(ha[sa][r], g0, G[sa][q][3 * r + 0]);
+                            G[sa][q][3 * r + 1] = fma(ha[sa][r], g1, G[sa][q][3 * r + 1]);
+                            G[sa][q][3 * r + 2] = fma(ha[sa][r], g2, G[sa][q][3 * r + 2]);
+                        }
+                        if constexpr (TWO)
+                        {
+#pragma unroll
+                            for (int r = 0; r < 3; r++)
+                            {
+                                G1[sa][q][3 * r + 0] = fma(h1[sa][r], g0, G1[sa][q][3 * r + 0]);
+                                G1[sa][q][3 * r + 1] = fma(h1[sa][r], g1, G1[sa][q][3 * r + 1]);
+                                G1[sa][q][3 * r + 2] = fma(h1[sa][r], g2, G1[sa][q][3 * r + 2]);
+                            }
+                        }
+                        if constexpr (LD)
+                        {
+#pragma unroll
+                            for (int r = 0; r < 3; r++)
+                            {
+                                R1[sa][q][3 * r + 0] = fma(ua[sa][r], v0, R1[sa][q][3 * r + 0]);
+                                R1[sa][q][3 * r + 1] = fma(ua[sa][r], v1, R1[sa][q][3 * r + 1]);
+                                R1[sa][q][3 * r + 2] = fma(ua[sa][r], v2, R1[sa][q][3 * r + 2]);
+                            }
                         }
                     }
                 }
@@ -343,24 +361,26 @@ __global__ void __launch_bounds__(NT) kCellGram(const RowArgs A, int c0, int c1,
             }
         }
 #pragma unroll
-        for (int q = 0; q < 2; q++)
-        {
-            const int b = p + q * HALF;
-            const double tr = G[q][0] + G[q][4] + G[q][8];
-            double* dst = Ke + ((size_t)(cell - c0) * ND + 3 * a) * ND + 3 * b;
+        for (int sa = 0; sa < AP; sa++)
 #pragma unroll
-            for (int i = 0; i < 3; i++)
+            for (int q = 0; q < 2; q++)
+            {
+                const int b = p + q * HALF;
+                const double tr = G[sa][q][0] + G[sa][q][4] + G[sa][q][8];
+                double* dst = Ke + ((size_t)(cell - c0) * ND + 3 * (a + sa * HALF)) * ND + 3 * b;
 #pragma unroll
-                for (int j = 0; j < 3; j++)
-                {
-                    double v = (TWO ? G1[q][TWO ? 3 * i + j : 0] : r * G[q][3 * i + j]) + G[q][3 * j + i];
-                    if (i == j)
-                        v += tr;
-                    if constexpr (LD)
-                        v += R1[q][3 * i + j];
-                    dst[(size_t)i * ND + j] = v;
-                }
-        }
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++)
+                    {
+                        double v = (TWO ? G1[sa][q][TWO ? 3 * i + j : 0] : r * G[sa][q][3 * i + j]) + G[sa][q][3 * j + i];
+                        if (i == j)
+                            v += tr;
+                        if constexpr (LD)
+                            v += R1[sa][q][3 * i + j];
+                        dst[(size_t)i * ND + j] = v;
+                    }
+            }
     }
     for (int k = t; k < CPW * NND * 3; k += NT)
     {

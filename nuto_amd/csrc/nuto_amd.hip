@@ -108,7 +108,8 @@ struct nuto_ctx
     // staged cell matrices and vectors of the two-phase generic assembly
     DevBuf keStage, reStage;
     int maxAdj = -1;  // largest node neighbourhood (LDS rows of the wave gather), from the node graph
-    int cellGram = -1;  // cell-matrix kernel of the two-phase assembly: 1 Gram form, 0 by columns, -1 per element
+    int cellGram = -1;  // cell-matrix kernel of the two-phase assembly: 0 by columns, 1 / 2 Gram form with 1 / 2
+                        // row nodes per lane, -1 the element's default
 };
 
 namespace
@@ -363,58 +364,88 @@ bool TryStage(nuto_ctx* ctx, int nd, long long cells)
 
 // Two-phase assembly of the cells [c0, c1) and the rows of the dof nodes [p0, p1) (the rows'
 // incident cells must lie in [c0, c1): a whole mesh of a batch, or the whole group).
-// NUTO_AMD_CELL_GRAM=0 computes the cell matrices column by column (kCellMatrices) instead of in
-// Gram form (kCellGram)
-bool CellGramOff()
+// NUTO_AMD_CELL_GRAM=0 / 1 / 2 forces the cell-matrix form of every two-phase launch: by columns
+// (kCellMatrices), Gram form with one or two row nodes per lane (kCellGram); -1 when unset
+int CellFormEnv()
 {
     const char* v = std::getenv("NUTO_AMD_CELL_GRAM");
-    return v && v[0] == '0';
+    return v && v[0] >= '0' && v[0] <= '2' ? v[0] - '0' : -1;
 }
 
-// kCellGram's shape per element: cells per workgroup, threads, IPs per chunk, unroll of the IP loop
-// of the blocks (3 only for hex20, whose long chunk hides the LDS latency; the others lose occupancy)
-template <int NND>
+// kCellGram's shape per element and row nodes per lane (AP): cells per workgroup, threads, IPs per
+// chunk, unroll of the IP loop of the blocks (3 only for hex20 with AP = 1, whose long chunk hides
+// the LDS latency; the others lose occupancy)
+template <int NND, int AP>
 struct GramShape;
 template <>
-struct GramShape<4>  // tet4: 8 lanes per cell
+struct GramShape<4, 1>  // tet4: 8 lanes per cell
 {
     static constexpr int CPW = 8, NT = 64, ICH = 1, UNR = 1;
 };
 template <>
-struct GramShape<10>  // tet10: 50 lanes per cell
+struct GramShape<10, 1>  // tet10: 50 lanes per cell
 {
     static constexpr int CPW = 2, NT = 128, ICH = 4, UNR = 1;
 };
 template <>
-struct GramShape<8>  // hex8: 32 lanes per cell
+struct GramShape<8, 1>  // hex8: 32 lanes per cell
 {
     static constexpr int CPW = 2, NT = 64, ICH = 4, UNR = 1;
 };
 template <>
-struct GramShape<20>  // hex20: 200 lanes per cell
+struct GramShape<20, 1>  // hex20: 200 lanes per cell
 {
     static constexpr int CPW = 1, NT = 256, ICH = 9, UNR = 3;
 };
-
-template <int KIND, int NNC, int NND, int NIP>
-void LaunchCellMatrices(const RowArgs& a, bool gram, int c0, int c1, double* ke, double* re, hipStream_t stream)
+template <>
+struct GramShape<4, 2>  // tet4: 4 lanes per cell
 {
-    if (!gram || CellGramOff())
+    static constexpr int CPW = 16, NT = 64, ICH = 1, UNR = 1;
+};
+template <>
+struct GramShape<10, 2>  // tet10: 25 lanes per cell
+{
+    static constexpr int CPW = 2, NT = 64, ICH = 4, UNR = 1;
+};
+template <>
+struct GramShape<8, 2>  // hex8: 16 lanes per cell
+{
+    static constexpr int CPW = 4, NT = 64, ICH = 4, UNR = 1;
+};
+template <>
+struct GramShape<20, 2>  // hex20: 100 lanes per cell
+{
+    static constexpr int CPW = 2, NT = 256, ICH = 9, UNR = 1;
+};
+
+template <int KIND, int NNC, int NND, int NIP, int AP, bool TWO>
+void LaunchGram(const RowArgs& a, int c0, int c1, double* ke, double* re, hipStream_t stream)
+{
+    using Sh = GramShape<NND, AP>;
+    const dim3 grid((unsigned)((c1 - c0 + Sh::CPW - 1) / Sh::CPW));
+    hipLaunchKernelGGL((kCellGram<KIND, NNC, NND, NIP, Sh::CPW, Sh::NT, Sh::ICH, Sh::UNR, AP, TWO>), grid,
+                       dim3(Sh::NT), 0, stream, a, c0, c1, ke, re);
+}
+
+// form: 0 by columns (kCellMatrices), 1 Gram form with one row node per lane, 2 with two (four
+// accumulated blocks per lane; UNILATERAL damage application, which needs a second Gram, keeps one)
+template <int KIND, int NNC, int NND, int NIP>
+void LaunchCellMatrices(const RowArgs& a, int form, int c0, int c1, double* ke, double* re, hipStream_t stream)
+{
+    if (form == 0)
     {
         constexpr int CPW = 64 / (NND * 3);
         hipLaunchKernelGGL((kCellMatrices<KIND, NNC, NND, NIP>), dim3((unsigned)((c1 - c0 + CPW - 1) / CPW)), dim3(64),
                            0, stream, a, c0, c1, ke, re);
         return;
     }
-    using Sh = GramShape<NND>;
-    const dim3 grid((unsigned)((c1 - c0 + Sh::CPW - 1) / Sh::CPW));
     // one Gram unless UNILATERAL damage application makes c1 / c2 vary between IPs
     if (KIND == kLocalDamage && a.P.damage_application != 0)
-        hipLaunchKernelGGL((kCellGram<KIND, NNC, NND, NIP, Sh::CPW, Sh::NT, Sh::ICH, Sh::UNR, true>), grid, dim3(Sh::NT), 0,
-                           stream, a, c0, c1, ke, re);
+        LaunchGram<KIND, NNC, NND, NIP, 1, true>(a, c0, c1, ke, re, stream);
+    else if (form == 2)
+        LaunchGram<KIND, NNC, NND, NIP, 2, false>(a, c0, c1, ke, re, stream);
     else
-        hipLaunchKernelGGL((kCellGram<KIND, NNC, NND, NIP, Sh::CPW, Sh::NT, Sh::ICH, Sh::UNR, false>), grid, dim3(Sh::NT), 0,
-                           stream, a, c0, c1, ke, re);
+        LaunchGram<KIND, NNC, NND, NIP, 1, false>(a, c0, c1, ke, re, stream);
 }
 
 template <int KIND, int NNC, int NND, int NIP>
@@ -427,9 +458,10 @@ void LaunchTwoPhaseT(nuto_ctx* ctx, const RowArgs& a, bool wr, int c0, int c1, i
     double* ke = ctx->keStage.As<double>();
     double* re = ctx->reStage.As<double>();
     // the Gram form wins on the hexahedra (32 and 200 lanes per cell); the column kernel is as fast or
-    // faster on the tetrahedra, whose cell matrices are small (a batch's autotune measures both)
-    const bool gram = ctx->cellGram >= 0 ? ctx->cellGram == 1 : NND >= 8;
-    LaunchCellMatrices<KIND, NNC, NND, NIP>(a, gram, c0, c1, ke, re, stream);
+    // faster on the tetrahedra, whose cell matrices are small (a batch's autotune measures every form)
+    const int env = CellFormEnv();
+    const int form = env >= 0 ? env : (ctx->cellGram >= 0 ? ctx->cellGram : (NND >= 8 ? 1 : 0));
+    LaunchCellMatrices<KIND, NNC, NND, NIP>(a, form, c0, c1, ke, re, stream);
     if (ctx->maxAdj < 0)
     {
         int m = 0;
@@ -2465,8 +2497,8 @@ int nuto_batch_autotune(nuto_batch* b, const double* u, const double* e, const d
             std::sort(rep.begin(), rep.end());
             return (double)rep[rep.size() / 2];
         };
-        // the two-phase modes (not for gradient damage): each group's cell-matrix kernel (Gram form or by
-        // columns) and staging chunk (the whole group when it fits, 1 GiB, 256 MiB, 64 MiB of staging),
+        // the two-phase modes (not for gradient damage): each group's cell-matrix kernel (by columns, Gram
+        // form with one or two row nodes per lane) and staging chunk (the whole group when it fits, 1 GiB, 256 MiB, 64 MiB of staging),
         // timed on the group's two-phase launch
         const bool staged = !IsGdm(b);
         if (staged)
@@ -2488,7 +2520,7 @@ int nuto_batch_autotune(nuto_batch* b, const double* u, const double* e, const d
                 double bestT = 1e300;
                 int bestGram = 1;
                 long long bestChunk = chunks[0];
-                for (int gram = 0; gram < 2; gram++)
+                for (int gram = 0; gram < 3; gram++)
                     for (long long c : chunks)
                     {
                         g.ctx->cellGram = gram;

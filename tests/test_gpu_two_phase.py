@@ -1,8 +1,9 @@
 """The two forms of the generic 3D matrix assembly agree with the oracle and with each other:
-the two-phase kernels (cell matrices, then the row gather, with the wave-per-node LDS gather or
-the lane-per-node gather that serves nodes whose rows exceed the LDS budget) and the row-owner
-kernel (NUTO_AMD_TWO_PHASE=0), for linear elasticity and local damage on tet4, tet10, hex8 and
-hex20 (jittered meshes, per-cell material fields), at the parity bar of the other tests."""
+the two-phase kernels (cell matrices by columns or in Gram form with one or two row nodes per lane,
+NUTO_AMD_CELL_GRAM=0 / 1 / 2, then the row gather, with the wave-per-node LDS gather or the
+lane-per-node gather that serves nodes whose rows exceed the LDS budget) and the row-owner kernel
+(NUTO_AMD_TWO_PHASE=0), for linear elasticity and local damage on tet4, tet10, hex8 and hex20
+(jittered meshes, per-cell material fields), at the parity bar of the other tests."""
 import os
 
 import numpy as np
@@ -38,7 +39,9 @@ def test_two_phase_forms(ct, dt, kind):
     p = Pair(ct, dt, X, C, *(le_params(3) if kind == "le" else ld_params()), False)
     u, _, kap = p.inputs(u_range=4e-4, k_range=1.5e-4)
     ref = p.orc.assemble(u, None, kap if kind == "ld" else None)
-    forms = {"wave": {}, "lane": {"NUTO_AMD_GATHER_LANE": "1"}, "row-owner": {"NUTO_AMD_TWO_PHASE": "0"}}
+    forms = {"wave": {}, "lane": {"NUTO_AMD_GATHER_LANE": "1"}, "row-owner": {"NUTO_AMD_TWO_PHASE": "0"},
+             "columns": {"NUTO_AMD_CELL_GRAM": "0"}, "gram1": {"NUTO_AMD_CELL_GRAM": "1"},
+             "gram2": {"NUTO_AMD_CELL_GRAM": "2"}}
     out = {}
     for name, env in forms.items():
         R, K = assemble(p, u, kap, env)
@@ -60,18 +63,19 @@ def test_two_phase_with_material_fields():
     gf = rng.uniform(0.05, 0.15, p.ncells)
     p.orc.set_theta(E, ft, gf)
     ref = p.orc.assemble(u, None, kap)
-    for env in ({}, {"NUTO_AMD_TWO_PHASE": "0"}):
-        old = os.environ.get("NUTO_AMD_TWO_PHASE")
+    for env in ({}, {"NUTO_AMD_TWO_PHASE": "0"}, {"NUTO_AMD_CELL_GRAM": "2"}, {"NUTO_AMD_CELL_GRAM": "0"}):
+        old = {k: os.environ.get(k) for k in env}
         os.environ.update(env)
         try:
             ctx = p.context()
             ctx.set_material_fields(E, ft, gf)
             R, K = ctx.build_vector_matrix(to_dev(u), None, to_dev(kap))
         finally:
-            if old is None:
-                os.environ.pop("NUTO_AMD_TWO_PHASE", None)
-            else:
-                os.environ["NUTO_AMD_TWO_PHASE"] = old
+            for k, v in old.items():
+                if v is None:
+                    os.environ.pop(k, None)
+                else:
+                    os.environ[k] = v
         check_values(R.cpu().numpy(), ref["R"], ref["R_bound"], f"R {env}")
         check_values(K.cpu().numpy(), ref["val"], ref["bound"], f"K {env}")
 
@@ -83,6 +87,7 @@ def test_two_phase_unilateral_damage(ct, dt):
     p = Pair(ct, dt, X, C, *ld_params(N.UNILATERAL), False)
     u, _, kap = p.inputs(u_range=6e-4, k_range=2e-4)
     ref = p.orc.assemble(u, None, kap)
-    R, K = assemble(p, u, kap, {})
-    check_values(R, ref["R"], ref["R_bound"], "R")
-    check_values(K, ref["val"], ref["bound"], "K")
+    for form in ("1", "2", "0"):  # the second Gram keeps one row node per lane for form 2
+        R, K = assemble(p, u, kap, {"NUTO_AMD_CELL_GRAM": form})
+        check_values(R, ref["R"], ref["R_bound"], f"R (form {form})")
+        check_values(K, ref["val"], ref["bound"], f"K (form {form})")
