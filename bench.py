@@ -208,10 +208,12 @@ def pcie_inclusive(ctx, u, e, kap, R, K, reps=3):
             "note": "pinned host buffers; copies, the assembly and copies back in one timed region"}
 
 
-def batched(n_meshes, steps, warmup):
+def batched(n_meshes, steps, warmup, form=-1, tune=True):
     """Mixed-type batch: meshes with log-uniform element counts in [1e3, 5e4], tet4 / tet10 / hex8 /
     hex20 drawn uniformly, jittered grids (10 % of the spacing), local isotropic damage. Compares the
-    tuned batched launch (graph replay) with one launch per mesh on the same data."""
+    tuned batched launch (graph replay) with one launch per mesh on the same data. form >= 0 forces
+    the cell-matrix kernel of the two-phase launches (0 columns, 1 / 2 Gram, 3 Gram on MFMA); without
+    tune the two-phase launch per element type (mode 5, whole groups) is timed as it is."""
     import numpy as np
     import torch
 
@@ -221,6 +223,7 @@ def batched(n_meshes, steps, warmup):
 
     t0 = time.time()
     meshes = W.mixed_batch(n_meshes)
+    print(f"batch: {n_meshes} meshes generated ({time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
     b = N.Batch()
     b.set_integrand(N.local_damage_params())
     for ct, dt, X, C, ranks, nn in meshes:
@@ -232,7 +235,17 @@ def batched(n_meshes, steps, warmup):
     R = torch.empty(sizes["R"], dtype=torch.float64, device="cuda")
     K = torch.empty(sizes["K"], dtype=torch.float64, device="cuda")
     setup = time.time() - t0
-    ms_modes, best = b.autotune(u, None, kap, R, K)
+    print(f"batch: patterns built ({setup:.1f} s)", file=sys.stderr, flush=True)
+    if form >= 0:
+        os.environ["NUTO_AMD_CELL_GRAM"] = str(form)
+    t1 = time.time()
+    if tune:
+        ms_modes, best = b.autotune(u, None, kap, R, K)
+    else:
+        b.set_stage_bytes(0)
+        ms_modes, best = [-1.0] * 7, 5
+    tune_s = time.time() - t1
+    print(f"batch: tuned in {tune_s:.1f} s (mode {best})", file=sys.stderr, flush=True)
 
     def timed(mode):
         for _ in range(warmup):
@@ -246,9 +259,9 @@ def batched(n_meshes, steps, warmup):
         torch.cuda.synchronize()
         return float(np.median([a.elapsed_time(z) for a, z in evs]))
 
-    graph_ms = timed(-1)
+    graph_ms = timed(-1 if tune else 5)
     # the plain per-mesh launches: the faster of the two kernel forms, one mesh at a time
-    per_mesh_mode = 6 if 0 <= ms_modes[6] < ms_modes[4] else 4
+    per_mesh_mode = 6 if 0 <= ms_modes[6] < ms_modes[4] or not tune else 4
     per_mesh_ms = timed(per_mesh_mode)
     counts = {}
     for ct, dt, *_ in meshes:
@@ -256,12 +269,13 @@ def batched(n_meshes, steps, warmup):
              (N.HEX8, N.HEX20): "hex20"}[(ct, dt)]
         counts[k] = counts.get(k, 0) + 1
     names = {N.TET4: "tet4", N.TET10: "tet10", N.HEX8: "hex8", N.HEX20: "hex20"}
-    kernels = {}
+    forms = {0: "columns", 1: "gram, 1 row node per lane", 2: "gram, 2 row nodes per lane",
+             3: "gram on the matrix cores (MFMA)", -1: "default"}
+    kernels = []
     for g in range(b.group_info(0)["groups"]):
         gi = b.group_info(g)
-        kernels[names[gi["dof_type"]]] = {"cell_kernel": {0: "columns", 1: "gram, 1 row node per lane", 2: "gram, 2 row nodes per lane",
-                                                         -1: "default"}[gi["cell_kernel"]],
-                                          "cells": gi["cells"], "stage_cells": gi["stage_cells"]}
+        kernels.append({"type": names[gi["dof_type"]], "meshes": gi["meshes"], "cells": gi["cells"],
+                        "cell_kernel": forms[gi["cell_kernel"]], "stage_cells": gi["stage_cells"]})
     out_bytes = 8 * (sizes["K"] + sizes["R"])
     return {"workload": f"{n_meshes} meshes, cells log-uniform in [1e3, 5e4], types {counts}, jitter 10 %, "
                         f"local isotropic damage R+K", "cells": cells, "nnz": sizes["K"],
@@ -271,7 +285,8 @@ def batched(n_meshes, steps, warmup):
             "per_mesh_mode": per_mesh_mode,
             "autotune_ms": {"fused64": ms_modes[0], "fused128": ms_modes[1], "fused256": ms_modes[2],
                             "per_type": ms_modes[3], "per_mesh": ms_modes[4], "two_phase_per_type": ms_modes[5],
-                            "two_phase_per_mesh": ms_modes[6]}, "setup_s": setup}
+                            "two_phase_per_mesh": ms_modes[6]}, "setup_s": setup, "autotune_s": tune_s,
+            "forced_form": form if form >= 0 else None}
 
 
 def batched_adjoint(n_meshes, steps, warmup):
@@ -411,6 +426,10 @@ def main():
     ap.add_argument("--solve-iters", type=int, default=20,
                     help="BiCGSTAB iterations of the constrained solve on the assembled system (0: skip)")
     ap.add_argument("--batch-meshes", type=int, default=256, help="meshes of the mixed-type batch (0: skip)")
+    ap.add_argument("--batch-form", type=int, default=-1,
+                    help="force the cell-matrix kernel of the mixed batch (0 columns, 1 / 2 Gram, 3 Gram on MFMA)")
+    ap.add_argument("--batch-no-tune", action="store_true",
+                    help="time the mixed batch's two-phase launch per element type without the autotune")
     ap.add_argument("--batch-gdm-meshes", type=int, default=256,
                     help="meshes of the gradient-damage residual + adjoint batch (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -517,7 +536,8 @@ def main():
         rec["adjoint"] = adjoint_ratio(args.adjoint_n, args.steps, args.warmup)
         torch.cuda.empty_cache()
     if args.batch_meshes:
-        rec["batched"] = batched(args.batch_meshes, min(args.steps, 10), args.warmup)
+        rec["batched"] = batched(args.batch_meshes, min(args.steps, 10), args.warmup, args.batch_form,
+                                 not args.batch_no_tune)
         torch.cuda.empty_cache()
     if args.batch_gdm_meshes:
         rec["batched_adjoint"] = batched_adjoint(args.batch_gdm_meshes, min(args.steps, 10), args.warmup)

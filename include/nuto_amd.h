@@ -318,8 +318,9 @@ int nuto_batch_add_mesh(nuto_batch* batch, int coord_type, int dof_type, int n_c
 /* The meshes of one element pair, in the order added, fill groups (super-meshes with one CSR, one
  * launch each) of at most max_dof_nodes dof nodes; a larger mesh forms a group of its own. 0 (the
  * default): one group per element pair, except gradient damage, whose groups hold at most
- * (2^31 - 1) / (16 * 32) nodes so that each group's CSR stays within int32 offsets (the node kernel
- * takes at most 32 neighbours per node). Takes effect at the next nuto_batch_build. */
+ * (2^31 - 1) / (16 * 32) nodes (the node kernel takes at most 32 neighbours per node). In every
+ * case a group closes before its CSR would exceed 2^31 - 1 nonzeros (int32 offsets). Takes effect
+ * at the next nuto_batch_build. */
 int nuto_batch_set_group_limit(nuto_batch* batch, long long max_dof_nodes);
 /* Staging of the two-phase launches (mode 5; 3D momentum balance; mode 6 stages one mesh at a
  * time): each group stages the cell
@@ -334,8 +335,9 @@ int nuto_batch_sizes(nuto_batch* batch, long long* sizes);
 int nuto_batch_mesh_info(nuto_batch* batch, int mesh, long long* info /* 17 */);
 int nuto_batch_copy_pattern(nuto_batch* batch, int mesh, int* row_ptr, int* col);
 /* info[7] of a group: dof element type, meshes, dof nodes, cells, cell-matrix kernel of its
- * two-phase launch (0 by columns, 1 / 2 Gram form with one / two row nodes per lane, -1 the
- * element's default until nuto_batch_autotune times each), number of groups, cells per staging
+ * two-phase launch (0 by columns, 1 / 2 Gram form with one / two row nodes per lane, 3 Gram form
+ * on the FP64 matrix cores, -1 the element's default until nuto_batch_autotune times each),
+ * number of groups, cells per staging
  * chunk (0: the whole group, -1: not chosen yet) */
 int nuto_batch_group_info(nuto_batch* batch, int group, long long* info /* 7 */);
 /* per-cell (E, f_t, G_f) over all cells, meshes in the order they were added (host arrays) */

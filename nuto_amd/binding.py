@@ -789,8 +789,9 @@ class Batch:
 
     def group_info(self, g):
         """A group's dof element type, meshes, dof nodes, cells, cell-matrix kernel (0 by columns,
-        1 / 2 Gram form with one / two row nodes per lane, -1 the element's default), the number of
-        groups and the cells per staging chunk (0: the whole group, -1: not chosen yet)."""
+        1 / 2 Gram form with one / two row nodes per lane, 3 Gram form on the matrix cores, -1 the
+        element's default), the number of groups and the cells per staging chunk (0: the whole
+        group, -1: not chosen yet)."""
         info = np.zeros(7, np.int64)
         self._chk(lib().nuto_batch_group_info(self.h, g, _np_ptr(info)))
         return dict(zip(("dof_type", "meshes", "dof_nodes", "cells", "cell_kernel", "groups", "stage_cells"),

@@ -34,38 +34,34 @@ namespace gram
 constexpr int kIpRec = 30;  // per IP: J^-1 (9), s, c1, c2, sigma (6), t (6), d eps_eq / d eps (6)
 }
 
-// CPW cells per workgroup of NT threads; lanes [0, CPW * LPC) own blocks. AP = 1: a lane owns row
-// node a and column nodes p, p + NND/2 (LPC = NND^2 / 2 lanes per cell); AP = 2: row nodes a, a +
-// NND/2 as well (LPC = NND^2 / 4), four blocks per lane from the same g and u, v reads (fewer LDS
-// reads per block, more registers)
-template <int KIND, int NNC, int NND, int NIP, int CPW, int NT, int ICH, int UNR, int AP, bool TWO>
-__global__ void __launch_bounds__(NT) kCellGram(const RowArgs A, int c0, int c1, double* __restrict__ Ke,
-                                                double* __restrict__ Re)
+// per-cell material (E, kappa0, beta; per-cell fields when given)
+__device__ inline void CellMaterial(const RowArgs& A, int cell, double& E, double& k0, double& beta)
+{
+    E = A.P.E;
+    k0 = A.P.kappa0;
+    beta = A.P.beta;
+    if (A.thE)
+    {
+        E = A.thE[cell];
+        k0 = A.thFt[cell] / E;
+        beta = A.thFt[cell] / A.thGf[cell];
+    }
+}
+
+// Phase 1 of the cell-matrix kernels for the cells cellBase .. cellBase + CPW - 1 (clamped to c1 - 1)
+// on NT threads: coordinates and displacements to LDS, J and the reference displacement gradient of
+// every IP, then each IP's state record (gram::kIpRec doubles: J^-1, s = detJ w, c1, c2, sigma, t,
+// d eps_eq / d eps). Ends with the records written (no barrier after them).
+template <int KIND, int NNC, int NND, int NIP, int CPW, int NT>
+__device__ inline void CellIpRecords(const RowArgs& A, int c1, int cellBase, double (*sX)[NNC * 3],
+                                     double (*sU)[NND * 3], double (*sJG)[NIP][18],
+                                     double (*sIp)[NIP][gram::kIpRec])
 {
     using namespace gram;
-    constexpr int ND = 3 * NND, HALF = NND / 2, LPC = (AP == 2 ? HALF : NND) * HALF;
-    static_assert(AP == 1 || AP == 2, "one or two row nodes per lane");
     constexpr int S = 1 + NNC * 3 + NND + NND * 3;  // IP table row: w, dNc, N, dNd
-    constexpr bool LD = KIND == kLocalDamage;
-    static_assert(NND % 2 == 0, "column nodes come in pairs");
-    static_assert(CPW * LPC <= NT, "a lane per (cell, a, pair)");
-
-    __shared__ double sX[CPW][NNC * 3];
-    __shared__ double sU[CPW][NND * 3];
-    __shared__ double sJG[CPW][NIP][18];
-    __shared__ double sIp[CPW][NIP][kIpRec];
-    // per (node, IP): g (0..2), u (3..5), v (6..8); the odd record length (3 or 9 doubles) puts the
-    // records that the lanes of a wave read at once (consecutive nodes) on distinct LDS banks
-    constexpr int NR = LD ? 9 : 3;
-    __shared__ double sGU[CPW][ICH][NND][NR];
-    __shared__ double sRb[CPW][ICH][NND][3];
-    __shared__ double sR[CPW][NND * 3];
-
     const int t = (int)threadIdx.x;
-    const int cellBase = c0 + (int)blockIdx.x * CPW;
     const LawParams& P = A.P;
-
-    // ---- cell data: coordinates, displacements; R accumulators ---------------------------------
+    // ---- cell data: coordinates, displacements ------------------------------------------------------
     for (int k = t; k < CPW * NNC * 3; k += NT)
     {
         const int cl = k / (NNC * 3), r = k - cl * NNC * 3;
@@ -80,21 +76,8 @@ __global__ void __launch_bounds__(NT) kCellGram(const RowArgs A, int c0, int c1,
         const int cell = cellBase + cl < c1 ? cellBase + cl : c1 - 1;
         const int q = A.dconn[(size_t)cell * NND + r / 3];
         sU[cl][r] = A.u[(size_t)q * 3 + r % 3];
-        sR[cl][r] = 0.0;
     }
 
-    // per-cell material (E, kappa0, beta; per-cell fields when given) for the IP lanes
-    auto material = [&](int cell, double& E, double& k0, double& beta) {
-        E = P.E;
-        k0 = P.kappa0;
-        beta = P.beta;
-        if (A.thE)
-        {
-            E = A.thE[cell];
-            k0 = A.thFt[cell] / E;
-            beta = A.thFt[cell] / A.thGf[cell];
-        }
-    };
 
     __syncthreads();  // cell data loaded
     // ---- J = X^T dNc and the reference displacement gradient U^T dNd of every IP ----------------
@@ -152,7 +135,7 @@ __global__ void __launch_bounds__(NT) kCellGram(const RowArgs A, int c0, int c1,
                     rec[3 * r + m] = inv[r][m];
             rec[9] = s;
             double E, k0, beta;
-            material(cell, E, k0, beta);
+            CellMaterial(A, cell, E, k0, beta);
             if constexpr (KIND == kLinearElastic)
             {
                 const double f = E / ((1.0 + P.nu) * (1.0 - 2.0 * P.nu));
@@ -190,6 +173,42 @@ __global__ void __launch_bounds__(NT) kCellGram(const RowArgs A, int c0, int c1,
             }
         }
     }
+}
+
+// CPW cells per workgroup of NT threads; lanes [0, CPW * LPC) own blocks. AP = 1: a lane owns row
+// node a and column nodes p, p + NND/2 (LPC = NND^2 / 2 lanes per cell); AP = 2: row nodes a, a +
+// NND/2 as well (LPC = NND^2 / 4), four blocks per lane from the same g and u, v reads (fewer LDS
+// reads per block, more registers)
+template <int KIND, int NNC, int NND, int NIP, int CPW, int NT, int ICH, int UNR, int AP, bool TWO>
+__global__ void __launch_bounds__(NT) kCellGram(const RowArgs A, int c0, int c1, double* __restrict__ Ke,
+                                                double* __restrict__ Re)
+{
+    using namespace gram;
+    constexpr int ND = 3 * NND, HALF = NND / 2, LPC = (AP == 2 ? HALF : NND) * HALF;
+    static_assert(AP == 1 || AP == 2, "one or two row nodes per lane");
+    constexpr int S = 1 + NNC * 3 + NND + NND * 3;  // IP table row: w, dNc, N, dNd
+    constexpr bool LD = KIND == kLocalDamage;
+    static_assert(NND % 2 == 0, "column nodes come in pairs");
+    static_assert(CPW * LPC <= NT, "a lane per (cell, a, pair)");
+
+    __shared__ double sX[CPW][NNC * 3];
+    __shared__ double sU[CPW][NND * 3];
+    __shared__ double sJG[CPW][NIP][18];
+    __shared__ double sIp[CPW][NIP][kIpRec];
+    // per (node, IP): g (0..2), u (3..5), v (6..8); the odd record length (3 or 9 doubles) puts the
+    // records that the lanes of a wave read at once (consecutive nodes) on distinct LDS banks
+    constexpr int NR = LD ? 9 : 3;
+    __shared__ double sGU[CPW][ICH][NND][NR];
+    __shared__ double sRb[CPW][ICH][NND][3];
+    __shared__ double sR[CPW][NND * 3];
+
+    const int t = (int)threadIdx.x;
+    const int cellBase = c0 + (int)blockIdx.x * CPW;
+    const LawParams& P = A.P;
+
+    for (int k = t; k < CPW * NND * 3; k += NT)
+        sR[k / (NND * 3)][k % (NND * 3)] = 0.0;
+    CellIpRecords<KIND, NNC, NND, NIP, CPW, NT>(A, c1, cellBase, sX, sU, sJG, sIp);
 
     // block lane: cell bc, row nodes a (+ HALF when AP = 2), column nodes b1 = p, b2 = p + HALF
     const bool blk = t < CPW * LPC;
@@ -351,7 +370,7 @@ __global__ void __launch_bounds__(NT) kCellGram(const RowArgs A, int c0, int c1,
         if constexpr (!TWO)
         {
             double E, k0, beta;
-            material(cell, E, k0, beta);
+            CellMaterial(A, cell, E, k0, beta);
             if constexpr (KIND == kLinearElastic)
                 r = 2.0 * P.nu / (1.0 - 2.0 * P.nu);  // lambda / mu
             else

@@ -18,6 +18,7 @@
 #include "batch.hpp"
 #include "cell_fields.hpp"
 #include "cell_gram.hpp"
+#include "cell_mfma.hpp"
 #include "cell_matrices.hpp"
 #include "solve.hpp"
 #include "mesh.hpp"
@@ -109,7 +110,7 @@ struct nuto_ctx
     DevBuf keStage, reStage;
     int maxAdj = -1;  // largest node neighbourhood (LDS rows of the wave gather), from the node graph
     int cellGram = -1;  // cell-matrix kernel of the two-phase assembly: 0 by columns, 1 / 2 Gram form with 1 / 2
-                        // row nodes per lane, -1 the element's default
+                        // row nodes per lane, 3 Gram form on the matrix cores, -1 the element's default
 };
 
 namespace
@@ -364,12 +365,13 @@ bool TryStage(nuto_ctx* ctx, int nd, long long cells)
 
 // Two-phase assembly of the cells [c0, c1) and the rows of the dof nodes [p0, p1) (the rows'
 // incident cells must lie in [c0, c1): a whole mesh of a batch, or the whole group).
-// NUTO_AMD_CELL_GRAM=0 / 1 / 2 forces the cell-matrix form of every two-phase launch: by columns
-// (kCellMatrices), Gram form with one or two row nodes per lane (kCellGram); -1 when unset
+// NUTO_AMD_CELL_GRAM=0 / 1 / 2 / 3 forces the cell-matrix form of every two-phase launch: by columns
+// (kCellMatrices), Gram form with one or two row nodes per lane (kCellGram), Gram form on the
+// matrix cores (kCellMfma); -1 when unset
 int CellFormEnv()
 {
     const char* v = std::getenv("NUTO_AMD_CELL_GRAM");
-    return v && v[0] >= '0' && v[0] <= '2' ? v[0] - '0' : -1;
+    return v && v[0] >= '0' && v[0] <= '3' ? v[0] - '0' : -1;
 }
 
 // kCellGram's shape per element and row nodes per lane (AP): cells per workgroup, threads, IPs per
@@ -428,7 +430,9 @@ void LaunchGram(const RowArgs& a, int c0, int c1, double* ke, double* re, hipStr
 }
 
 // form: 0 by columns (kCellMatrices), 1 Gram form with one row node per lane, 2 with two (four
-// accumulated blocks per lane; UNILATERAL damage application, which needs a second Gram, keeps one)
+// accumulated blocks per lane), 3 Gram form on the matrix cores (kCellMfma). UNILATERAL damage
+// application, which needs a second Gram, runs forms 2 and 3 as 1.
+constexpr int kCellForms = 4;
 template <int KIND, int NNC, int NND, int NIP>
 void LaunchCellMatrices(const RowArgs& a, int form, int c0, int c1, double* ke, double* re, hipStream_t stream)
 {
@@ -442,6 +446,9 @@ void LaunchCellMatrices(const RowArgs& a, int form, int c0, int c1, double* ke, 
     // one Gram unless UNILATERAL damage application makes c1 / c2 vary between IPs
     if (KIND == kLocalDamage && a.P.damage_application != 0)
         LaunchGram<KIND, NNC, NND, NIP, 1, true>(a, c0, c1, ke, re, stream);
+    else if (form == 3)
+        hipLaunchKernelGGL((kCellMfma<KIND, NNC, NND, NIP>), dim3((unsigned)(c1 - c0)), dim3(MfmaShape<NND>::NT), 0,
+                           stream, a, c0, c1, ke, re);
     else if (form == 2)
         LaunchGram<KIND, NNC, NND, NIP, 2, false>(a, c0, c1, ke, re, stream);
     else
@@ -2198,6 +2205,10 @@ int nuto_batch_build(nuto_batch* b)
         // pair, in the order they were added, fill groups of at most `cap` dof nodes (a larger mesh
         // forms a group of its own)
         const long long cap = b->groupNodes > 0 ? b->groupNodes : (gdm ? kGdmGroupNodes : LLONG_MAX);
+        // a group's CSR has int32 offsets: when the type's cells could exceed 2^31 - 1 nonzeros (each
+        // cell adds at most (components x nodes)^2), the meshes' exact counts close groups in time
+        const long long kMaxNnz = 2147483647LL;
+        const int nc = gdm ? 4 : 3;  // components per dof node
         const int order[4][2] = {{kHex8, kHex20}, {kTet4, kTet10}, {kHex8, kHex8}, {kTet4, kTet4}};
         std::vector<BatchGroup> plan;
         for (const auto& ctdt : order)
@@ -2207,21 +2218,41 @@ int nuto_batch_build(nuto_batch* b)
             g.dt = ctdt[1];
             int rk, rn;
             BatchTypeOf(g.ct, g.dt, g.type, rk, rn);
-            long long nodes = 0;
+            const long long nd = (long long)nc * NodesOf(g.dt);
+            long long bound = 0;
+            for (const BatchMesh& bm : b->meshes)
+                if (bm.ct == g.ct && bm.dt == g.dt)
+                    bound += (long long)bm.nCells * nd * nd;
+            const bool exact = bound > kMaxNnz;
+            long long nodes = 0, nnz = 0;
             for (size_t m = 0; m < b->meshes.size(); m++)
             {
-                if (b->meshes[m].ct != g.ct || b->meshes[m].dt != g.dt)
+                const BatchMesh& bm = b->meshes[m];
+                if (bm.ct != g.ct || bm.dt != g.dt)
                     continue;
                 Require(!gdm || (g.ct == kHex8 && g.dt == kHex8), NUTO_ERR_UNSUPPORTED,
                         "gradient-damage batches hold hex8 meshes only");
-                if (!g.meshes.empty() && nodes + b->meshes[m].nDof > cap)
+                long long mnz = 0;
+                if (exact)
+                {
+                    NodeGraph mg;
+                    std::string err;
+                    Require(BuildNodeGraph(bm.nCells, NodesOf(bm.dt), bm.ranks.data(), bm.nDof, mg, err),
+                            NUTO_ERR_ARGUMENT, err);
+                    mnz = (long long)nc * nc * (long long)mg.adj.size();
+                    Require(mnz <= kMaxNnz, NUTO_ERR_UNSUPPORTED,
+                            "a mesh of the batch exceeds 2^31 - 1 nonzeros (int32 CSR offsets)");
+                }
+                if (!g.meshes.empty() && (nodes + bm.nDof > cap || nnz + mnz > kMaxNnz))
                 {
                     plan.push_back(g);
                     g.meshes.clear();
                     nodes = 0;
+                    nnz = 0;
                 }
                 g.meshes.push_back((int)m);
-                nodes += b->meshes[m].nDof;
+                nodes += bm.nDof;
+                nnz += mnz;
             }
             if (!g.meshes.empty())
                 plan.push_back(g);
@@ -2480,12 +2511,15 @@ int nuto_batch_autotune(nuto_batch* b, const double* u, const double* e, const d
         hipEvent_t t0, t1;
         Check(hipEventCreate(&t0), "event");
         Check(hipEventCreate(&t1), "event");
-        // median of 5 timed runs after one warm-up
+        // median of 5 timed runs after one warm-up; a launch of more than 50 ms is timed once (large
+        // batches: the spread is small against the run and the tuning stays within seconds)
         auto median = [&](auto&& launch) {
             launch();
             std::vector<float> rep;
             for (int it = 0; it < 5; it++)
             {
+                if (it == 1 && rep[0] > 50.0f)
+                    break;
                 Check(hipEventRecord(t0, b->stream), "event");
                 launch();
                 Check(hipEventRecord(t1, b->stream), "event");
@@ -2498,7 +2532,7 @@ int nuto_batch_autotune(nuto_batch* b, const double* u, const double* e, const d
             return (double)rep[rep.size() / 2];
         };
         // the two-phase modes (not for gradient damage): each group's cell-matrix kernel (by columns, Gram
-        // form with one or two row nodes per lane) and staging chunk (the whole group when it fits, 1 GiB, 256 MiB, 64 MiB of staging),
+        // form with one or two row nodes per lane or on the matrix cores) and staging chunk (the whole group when it fits, 1 GiB, 256 MiB, 64 MiB of staging),
         // timed on the group's two-phase launch
         const bool staged = !IsGdm(b);
         if (staged)
@@ -2520,7 +2554,7 @@ int nuto_batch_autotune(nuto_batch* b, const double* u, const double* e, const d
                 double bestT = 1e300;
                 int bestGram = 1;
                 long long bestChunk = chunks[0];
-                for (int gram = 0; gram < 3; gram++)
+                for (int gram = 0; gram < kCellForms; gram++)
                     for (long long c : chunks)
                     {
                         g.ctx->cellGram = gram;

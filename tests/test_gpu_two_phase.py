@@ -1,6 +1,6 @@
 """The two forms of the generic 3D matrix assembly agree with the oracle and with each other:
-the two-phase kernels (cell matrices by columns or in Gram form with one or two row nodes per lane,
-NUTO_AMD_CELL_GRAM=0 / 1 / 2, then the row gather, with the wave-per-node LDS gather or the
+the two-phase kernels (cell matrices by columns, in Gram form with one or two row nodes per lane or
+on the matrix cores, NUTO_AMD_CELL_GRAM=0 / 1 / 2 / 3, then the row gather, with the wave-per-node LDS gather or the
 lane-per-node gather that serves nodes whose rows exceed the LDS budget) and the row-owner kernel
 (NUTO_AMD_TWO_PHASE=0), for linear elasticity and local damage on tet4, tet10, hex8 and hex20
 (jittered meshes, per-cell material fields), at the parity bar of the other tests."""
@@ -41,7 +41,7 @@ def test_two_phase_forms(ct, dt, kind):
     ref = p.orc.assemble(u, None, kap if kind == "ld" else None)
     forms = {"wave": {}, "lane": {"NUTO_AMD_GATHER_LANE": "1"}, "row-owner": {"NUTO_AMD_TWO_PHASE": "0"},
              "columns": {"NUTO_AMD_CELL_GRAM": "0"}, "gram1": {"NUTO_AMD_CELL_GRAM": "1"},
-             "gram2": {"NUTO_AMD_CELL_GRAM": "2"}}
+             "gram2": {"NUTO_AMD_CELL_GRAM": "2"}, "mfma": {"NUTO_AMD_CELL_GRAM": "3"}}
     out = {}
     for name, env in forms.items():
         R, K = assemble(p, u, kap, env)
@@ -63,7 +63,8 @@ def test_two_phase_with_material_fields():
     gf = rng.uniform(0.05, 0.15, p.ncells)
     p.orc.set_theta(E, ft, gf)
     ref = p.orc.assemble(u, None, kap)
-    for env in ({}, {"NUTO_AMD_TWO_PHASE": "0"}, {"NUTO_AMD_CELL_GRAM": "2"}, {"NUTO_AMD_CELL_GRAM": "0"}):
+    for env in ({}, {"NUTO_AMD_TWO_PHASE": "0"}, {"NUTO_AMD_CELL_GRAM": "2"}, {"NUTO_AMD_CELL_GRAM": "3"},
+                {"NUTO_AMD_CELL_GRAM": "0"}):
         old = {k: os.environ.get(k) for k in env}
         os.environ.update(env)
         try:
@@ -87,7 +88,7 @@ def test_two_phase_unilateral_damage(ct, dt):
     p = Pair(ct, dt, X, C, *ld_params(N.UNILATERAL), False)
     u, _, kap = p.inputs(u_range=6e-4, k_range=2e-4)
     ref = p.orc.assemble(u, None, kap)
-    for form in ("1", "2", "0"):  # the second Gram keeps one row node per lane for form 2
+    for form in ("1", "2", "3", "0"):  # the second Gram runs forms 2 and 3 as form 1
         R, K = assemble(p, u, kap, {"NUTO_AMD_CELL_GRAM": form})
         check_values(R, ref["R"], ref["R_bound"], f"R (form {form})")
         check_values(K, ref["val"], ref["bound"], f"K (form {form})")
