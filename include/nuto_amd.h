@@ -112,8 +112,11 @@ int nuto_set_dof_type(nuto_ctx* ctx, int slot, int dof_type, int ncomp, int n_do
 /* The integrand bound into the VectorFunction / MatrixFunction (CellInterface.h:19-20, Bind.h:11-22). */
 int nuto_set_integrand(nuto_ctx* ctx, const nuto_params* params);
 
-/* IntegrationTypeBase of the cells: kind 0 = IntegrationTypeTensorProduct<dim>(n, GAUSS),
- * kind 1 = IntegrationTypeTetrahedron(order n) (IntegrationCompanion.cpp:18-52). */
+/* IntegrationTypeBase of the cells: kind 0 = IntegrationTypeTensorProduct<dim>(n, GAUSS), n = 1..10
+ * (Legendre roots by Newton iteration, Legendre.cpp:23-55), kind 1 = IntegrationTypeTetrahedron
+ * (order n = 1, 2) (IntegrationCompanion.cpp:18-52). Kernels exist for quad4 with 1, 2, 3, 4, 5 and
+ * hex8 with 2, 3, 4 points per direction (hex20: 3); other valid rules give NUTO_ERR_UNSUPPORTED at
+ * the first assembly. */
 int nuto_set_rule(nuto_ctx* ctx, int kind, int n);
 
 /* Leading dimension of the history matrix (rows of GradientDamage::mKappas, GradientDamage.h:137). */

@@ -184,10 +184,58 @@ struct Rule
     }
 };
 
-// Gauss-Legendre points/weights on [-1, 1] for n = 1..3 in closed form (the roots of P_n); these
-// equal the Newton fixed point of the reference rule to the last bit for n = 1, 2 and to 1 ulp for 3.
+// Gauss-Legendre points/weights on [-1, 1] (IntegrationTypeTensorProduct.cpp, Quadrature.cpp:26-39)
+// for n = 1..3 in closed form (the roots of P_n); these equal the Newton fixed point of the reference
+// rule to the last bit for n = 1, 2 and to 1 ulp for 3. For n = 4..kMaxGauss: Newton's method on P_n
+// from x0 = cos(pi (i + 3/4) / (n + 1/2)) (the textbook start, Abramowitz & Stegun 22.16.6), P_n and
+// P_n' by the three-term recurrence, until the step stops changing x; w = 2 / ((1 - x^2) P_n'(x)^2).
+// Points ascending, the negative half mirrored from the positive one (exact symmetry).
+constexpr int kMaxGauss = 10;
+
+inline void LegendreAt(int n, double x, double& p, double& dp)
+{
+    double p0 = 1.0, p1 = x;
+    for (int k = 2; k <= n; k++)
+    {
+        const double p2 = ((2 * k - 1) * x * p1 - (k - 1) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+    }
+    p = p1;                                   // P_n (n >= 1)
+    dp = n * (x * p1 - p0) / (x * x - 1.0);   // P_n' from P_n and P_{n-1} (x strictly inside (-1, 1))
+}
+
 inline bool Gauss1D(int n, std::vector<double>& x, std::vector<double>& w)
 {
+    if (n >= 4 && n <= kMaxGauss)
+    {
+        x.assign(n, 0.0);
+        w.assign(n, 0.0);
+        for (int i = 0; i < (n + 1) / 2; i++)
+        {
+            // the i-th largest root
+            double r = std::cos(3.14159265358979323846 * (i + 0.75) / (n + 0.5));
+            for (int it = 0; it < 100; it++)
+            {
+                double p, dp;
+                LegendreAt(n, r, p, dp);
+                const double rn = r - p / dp;
+                if (rn == r)
+                    break;
+                r = rn;
+            }
+            if (n % 2 == 1 && i == n / 2)
+                r = 0.0;  // the middle root of an odd rule
+            double p, dp;
+            LegendreAt(n, r, p, dp);
+            const double wt = 2.0 / ((1.0 - r * r) * dp * dp);
+            x[n - 1 - i] = r;
+            x[i] = -r;
+            w[n - 1 - i] = wt;
+            w[i] = wt;
+        }
+        return true;
+    }
     switch (n)
     {
     case 1:

@@ -505,7 +505,8 @@ void LaunchRowsT(nuto_ctx* ctx, const RowArgs& a, bool wr, bool wk)
     const int n = a.nNodes;
     if (n == 0)
         return;
-    if constexpr (DIM == 3 && KIND != kGradientDamage && NND * 3 <= 64)
+    // (the staging kernels keep every IP's state in LDS: rules up to 27 points)
+    if constexpr (DIM == 3 && KIND != kGradientDamage && NND * 3 <= 64 && NIP <= 27)
     {
         // the single-pass kernel serves when the staging does not fit in device memory
         if (wk && !TwoPhaseOff() && TryStage(ctx, NND * 3, a.nCells))
@@ -593,13 +594,21 @@ void Dispatch(nuto_ctx* ctx, const RowArgs& a, bool wr, bool wk, int mode, doubl
     NUTO_CASE(kLinearElastic, 2, kQuad4, 4, kQuad4, 4, 4, 4)
     NUTO_CASE(kLinearElastic, 2, kQuad4, 4, kQuad4, 4, 1, 4)
     NUTO_CASE(kLinearElastic, 2, kQuad4, 4, kQuad4, 4, 9, 4)
+    NUTO_CASE(kLinearElastic, 2, kQuad4, 4, kQuad4, 4, 16, 4)
+    NUTO_CASE(kLinearElastic, 2, kQuad4, 4, kQuad4, 4, 25, 4)
     NUTO_CASE(kLinearElastic, 3, kHex8, 8, kHex8, 8, 8, 8)
+    NUTO_CASE(kLinearElastic, 3, kHex8, 8, kHex8, 8, 27, 8)
+    NUTO_CASE(kLinearElastic, 3, kHex8, 8, kHex8, 8, 64, 8)
     NUTO_CASE(kLinearElastic, 3, kTet4, 4, kTet4, 4, 1, 4)
     NUTO_CASE(kLinearElastic, 3, kTet4, 4, kTet10, 10, 4, 5)
     NUTO_CASE(kLinearElastic, 3, kTet10, 10, kTet10, 10, 4, 5)
     NUTO_CASE(kLinearElastic, 3, kHex8, 8, kHex20, 20, 27, 5)
     NUTO_CASE(kGradientDamage, 3, kHex8, 8, kHex8, 8, 8, 4)
+    NUTO_CASE(kGradientDamage, 3, kHex8, 8, kHex8, 8, 27, 4)
+    NUTO_CASE(kGradientDamage, 3, kHex8, 8, kHex8, 8, 64, 4)
     NUTO_CASE(kLocalDamage, 3, kHex8, 8, kHex8, 8, 8, 8)
+    NUTO_CASE(kLocalDamage, 3, kHex8, 8, kHex8, 8, 27, 8)
+    NUTO_CASE(kLocalDamage, 3, kHex8, 8, kHex8, 8, 64, 8)
     NUTO_CASE(kLocalDamage, 3, kTet4, 4, kTet4, 4, 1, 4)
     NUTO_CASE(kLocalDamage, 3, kTet4, 4, kTet10, 10, 4, 5)
     NUTO_CASE(kLocalDamage, 3, kTet10, 10, kTet10, 10, 4, 5)
