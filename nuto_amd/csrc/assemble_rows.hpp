@@ -449,6 +449,15 @@ __global__ void __launch_bounds__(64) kAssembleRows(const RowArgs A)
     AssembleRowsBody<KIND, DIM, NNC, NND, NIP, BH, WR, WK>(A, blockIdx.x * blockDim.x + threadIdx.x, A.nNodes);
 }
 
+// The rows of the dof nodes nodes[0 .. n) only (the nodes a specialised kernel leaves out)
+template <int KIND, int DIM, int NNC, int NND, int NIP, int BH, bool WR, bool WK>
+__global__ void __launch_bounds__(64) kAssembleRowsList(const RowArgs A, const int* __restrict__ nodes, int n)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n)
+        AssembleRowsBody<KIND, DIM, NNC, NND, NIP, BH, WR, WK>(A, nodes[i], A.nNodes);
+}
+
 // History update (GradientDamage::Update, GradientDamage.h:122-125, or EvolutionImplicit::Update,
 // LocalIsotropicDamage.h:130-133): one lane per (cell, ip).
 template <int KIND, int DIM, int NNC, int NND, int NIP>

@@ -1153,6 +1153,8 @@ struct GdmHex8Plan
     DevBuf posOf;        // node -> table position (Morton order; NUTO_AMD_GDM_ORDER=0: node order, empty)
     DevBuf ipState;      // pre-pass state records (NUTO_AMD_GDM_PREPASS=0: recompute in the node kernel)
     DevBuf geoCell;      // affine meshes: IP 0's geometry record per cell
+    DevBuf extra;        // nodes beyond the kernel's bounds (more than 8 cells or 32 neighbours)
+    int nExtra = 0;      // ... assembled by the generic row-owner kernel
     bool prepass = true;
     bool affine = false;  // every cell affine: MODE 2 of the node kernel
     int nCells = 0;
@@ -1161,8 +1163,15 @@ struct GdmHex8Plan
     {
         ready = false;
         nNodes = 0;
-        for (DevBuf* b : {&nodeHead, &nodeCell, &nodeHist, &nodeLa, &nodeSlot, &geo, &ipState, &posOf, &geoCell})
+        for (DevBuf* b : {&nodeHead, &nodeCell, &nodeHist, &nodeLa, &nodeSlot, &geo, &ipState, &posOf, &geoCell, &extra})
             b->Release();
+        nExtra = 0;
+    }
+
+    // every node in the tables (the adjoint and the batched launches need that)
+    bool Complete() const
+    {
+        return nExtra == 0;
     }
 
     bool Usable(int kind, int coordType, int dofType, int nip) const
@@ -1184,11 +1193,19 @@ struct GdmHex8Plan
         const int n = g.nNodes;
         if (nRows != 4 * n)  // 3 displacement rows and 1 eeq row per node
             return false;
+        // nodes beyond the kernel's bounds stay out of the tables (the generic kernel assembles them)
+        std::vector<char> out(n, 0);
+        std::vector<int> extraNodes;
         for (int p = 0; p < n; p++)
         {
             if (g.incPtr[p + 1] - g.incPtr[p] > gdm8::kMaxInc || g.adjPtr[p + 1] - g.adjPtr[p] > gdm8::kMaxAdj)
-                return false;
+            {
+                out[p] = 1;
+                extraNodes.push_back(p);
+            }
         }
+        if (n > 0 && (int)extraNodes.size() == n)
+            return false;
         // traversal order: position s holds node ord[s]
         std::vector<int> ord(n);
         bool reordered = false;
@@ -1204,16 +1221,23 @@ struct GdmHex8Plan
             for (int s = 0; s < n; s++)
                 ord[s] = s;
         }
+        if (!extraNodes.empty())
+        {
+            // drop the outside nodes from the traversal; they keep position -1
+            ord.erase(std::remove_if(ord.begin(), ord.end(), [&](int p) { return out[p] != 0; }), ord.end());
+            reordered = true;
+        }
+        const int nIn = (int)ord.size();
         // CSR rows: displacement rows 3p .. 3p + 2, eeq row 3n + p
         std::vector<int> rp((size_t)4 * n + 1);
         if ((err = hipMemcpyAsync(rp.data(), rowPtr, rp.size() * sizeof(int), hipMemcpyDeviceToHost, stream)) !=
                 hipSuccess ||
             (err = hipStreamSynchronize(stream)) != hipSuccess)
             return false;
-        std::vector<int4> head(n);
-        std::vector<int> cell((size_t)n * 8, -1), hist(cellIds ? (size_t)n * 8 : 0, 0), pos(reordered ? n : 0);
-        std::vector<unsigned char> la((size_t)n * 8, 0), slot((size_t)n * 64, 0);
-        for (int s = 0; s < n; s++)
+        std::vector<int4> head(nIn);
+        std::vector<int> cell((size_t)nIn * 8, -1), hist(cellIds ? (size_t)nIn * 8 : 0, 0), pos(reordered ? n : 0, -1);
+        std::vector<unsigned char> la((size_t)nIn * 8, 0), slot((size_t)nIn * 64, 0);
+        for (int s = 0; s < nIn; s++)
         {
             const int p = ord[s];
             head[s] = make_int4(p, rp[(size_t)3 * p], rp[(size_t)3 * p + 1] - rp[(size_t)3 * p], rp[(size_t)3 * n + p]);
@@ -1234,6 +1258,8 @@ struct GdmHex8Plan
             (reordered && (err = posOf.Upload(pos.data(), pos.size() * sizeof(int), stream)) != hipSuccess) ||
             (err = nodeCell.Upload(cell.data(), cell.size() * sizeof(int), stream)) != hipSuccess ||
             (err = nodeLa.Upload(la.data(), la.size(), stream)) != hipSuccess ||
+            (!extraNodes.empty() &&
+             (err = extra.Upload(extraNodes.data(), extraNodes.size() * sizeof(int), stream)) != hipSuccess) ||
             (err = nodeSlot.Upload(slot.data(), slot.size(), stream)) != hipSuccess ||
             (cellIds && (err = nodeHist.Upload(hist.data(), hist.size() * sizeof(int), stream)) != hipSuccess) ||
             (err = geo.Alloc((size_t)nCells * 8 * gdm8::kGeo * sizeof(double))) != hipSuccess)
@@ -1292,7 +1318,8 @@ struct GdmHex8Plan
             return false;
         }
         this->nCells = nCells;
-        nNodes = n;
+        nNodes = nIn;
+        nExtra = (int)extraNodes.size();
         const char* env = std::getenv("NUTO_AMD_GDM_XCD_MAP");
         xcdMap = env ? std::atoi(env) : 1;
         grid = xcdMap ? (nNodes + 7) / 8 * 8 : nNodes;
@@ -1353,14 +1380,29 @@ struct GdmHex8Plan
     // state: device memory of StateDoubles() doubles for the pre-pass records (nullptr: the plan's own)
     void Launch(const RowArgs& a, bool wr, bool wk, hipStream_t stream, double* state = nullptr)
     {
-        if (nNodes == 0)
-            return;
-        if (!state && ipState.p)
-            state = ipState.As<double>();
-        if (a.P.damage_application == 0)
-            LaunchT<true>(a, wr, wk, state, stream);
-        else
-            LaunchT<false>(a, wr, wk, state, stream);
+        if (nNodes > 0)
+        {
+            if (!state && ipState.p)
+                state = ipState.As<double>();
+            if (a.P.damage_application == 0)
+                LaunchT<true>(a, wr, wk, state, stream);
+            else
+                LaunchT<false>(a, wr, wk, state, stream);
+        }
+        if (nExtra > 0)
+        {
+            const dim3 grid((unsigned)((nExtra + 63) / 64));
+            const int* list = extra.As<int>();
+            if (wr && wk)
+                hipLaunchKernelGGL((kAssembleRowsList<kGradientDamage, 3, 8, 8, 8, 4, true, true>), grid, dim3(64), 0,
+                                   stream, a, list, nExtra);
+            else if (wr)
+                hipLaunchKernelGGL((kAssembleRowsList<kGradientDamage, 3, 8, 8, 8, 4, true, false>), grid, dim3(64), 0,
+                                   stream, a, list, nExtra);
+            else
+                hipLaunchKernelGGL((kAssembleRowsList<kGradientDamage, 3, 8, 8, 8, 4, false, true>), grid, dim3(64), 0,
+                                   stream, a, list, nExtra);
+        }
     }
 
     // R + K of the nodes [first, end) only (one mesh of a batch), plain block order

@@ -1198,8 +1198,7 @@ int nuto_build_vector_matrix_noalloc(nuto_ctx* ctx, const double* u, const doubl
                 "node tables not prepared: set the integrand and the rule before nuto_build_pattern, or call "
                 "nuto_build_vector_matrix once");
         Require(ctx->gdm.Usable(ctx->params.kind, ctx->coordType, ctx->f[0].type, ctx->nip), NUTO_ERR_UNSUPPORTED,
-                "the allocation-free assembly needs hex8 cells and dofs, 2x2x2 Gauss points and at most 8 cells "
-                "and 32 neighbours per node");
+                "the allocation-free assembly needs hex8 cells and dofs and 2x2x2 Gauss points");
         Require(ctx->gdm.StateDoubles() <= kScratchDoubles, NUTO_ERR_ARGUMENT,
                 "the pre-pass records of " + std::to_string(ctx->nCells) + " cells exceed the device scratch of " +
                     std::to_string(kScratchBytes >> 20) + " MiB (at most " + std::to_string(kScratchMaxCells) +
@@ -1629,7 +1628,8 @@ int nuto_vjp(nuto_ctx* ctx, const double* u, const double* e, const double* kapp
         if (!ctx->tabValid)
             BuildTables(ctx);
         PrepareGdm(ctx);
-        Require(ctx->gdm.Usable(ctx->params.kind, ctx->coordType, ctx->f[0].type, ctx->nip), NUTO_ERR_UNSUPPORTED,
+        Require(ctx->gdm.Usable(ctx->params.kind, ctx->coordType, ctx->f[0].type, ctx->nip) && ctx->gdm.Complete(),
+                NUTO_ERR_UNSUPPORTED,
                 "the adjoint needs hex8 cells and dofs, 2x2x2 Gauss points and at most 8 cells and 32 "
                 "neighbours per node");
         RowArgs a = MakeRowArgs(ctx, u, e, kappa, nullptr, nullptr);
@@ -2314,7 +2314,8 @@ int nuto_batch_build(nuto_batch* b)
                 if (gdm)
                 {
                     PrepareGdm(c);
-                    Require(c->gdm.Usable(c->params.kind, c->coordType, c->f[0].type, c->nip), NUTO_ERR_UNSUPPORTED,
+                    Require(c->gdm.Usable(c->params.kind, c->coordType, c->f[0].type, c->nip) && c->gdm.Complete(),
+                            NUTO_ERR_UNSUPPORTED,
                             "gradient-damage batch meshes need at most 8 cells and 32 neighbours per node");
                 }
             }));
