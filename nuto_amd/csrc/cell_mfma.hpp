@@ -3,8 +3,8 @@
 // with the sums over the IPs as products of LDS-staged matrices (ND = 3 x nodes, row 3n + c):
 //     G  = (c2 B)^T B,  B[ip][3n + c] = g_n[c] = (dN_n/dX)_c at IP ip, c2 = s G (1 - omega)
 //     R1 = U^T V,       U[ip][3n + c] = (B_n^T t)_c, V[ip][3n + c] = (B_n^T d eps_eq / d eps)_c
-// The IP count is padded to NK (a multiple of 4, zero rows) and ND to NDP (a multiple of 16, zero
-// columns). One workgroup of NDP / 16 waves per cell: wave w owns the 16-row tile row w of G and R1
+// The IPs are summed in chunks padded to a multiple of 4 (zero rows) and ND is padded to NDP (a
+// multiple of 16, zero columns). One workgroup of NDP / 16 waves per cell: wave w owns the 16-row tile row w of G and R1
 // (NDP / 16 tiles of each). Each IP term is read from LDS once per wave instead of once per block
 // lane as in kCellGram, whose LDS reads bound it.
 //
@@ -31,86 +31,56 @@ struct MfmaShape
 };
 
 // UNILATERAL damage application (a second Gram) is not provided here: the caller uses kCellGram.
+// The IPs are processed in chunks of KC (8, or NIP rounded up to 4 for smaller rules), so only one
+// chunk of B, U, V is staged at a time; the IP states, the chunk's staging and the final G share
+// one LDS pool (35 KB for hex20), which leaves room for 3 workgroups per CU.
 template <int KIND, int NNC, int NND, int NIP>
-__global__ void __launch_bounds__(MfmaShape<NND>::NT) kCellMfma(const RowArgs A, int c0, int c1,
-                                                                double* __restrict__ Ke, double* __restrict__ Re)
+__global__ void __launch_bounds__(MfmaShape<NND>::NT) __attribute__((amdgpu_waves_per_eu(4)))
+kCellMfma(const RowArgs A, int c0, int c1, double* __restrict__ Ke, double* __restrict__ Re)
 {
     using namespace gram;
     using Sh = MfmaShape<NND>;
     constexpr int ND = Sh::ND, NDP = Sh::NDP, TILES = Sh::TILES, NT = Sh::NT;
-    constexpr int NK = (NIP + 3) / 4 * 4;
+    constexpr int KC = NIP >= 8 ? 8 : (NIP + 3) / 4 * 4;  // IPs per chunk (a multiple of 4)
+    constexpr int NCH = (NIP + KC - 1) / KC;
     constexpr int LDM = NDP + 2;  // row stride of B, U, V: the 4 IP rows of one read fall on other banks
-    constexpr int LDG = NDP + 1;  // row stride of G
+    constexpr int LDG = ND + 1;   // row stride of G (rows and columns below ND only)
     constexpr bool LD = KIND == kLocalDamage;
     constexpr int NMAT = LD ? 3 : 1;
-    constexpr int STAGE = NMAT * NK * LDM > NDP * LDG ? NMAT * NK * LDM : NDP * LDG;
     constexpr int S = 1 + NNC * 3 + NND + NND * 3;  // IP table row: w, dNc, N, dNd
+    // pool: [J/grad records NIP x 18 | IP states NIP x kIpRec | B, U, V chunks | R terms KC x ND], then G
+    constexpr int OJG = 0, OIP = OJG + NIP * 18, OST = OIP + NIP * kIpRec, ORB = OST + NMAT * KC * LDM;
+    constexpr int POOLA = ORB + KC * ND, POOLB = ND * LDG;
+    constexpr int POOL = POOLA > POOLB ? POOLA : POOLB;
 
     __shared__ double sX[1][NNC * 3];
     __shared__ double sU[1][NND * 3];
-    __shared__ double sJG[1][NIP][18];
-    __shared__ double sIp[1][NIP][kIpRec];
-    __shared__ double sC2[NK];
-    __shared__ double sRb[NIP][ND];
-    __shared__ double sStage[STAGE];  // B, U, V for the products, then G
-    double* sB = sStage;
-    double* sUm = sStage + NK * LDM;
-    double* sVm = sStage + 2 * NK * LDM;
-    double* sG = sStage;
+    __shared__ double sC2[KC];
+    __shared__ double sR[ND];
+    __shared__ __align__(16) double sPool[POOL];
+    auto sJG = reinterpret_cast<double(*)[NIP][18]>(sPool + OJG);
+    auto sIp = reinterpret_cast<double(*)[NIP][kIpRec]>(sPool + OIP);
+    double* sB = sPool + OST;
+    double* sUm = sPool + OST + KC * LDM;
+    double* sVm = sPool + OST + 2 * KC * LDM;
+    double* sRb = sPool + ORB;  // [i][dof] of the chunk
+    double* sG = sPool;         // after the last chunk
 
     const int t = (int)threadIdx.x;
     const int cell = c0 + (int)blockIdx.x;  // grid: c1 - c0 workgroups
+    for (int k = t; k < ND; k += NT)
+        sR[k] = 0.0;
     CellIpRecords<KIND, NNC, NND, NIP, 1, NT>(A, c1, cell, sX, sU, sJG, sIp);
-    __syncthreads();
-
-    // ---- per (node, IP): g_n, u_n, v_n into B, U, V; the R terms s B_n^T sigma ------------------------
-    for (int k = t; k < NIP * NND; k += NT)
+    // zero dof columns ND .. NDP - 1 of the staged matrices (never written by the chunks)
+    if constexpr (NDP > ND)
     {
-        const int ip = k / NND, n = k - ip * NND;
-        const double* rec = sIp[0][ip];
-        const double* dN = A.tab + ip * S + 1 + NNC * 3 + NND + n * 3;
-        double g[3];
-#pragma unroll
-        for (int c = 0; c < 3; c++)
-            g[c] = dN[0] * rec[c] + dN[1] * rec[3 + c] + dN[2] * rec[6 + c];
-        double* b = sB + ip * LDM + 3 * n;
-        b[0] = g[0];
-        b[1] = g[1];
-        b[2] = g[2];
-        double bt[3];
-        BTv3(g, rec + 12, bt);
-        sRb[ip][3 * n] = rec[9] * bt[0];
-        sRb[ip][3 * n + 1] = rec[9] * bt[1];
-        sRb[ip][3 * n + 2] = rec[9] * bt[2];
-        if constexpr (LD)
+        for (int k = t; k < NMAT * KC * (NDP - ND); k += NT)
         {
-            double w[3];
-            BTv3(g, rec + 18, w);
-            double* u = sUm + ip * LDM + 3 * n;
-            u[0] = w[0];
-            u[1] = w[1];
-            u[2] = w[2];
-            BTv3(g, rec + 24, w);
-            double* v = sVm + ip * LDM + 3 * n;
-            v[0] = w[0];
-            v[1] = w[1];
-            v[2] = w[2];
+            const int m = k / (KC * (NDP - ND)), e = k - m * (KC * (NDP - ND));
+            sB[m * KC * LDM + (e / (NDP - ND)) * LDM + ND + e % (NDP - ND)] = 0.0;
         }
     }
-    // zero padding: IP rows NIP .. NK - 1, dof columns ND .. NDP - 1 of each matrix
-    constexpr int PADR = (NK - NIP) * NDP, PADC = NIP * (NDP - ND), PW = NDP > ND ? NDP - ND : 1;
-    for (int k = t; k < NMAT * (PADR + PADC); k += NT)
-    {
-        const int m = k / (PADR + PADC), e = k - m * (PADR + PADC);
-        const int row = e < PADR ? NIP + e / NDP : (e - PADR) / PW;
-        const int col = e < PADR ? e % NDP : ND + (e - PADR) % PW;
-        sStage[m * NK * LDM + row * LDM + col] = 0.0;
-    }
-    for (int k = t; k < NK; k += NT)
-        sC2[k] = k < NIP ? sIp[0][k][11] : 0.0;
-    __syncthreads();
 
-    // ---- products: wave w, tile row w, all tile columns ------------------------------------------
     const int w = t >> 6, l = t & 63;
     const int lr = l & 15, lk = l >> 4;
     MfmaF64x4 accG[TILES], accR[LD ? TILES : 1];
@@ -121,26 +91,99 @@ __global__ void __launch_bounds__(MfmaShape<NND>::NT) kCellMfma(const RowArgs A,
         if constexpr (LD)
             accR[tc] = MfmaF64x4{0.0, 0.0, 0.0, 0.0};
     }
-#pragma unroll
-    for (int ks = 0; ks < NK / 4; ks++)
+
+#pragma unroll 1
+    for (int ch = 0; ch < NCH; ch++)
     {
-        const int k = 4 * ks + lk;
-        const double aG = sC2[k] * sB[k * LDM + 16 * w + lr];
-        const double aU = LD ? sUm[k * LDM + 16 * w + lr] : 0.0;
-#pragma unroll
-        for (int tc = 0; tc < TILES; tc++)
+        const int ip0 = ch * KC;
+        // rows of this chunk: KC, the last chunk's IPs rounded up to a multiple of 4
+        const int rows = NIP - ip0 >= KC ? KC : (NIP - ip0 + 3) / 4 * 4;
+        __syncthreads();  // IP states written / the previous chunk's staging consumed
+        // ---- per (node, IP of the chunk): g_n, u_n, v_n into B, U, V; the R terms s B_n^T sigma ------
+        for (int k = t; k < rows * NND; k += NT)
         {
-            accG[tc] = __builtin_amdgcn_mfma_f64_16x16x4f64(aG, sB[k * LDM + 16 * tc + lr], accG[tc], 0, 0, 0);
+            const int i = k / NND, n = k - i * NND, ip = ip0 + i;
+            double* b = sB + i * LDM + 3 * n;
+            double* u = sUm + i * LDM + 3 * n;
+            double* v = sVm + i * LDM + 3 * n;
+            if (ip >= NIP)
+            {
+                // padding rows of the last chunk
+                b[0] = b[1] = b[2] = 0.0;
+                if constexpr (LD)
+                {
+                    u[0] = u[1] = u[2] = 0.0;
+                    v[0] = v[1] = v[2] = 0.0;
+                }
+                continue;
+            }
+            const double* rec = sIp[0][ip];
+            const double* dN = A.tab + ip * S + 1 + NNC * 3 + NND + n * 3;
+            double g[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                g[c] = dN[0] * rec[c] + dN[1] * rec[3 + c] + dN[2] * rec[6 + c];
+            b[0] = g[0];
+            b[1] = g[1];
+            b[2] = g[2];
+            double bt[3];
+            BTv3(g, rec + 12, bt);
+            sRb[i * ND + 3 * n] = rec[9] * bt[0];
+            sRb[i * ND + 3 * n + 1] = rec[9] * bt[1];
+            sRb[i * ND + 3 * n + 2] = rec[9] * bt[2];
             if constexpr (LD)
-                accR[tc] = __builtin_amdgcn_mfma_f64_16x16x4f64(aU, sVm[k * LDM + 16 * tc + lr], accR[tc], 0, 0, 0);
+            {
+                double wv[3];
+                BTv3(g, rec + 18, wv);
+                u[0] = wv[0];
+                u[1] = wv[1];
+                u[2] = wv[2];
+                BTv3(g, rec + 24, wv);
+                v[0] = wv[0];
+                v[1] = wv[1];
+                v[2] = wv[2];
+            }
+        }
+        for (int k = t; k < KC; k += NT)
+            sC2[k] = ip0 + k < NIP ? sIp[0][ip0 + k][11] : 0.0;
+        __syncthreads();
+        // ---- R_e: the chunk's IPs in order per dof ---------------------------------------------------
+        for (int k = t; k < ND; k += NT)
+        {
+            double acc = sR[k];
+            for (int i = 0; i < KC && ip0 + i < NIP; i++)
+                acc += sRb[i * ND + k];
+            sR[k] = acc;
+        }
+        // ---- products: wave w, tile row w, all tile columns --------------------------------------------
+#pragma unroll
+        for (int ks = 0; ks < KC / 4; ks++)
+        {
+            if (4 * ks >= rows)
+                break;  // uniform over the workgroup
+            const int k = 4 * ks + lk;
+            const double aG = sC2[k] * sB[k * LDM + 16 * w + lr];
+            const double aU = LD ? sUm[k * LDM + 16 * w + lr] : 0.0;
+#pragma unroll
+            for (int tc = 0; tc < TILES; tc++)
+            {
+                accG[tc] = __builtin_amdgcn_mfma_f64_16x16x4f64(aG, sB[k * LDM + 16 * tc + lr], accG[tc], 0, 0, 0);
+                if constexpr (LD)
+                    accR[tc] =
+                        __builtin_amdgcn_mfma_f64_16x16x4f64(aU, sVm[k * LDM + 16 * tc + lr], accR[tc], 0, 0, 0);
+            }
         }
     }
-    __syncthreads();  // B, U, V consumed: the stage now holds G
+    __syncthreads();  // every chunk consumed: the pool now holds G
 #pragma unroll
     for (int tc = 0; tc < TILES; tc++)
 #pragma unroll
         for (int i = 0; i < 4; i++)
-            sG[(16 * w + lk + 4 * i) * LDG + 16 * tc + lr] = accG[tc][i];
+        {
+            const int row = 16 * w + lk + 4 * i, col = 16 * tc + lr;
+            if (row < ND && col < ND)
+                sG[row * LDG + col] = accG[tc][i];
+        }
     __syncthreads();
 
     // ---- K_e = r G + blockwise G^T + tr I (+ R1); own entries of the lane ------------------------------
@@ -175,14 +218,8 @@ __global__ void __launch_bounds__(MfmaShape<NND>::NT) kCellMfma(const RowArgs A,
                     ke[(size_t)row * ND + col] = v;
                 }
             }
-        // R_e: the IPs in order per dof
         for (int k = t; k < ND; k += NT)
-        {
-            double acc = 0.0;
-            for (int ip = 0; ip < NIP; ip++)
-                acc += sRb[ip][k];
-            Re[(size_t)(cell - c0) * ND + k] = acc;
-        }
+            Re[(size_t)(cell - c0) * ND + k] = sR[k];
     }
 }
 
