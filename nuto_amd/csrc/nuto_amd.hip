@@ -1920,6 +1920,20 @@ long long MaxChunkCells(const nuto_batch* b, const BatchGroup& g)
     return best;
 }
 
+// Staging of every group's two-phase launch (mode 5) allocated now, before a graph capture of the
+// launch (which must not allocate); the chunk defaults where none is chosen yet
+void PrepareStaging(nuto_batch* b)
+{
+    if (IsGdm(b))
+        return;
+    for (BatchGroup& g : b->groups)
+    {
+        if (g.stageCells < 0)
+            g.stageCells = DefaultStageCells(g);
+        EnsureStage(g.ctx, 3 * NodesOf(g.dt), MaxChunkCells(b, g));
+    }
+}
+
 // Two-phase launches of a group in chunks of consecutive meshes whose cells fit stageCells (0: the
 // whole group at once; a mesh larger than the chunk is a chunk of its own). The rows of a mesh only
 // touch its own cells, so each chunk's staging is consumed by its gather before the next chunk
@@ -2201,6 +2215,7 @@ int nuto_batch_set_stage_bytes(nuto_batch* b, long long bytes)
         for (BatchGroup& g : b->groups)
             g.stageCells = bytes == 0 ? 0 : StageCellsOf(g, (size_t)bytes);
         DropGraph(b);
+        PrepareStaging(b);
     });
 }
 
@@ -2504,6 +2519,8 @@ int nuto_batch_assemble(nuto_batch* b, const double* u, const double* e, const d
             const int m = b->tuned >= 0 ? b->tuned : 3;
             if (m < 3)
                 UploadArgs(b, u, e, kappa, R, K);  // outside the capture
+            if (m == 5)
+                PrepareStaging(b);  // outside the capture
             const void* p[5] = {u, e, kappa, R, K};
             GraphRun(b, m, p, 5, [&] { LaunchBatch(b, m, u, e, kappa, R, K); });
         }
