@@ -204,7 +204,6 @@ __global__ void __launch_bounds__(NT) kCellGram(const RowArgs A, int c0, int c1,
 
     const int t = (int)threadIdx.x;
     const int cellBase = c0 + (int)blockIdx.x * CPW;
-    const LawParams& P = A.P;
 
     for (int k = t; k < CPW * NND * 3; k += NT)
         sR[k / (NND * 3)][k % (NND * 3)] = 0.0;
@@ -366,19 +365,8 @@ __global__ void __launch_bounds__(NT) kCellGram(const RowArgs A, int c0, int c1,
     const int cell = cellBase + bc;
     if (blk && cell < c1)
     {
-        double r = 0.0;  // one Gram: G1 = r G2 with r = c1 / c2 of the law
-        if constexpr (!TWO)
-        {
-            double E, k0, beta;
-            CellMaterial(A, cell, E, k0, beta);
-            if constexpr (KIND == kLinearElastic)
-                r = 2.0 * P.nu / (1.0 - 2.0 * P.nu);  // lambda / mu
-            else
-            {
-                const double m3K = 3.0 * E / (3.0 - 6.0 * P.nu), m2G = 2.0 * E / (2.0 + 2.0 * P.nu);
-                r = (m3K - m2G) / (1.5 * m2G);  // ((3K - 2G) / 3) / G
-            }
-        }
+        // one Gram: G1 = r G2 with r = c1 / c2 of the law (E cancels: one value per launch)
+        const double r = TWO ? 0.0 : (KIND == kLinearElastic ? A.P.rLE : A.P.rLD);
 #pragma unroll
         for (int sa = 0; sa < AP; sa++)
 #pragma unroll

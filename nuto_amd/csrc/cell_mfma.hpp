@@ -189,16 +189,8 @@ kCellMfma(const RowArgs A, int c0, int c1, double* __restrict__ Ke, double* __re
     // ---- K_e = r G + blockwise G^T + tr I (+ R1); own entries of the lane ------------------------------
     if (cell < c1)
     {
-        double r;  // one Gram: c1 / c2 of the law
-        if constexpr (KIND == kLinearElastic)
-            r = 2.0 * A.P.nu / (1.0 - 2.0 * A.P.nu);  // lambda / mu
-        else
-        {
-            double E, k0, beta;
-            CellMaterial(A, cell, E, k0, beta);
-            const double m3K = 3.0 * E / (3.0 - 6.0 * A.P.nu), m2G = 2.0 * E / (2.0 + 2.0 * A.P.nu);
-            r = (m3K - m2G) / (1.5 * m2G);  // ((3K - 2G) / 3) / G
-        }
+        // one Gram: c1 / c2 of the law (E cancels: one value per launch)
+        const double r = KIND == kLinearElastic ? A.P.rLE : A.P.rLD;
         double* ke = Ke + (size_t)(cell - c0) * ND * ND;
 #pragma unroll
         for (int tc = 0; tc < TILES; tc++)

@@ -25,6 +25,9 @@ struct LawParams
     double cK, cG, K1, K2;
     int interaction;         // gradient damage: 0 NonlocalInteraction::Constant, 1 Decreasing
     double iR, iEta;         // Decreasing(R, eta)
+    // derived: c1 / c2 of the Gram form of the isotropic tangents (cell_gram.hpp), independent of E:
+    // linear elasticity lambda / mu; damage with FULL application ((3K - 2G) / 3) / G
+    double rLE, rLD;
 };
 
 // NonlocalInteraction::Constant / Decreasing (GradientDamage.h:150-184): the factor g(omega) of the
@@ -48,6 +51,8 @@ inline LawParams MakeDerived(LawParams p)
     p.cG = 2.0 / (2.0 + 2.0 * p.nu);
     p.K1 = (p.k - 1.0) / (2.0 * p.k * (1.0 - 2.0 * p.nu));
     p.K2 = 3.0 / (p.k * (1.0 + p.nu) * (1.0 + p.nu));
+    p.rLE = 2.0 * p.nu / (1.0 - 2.0 * p.nu);
+    p.rLD = (p.cK - p.cG) / (1.5 * p.cG);
     return p;
 }
 
