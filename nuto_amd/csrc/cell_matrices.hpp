@@ -304,18 +304,40 @@ __global__ void __launch_bounds__(64) kGatherRows(const RowArgs A, int p0, int p
 
 // Phase 2 with one wave per dof node: the node's three rows are summed in LDS (3 L doubles, L =
 // 3 |adj(p)|, dynamic shared memory of 3 maxL doubles), each incident cell's three staged rows are
-// read with coalesced loads and added at their slots in cell order, then the rows are written with
-// coalesced stores (the rows of p are contiguous in the CSR).
+// read with coalesced 16-byte loads (3 ND is even and the rows start 16-byte aligned) and added at
+// their slots in cell order, then the rows are written with coalesced streaming stores (the rows of
+// p are contiguous in the CSR and are not read again here). The loads of CB cells are issued before
+// the first of them is added, so that a wave has CB cells' rows in flight at once.
 template <int NND, bool WR>
 __global__ void __launch_bounds__(64) kGatherRowsWave(const RowArgs A, int p0, int p1, int s0, const double* __restrict__ Ke,
                                                       const double* __restrict__ Re)
 {
     constexpr int ND = NND * 3;
+    constexpr int NH = 3 * ND / 2;          // 16-byte pairs of a cell's three staged rows
+    constexpr int NF = (NH + 63) / 64;      // pairs per lane and cell
+    constexpr int CB = NND >= 10 ? 2 : 4;   // cells in flight
+    static_assert(ND % 2 == 0, "pairs of a row");
     extern __shared__ double rows[];
     const int p = p0 + (int)blockIdx.x;
     const int lane = (int)threadIdx.x;
     if (p >= p1)
         return;
+    // pair h = lane + 64 k holds entries 2h, 2h + 1 of row i (a row has an even length): column
+    // nodes b and components jj of both
+    int fRow[NF], fNode[NF][2], fComp[NF][2];
+#pragma unroll
+    for (int k = 0; k < NF; k++)
+    {
+        const int f = 2 * (lane + 64 * k);
+        const int i = f / ND, rem = f - i * ND;
+        fRow[k] = i;
+#pragma unroll
+        for (int w = 0; w < 2; w++)
+        {
+            fNode[k][w] = (rem + w) / 3;
+            fComp[k][w] = rem + w - 3 * fNode[k][w];
+        }
+    }
     const int beg = A.incPtr[p], end = A.incPtr[p + 1];
     const int L = (A.adjPtr[p + 1] - A.adjPtr[p]) * 3;
     const int ru0 = A.rowPtr[p * 3];
@@ -323,26 +345,54 @@ __global__ void __launch_bounds__(64) kGatherRowsWave(const RowArgs A, int p0, i
         rows[f] = 0.0;
     double racc = 0.0;
     __syncthreads();
-    for (int r = beg; r < end; ++r)
+    for (int r0 = beg; r0 < end; r0 += CB)
     {
-        const int cell = A.incCell[r];
-        const int la = A.incLocal[r];
-        const unsigned short* sl = A.slot + (size_t)r * NND;
-        const double* src = Ke + ((size_t)(cell - s0) * ND + la * 3) * ND;
-        for (int f = lane; f < 3 * ND; f += 64)
+        double v[CB][NF][2], rv[CB];
+        int o[CB][NF][2];
+#pragma unroll
+        for (int c = 0; c < CB; c++)
         {
-            const int i = f / ND, rem = f - i * ND;
-            const int b = rem / 3, jj = rem - 3 * b;
-            const int q = sl[b] & 0x7fff;
-            rows[i * L + 3 * q + jj] += src[f];
+            const int r = r0 + c < end ? r0 + c : end - 1;  // past the last cell: loads of the last, unused
+            const int cell = A.incCell[r];
+            const int la = A.incLocal[r];
+            const unsigned short* sl = A.slot + (size_t)r * NND;
+            const double2* src = reinterpret_cast<const double2*>(Ke + ((size_t)(cell - s0) * ND + la * 3) * ND);
+#pragma unroll
+            for (int k = 0; k < NF; k++)
+            {
+                const int h = lane + 64 * k;
+                if (h < NH)
+                {
+                    const double2 t = src[h];
+                    v[c][k][0] = t.x;
+                    v[c][k][1] = t.y;
+#pragma unroll
+                    for (int w = 0; w < 2; w++)
+                        o[c][k][w] = fRow[k] * L + 3 * (sl[fNode[k][w]] & 0x7fff) + fComp[k][w];
+                }
+            }
+            rv[c] = (WR && lane < 3) ? Re[(size_t)(cell - s0) * ND + la * 3 + lane] : 0.0;
         }
-        if (WR && lane < 3)
-            racc += Re[(size_t)(cell - s0) * ND + la * 3 + lane];
-        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < CB; c++)
+        {
+            if (r0 + c < end)
+            {
+#pragma unroll
+                for (int k = 0; k < NF; k++)
+                    if (lane + 64 * k < NH)
+                    {
+                        rows[o[c][k][0]] += v[c][k][0];
+                        rows[o[c][k][1]] += v[c][k][1];
+                    }
+                racc += rv[c];
+                __syncthreads();  // the next cell may add at the same entries
+            }
+        }
     }
     double* dst = A.K + ru0;
     for (int f = lane; f < 3 * L; f += 64)
-        dst[f] = rows[f];
+        __builtin_nontemporal_store(rows[f], dst + f);
     if (WR && lane < 3)
         A.R[(size_t)p * 3 + lane] = racc;
 }
