@@ -33,7 +33,7 @@ struct MfmaShape
 // UNILATERAL damage application (a second Gram) is not provided here: the caller uses kCellGram.
 // The IPs are processed in chunks of KC (8, or NIP rounded up to 4 for smaller rules), so only one
 // chunk of B, U, V is staged at a time; the IP states, the chunk's staging and the final G share
-// one LDS pool (35 KB for hex20), which leaves room for 3 workgroups per CU.
+// one LDS pool (34 KB for hex20 with the block traces), which leaves room for 4 workgroups per CU.
 template <int KIND, int NNC, int NND, int NIP>
 __global__ void __launch_bounds__(MfmaShape<NND>::NT) __attribute__((amdgpu_waves_per_eu(4)))
 kCellMfma(const RowArgs A, int c0, int c1, double* __restrict__ Ke, double* __restrict__ Re)
@@ -50,7 +50,7 @@ kCellMfma(const RowArgs A, int c0, int c1, double* __restrict__ Ke, double* __re
     constexpr int S = 1 + NNC * 3 + NND + NND * 3;  // IP table row: w, dNc, N, dNd
     // pool: [J/grad records NIP x 18 | IP states NIP x kIpRec | B, U, V chunks | R terms KC x ND], then G
     constexpr int OJG = 0, OIP = OJG + NIP * 18, OST = OIP + NIP * kIpRec, ORB = OST + NMAT * KC * LDM;
-    constexpr int POOLA = ORB + KC * ND, POOLB = ND * LDG;
+    constexpr int POOLA = ORB + KC * ND, POOLB = ND * LDG + NND * NND;  // G, then the block traces
     constexpr int POOL = POOLA > POOLB ? POOLA : POOLB;
 
     __shared__ double sX[1][NNC * 3];
@@ -65,6 +65,7 @@ kCellMfma(const RowArgs A, int c0, int c1, double* __restrict__ Ke, double* __re
     double* sVm = sPool + OST + 2 * KC * LDM;
     double* sRb = sPool + ORB;  // [i][dof] of the chunk
     double* sG = sPool;         // after the last chunk
+    double* sTr = sPool + ND * LDG;  // tr(G_ab) at a * NND + b, after G
 
     const int t = (int)threadIdx.x;
     const int cell = c0 + (int)blockIdx.x;  // grid: c1 - c0 workgroups
@@ -185,30 +186,61 @@ kCellMfma(const RowArgs A, int c0, int c1, double* __restrict__ Ke, double* __re
                 sG[row * LDG + col] = accG[tc][i];
         }
     __syncthreads();
+    for (int k = t; k < NND * NND; k += NT)
+    {
+        const int a = k / NND, b = k - a * NND;
+        const double* g = sG + 3 * a * LDG + 3 * b;
+        sTr[k] = g[0] + g[LDG + 1] + g[2 * LDG + 2];
+    }
+    __syncthreads();
 
     // ---- K_e = r G + blockwise G^T + tr I (+ R1); own entries of the lane ------------------------------
+    // Lane entries: rows 16 w + lk + 4 i (node a, component ii), columns 16 tc + lr (node b, component
+    // jj); G_ab^T[ii][jj] = G[3 a + jj][3 b + ii]. Rows and columns past ND are clamped for the LDS
+    // reads and not stored.
     if (cell < c1)
     {
         // one Gram: c1 / c2 of the law (E cancels: one value per launch)
         const double r = KIND == kLinearElastic ? A.P.rLE : A.P.rLD;
-        double* ke = Ke + (size_t)(cell - c0) * ND * ND;
+        int rowT[4], rowTr[4], rowII[4];
+        bool rowIn[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+        {
+            const int row = 16 * w + lk + 4 * i;
+            const int rc = row < ND ? row : ND - 1;
+            const int a = rc / 3, ii = rc - 3 * a;
+            rowT[i] = 3 * a * LDG + ii;
+            rowTr[i] = a * NND;
+            rowII[i] = ii;
+            rowIn[i] = row < ND;
+        }
+        int colT[TILES], colTr[TILES], colJJ[TILES];
+        bool colIn[TILES];
+#pragma unroll
+        for (int tc = 0; tc < TILES; tc++)
+        {
+            const int col = 16 * tc + lr;
+            const int cc = col < ND ? col : ND - 1;
+            const int b = cc / 3, jj = cc - 3 * b;
+            colT[tc] = jj * LDG + 3 * b;
+            colTr[tc] = b;
+            colJJ[tc] = jj;
+            colIn[tc] = col < ND;
+        }
+        double* ke = Ke + (size_t)(cell - c0) * ND * ND + (size_t)(16 * w + lk) * ND + lr;
 #pragma unroll
         for (int tc = 0; tc < TILES; tc++)
 #pragma unroll
             for (int i = 0; i < 4; i++)
             {
-                const int row = 16 * w + lk + 4 * i, col = 16 * tc + lr;
-                if (row < ND && col < ND)
-                {
-                    const int a = row / 3, ii = row - 3 * a, b = col / 3, jj = col - 3 * b;
-                    double v = r * accG[tc][i] + sG[(3 * a + jj) * LDG + 3 * b + ii];
-                    if (ii == jj)
-                        v += sG[3 * a * LDG + 3 * b] + sG[(3 * a + 1) * LDG + 3 * b + 1] +
-                             sG[(3 * a + 2) * LDG + 3 * b + 2];
-                    if constexpr (LD)
-                        v += accR[tc][i];
-                    ke[(size_t)row * ND + col] = v;
-                }
+                double v = r * accG[tc][i] + sG[rowT[i] + colT[tc]];
+                const double tr = sTr[rowTr[i] + colTr[tc]];
+                v = rowII[i] == colJJ[tc] ? v + tr : v;
+                if constexpr (LD)
+                    v += accR[tc][i];
+                if (rowIn[i] && colIn[tc])
+                    ke[(size_t)(4 * i) * ND + 16 * tc] = v;
             }
         for (int k = t; k < ND; k += NT)
             Re[(size_t)(cell - c0) * ND + k] = sR[k];
