@@ -31,9 +31,10 @@ struct MfmaShape
 };
 
 // UNILATERAL damage application (a second Gram) is not provided here: the caller uses kCellGram.
-// The IPs are processed in chunks of KC (8, or NIP rounded up to 4 for smaller rules), so only one
+// The IPs are processed in chunks of KC (12, 8 for rules of 8 to 11 points, or NIP rounded up to 4
+// for smaller rules; hex20: 240 of the 256 lanes stage a chunk's (node, IP) terms), so only one
 // chunk of B, U, V is staged at a time; the IP states, the chunk's staging and the final G share
-// one LDS pool (34 KB for hex20 with the block traces), which leaves room for 4 workgroups per CU.
+// one LDS pool (35 KB for hex20 with the block traces), which leaves room for 4 workgroups per CU.
 template <int KIND, int NNC, int NND, int NIP>
 __global__ void __launch_bounds__(MfmaShape<NND>::NT) __attribute__((amdgpu_waves_per_eu(4)))
 kCellMfma(const RowArgs A, int c0, int c1, double* __restrict__ Ke, double* __restrict__ Re)
@@ -41,7 +42,7 @@ kCellMfma(const RowArgs A, int c0, int c1, double* __restrict__ Ke, double* __re
     using namespace gram;
     using Sh = MfmaShape<NND>;
     constexpr int ND = Sh::ND, NDP = Sh::NDP, TILES = Sh::TILES, NT = Sh::NT;
-    constexpr int KC = NIP >= 8 ? 8 : (NIP + 3) / 4 * 4;  // IPs per chunk (a multiple of 4)
+    constexpr int KC = NIP >= 12 ? 12 : (NIP >= 8 ? 8 : (NIP + 3) / 4 * 4);  // IPs per chunk (a multiple of 4)
     constexpr int NCH = (NIP + KC - 1) / KC;
     constexpr int LDM = NDP + 2;  // row stride of B, U, V: the 4 IP rows of one read fall on other banks
     constexpr int LDG = ND + 1;   // row stride of G (rows and columns below ND only)
