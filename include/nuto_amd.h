@@ -285,7 +285,8 @@ int nuto_quasistatic_step(nuto_ctx* ctx, double* x, double* kappa, const double*
  *   d_theta (3 x n_cells)  = (dR / d theta)^T lambda for theta = (E, f_t, G_f) per cell, stored
  *                            [E row; f_t row; G_f row]; NULL skips it. Without material fields the
  *                            scalar parameters are the point (f_t = kappa0 E, G_f = f_t / beta).
- * kappa (the history) is a constant. All pointers are device pointers. Hex8 cells and dofs only. */
+ * kappa (the history) is a constant. All pointers are device pointers. Hex8 cells and dofs, 2x2x2
+ * Gauss points, any number of cells and neighbours per node. */
 int nuto_vjp(nuto_ctx* ctx, const double* u, const double* e, const double* kappa, const double* lambda,
              double* d_state, double* d_theta);
 

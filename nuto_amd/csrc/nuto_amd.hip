@@ -1628,10 +1628,9 @@ int nuto_vjp(nuto_ctx* ctx, const double* u, const double* e, const double* kapp
         if (!ctx->tabValid)
             BuildTables(ctx);
         PrepareGdm(ctx);
-        Require(ctx->gdm.Usable(ctx->params.kind, ctx->coordType, ctx->f[0].type, ctx->nip) && ctx->gdm.Complete(),
-                NUTO_ERR_UNSUPPORTED,
-                "the adjoint needs hex8 cells and dofs, 2x2x2 Gauss points and at most 8 cells and 32 "
-                "neighbours per node");
+        // nodes beyond the node kernel's bounds (more than 8 cells or 32 neighbours) get the list kernel
+        Require(ctx->gdm.Usable(ctx->params.kind, ctx->coordType, ctx->f[0].type, ctx->nip), NUTO_ERR_UNSUPPORTED,
+                "the adjoint needs hex8 cells and dofs and 2x2x2 Gauss points");
         RowArgs a = MakeRowArgs(ctx, u, e, kappa, nullptr, nullptr);
         LaunchGdmHex8Vjp(ctx->gdm, a, lambda, d_state, d_theta, ctx->stream);
         Check(hipGetLastError(), "vjp launch");

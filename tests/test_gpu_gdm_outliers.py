@@ -3,14 +3,18 @@ around a vertical axis in two layers, so the axis node of the middle level has 1
 and 33 neighbours (the node kernel takes at most 8 and 32). That node's rows come from the generic
 row-owner kernel, every other node's from the node kernel; R and K against the oracle, the
 outside node's rows bit-identical to the all-generic assembly, and the allocation-free entry point
-on the same mesh. The adjoint and batches still need every node in bounds (refused here)."""
+on the same mesh. The adjoint of that node (and d_theta of the cells whose local node 0 it is) comes
+from the adjoint's list kernel: d_state against K^T lambda of the oracle's K, d_theta against
+central finite differences of the oracle's R. Batches still need every node in bounds."""
 import numpy as np
 import pytest
 import torch
 
 import nuto_amd as N
 from nuto_amd import binding as B
+from nuto_amd import synthetic as S
 from pairs import Pair, check_values, to_dev
+from test_gpu_adjoint import kt_lambda
 from test_gpu_parity import gdm_params
 
 pytestmark = pytest.mark.gpu
@@ -59,10 +63,47 @@ def test_outside_node_by_the_generic_kernel(monkeypatch):
     for r in rows:
         assert np.array_equal(Kn[rp[r]:rp[r + 1]], Kg[rp[r]:rp[r + 1]]), r
         assert Rn[r] == Rg[r], r
-    # no node kernel for the adjoint on this mesh
-    with pytest.raises(N.NutoError) as ei:
-        ctx.vjp(to_dev(u), to_dev(e), to_dev(kap), to_dev(np.ones(ctx.n_rows)))
-    assert ei.value.code == 5
+
+
+def test_outside_node_adjoint():
+    X, C = fan_mesh()
+    p = Pair(N.HEX8, N.HEX8, X, C, *gdm_params(), True)
+    th = S.material_fields(p.ncells)
+    u, e, kap = p.inputs(u_range=4e-4, k_range=1.5e-4)
+    p.orc.set_theta(*S.perturbed_fields(*th))
+    r_obs = p.orc.assemble(u, e, kap, want_k=False)["R"]
+    p.orc.set_theta(*th)
+    ref = p.orc.assemble(u, e, kap)
+    lam = ref["R"] - r_obs
+    ctx = p.context()
+    ctx.set_material_fields(*th)
+    d_state, d_theta = ctx.vjp(to_dev(u), to_dev(e), to_dev(kap), to_dev(lam))
+    got = d_state.cpu().numpy()
+    kt, kb = kt_lambda(ref, lam, len(lam))
+    err = np.abs(got - kt)
+    assert (err <= 1e-12 * kb + 1e-300).all(), f"max err/bound {np.max(err / (kb + 1e-300)):.3e}"
+    # the cells of the upper layer have the middle axis node (beyond the bounds) as local node 0
+    assert all(C[c][0] == 11 for c in range(5, 10))
+
+    def L(E, ft, gf):
+        p.orc.set_theta(E, ft, gf)
+        R = p.orc.assemble(u, e, kap, want_k=False)["R"]
+        return 0.5 * np.sum((R - r_obs) ** 2)
+
+    got_t = d_theta.cpu().numpy()
+    base = [np.array(v, dtype=np.float64) for v in th]
+    fd = np.zeros_like(got_t)
+    for k in range(3):
+        for c in range(p.ncells):
+            h = 1e-6 * base[k][c]
+            plus = [v.copy() for v in base]
+            minus = [v.copy() for v in base]
+            plus[k][c] += h
+            minus[k][c] -= h
+            fd[k, c] = (L(*plus) - L(*minus)) / (2 * h)
+    p.orc.set_theta(*th)
+    scale = np.abs(fd).max(axis=1, keepdims=True)
+    assert (np.abs(got_t - fd) <= 1e-6 * scale).all(), f"max rel {np.max(np.abs(got_t - fd) / scale):.3e}"
 
 
 def test_outside_node_allocation_free():
