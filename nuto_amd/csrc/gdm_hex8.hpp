@@ -1168,12 +1168,6 @@ struct GdmHex8Plan
         nExtra = 0;
     }
 
-    // every node in the tables (the adjoint and the batched launches need that)
-    bool Complete() const
-    {
-        return nExtra == 0;
-    }
-
     bool Usable(int kind, int coordType, int dofType, int nip) const
     {
         return ready && kind == kGradientDamage && coordType == kHex8 && dofType == kHex8 && nip == 8;
@@ -1389,6 +1383,12 @@ struct GdmHex8Plan
             else
                 LaunchT<false>(a, wr, wk, state, stream);
         }
+        LaunchExtras(a, wr, wk, stream);
+    }
+
+    // R and K rows of the nodes beyond the node kernel's bounds (the generic row-owner body)
+    void LaunchExtras(const RowArgs& a, bool wr, bool wk, hipStream_t stream)
+    {
         if (nExtra > 0)
         {
             const dim3 grid((unsigned)((nExtra + 63) / 64));
@@ -1405,7 +1405,8 @@ struct GdmHex8Plan
         }
     }
 
-    // R + K of the nodes [first, end) only (one mesh of a batch), plain block order
+    // R + K of the nodes [first, end) only (one mesh of a batch), plain block order; the nodes beyond
+    // the bounds are not visited (LaunchExtras covers them for the whole plan)
     void LaunchRange(const RowArgs& a, int first, int end, hipStream_t stream)
     {
         if (end <= first)

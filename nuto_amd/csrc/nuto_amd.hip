@@ -2027,6 +2027,8 @@ void LaunchBatch(nuto_batch* b, int mode, const double* u, const double* e, cons
                 else
                     LaunchRange<kLocalDamage>(a, g.type, first, end, b->stream);
             }
+            if (IsGdm(b))
+                g.ctx->gdm.LaunchExtras(a, true, true, b->stream);
         }
     }
     Check(hipGetLastError(), "batch launch");
@@ -2328,9 +2330,8 @@ int nuto_batch_build(nuto_batch* b)
                 if (gdm)
                 {
                     PrepareGdm(c);
-                    Require(c->gdm.Usable(c->params.kind, c->coordType, c->f[0].type, c->nip) && c->gdm.Complete(),
-                            NUTO_ERR_UNSUPPORTED,
-                            "gradient-damage batch meshes need at most 8 cells and 32 neighbours per node");
+                    Require(c->gdm.Usable(c->params.kind, c->coordType, c->f[0].type, c->nip), NUTO_ERR_UNSUPPORTED,
+                            "gradient-damage batches need the hex8 node kernel's tables");
                 }
             }));
             G.nnz = nnz;

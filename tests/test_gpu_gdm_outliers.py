@@ -5,7 +5,7 @@ row-owner kernel, every other node's from the node kernel; R and K against the o
 outside node's rows bit-identical to the all-generic assembly, and the allocation-free entry point
 on the same mesh. The adjoint of that node (and d_theta of the cells whose local node 0 it is) comes
 from the adjoint's list kernel: d_state against K^T lambda of the oracle's K, d_theta against
-central finite differences of the oracle's R. Batches still need every node in bounds."""
+central finite differences of the oracle's R. Batches holding this mesh: tests/test_gpu_batch.py."""
 import numpy as np
 import pytest
 import torch
