@@ -1,10 +1,13 @@
 """Tensor Gauss rules above order 3 (Gauss1D by Newton iteration on the Legendre polynomial, any
 order the reference's IntegrationTypeTensorProduct takes up to 10; kernels for quad4 with 3, 4, 5
-points and hex8 with 3, 4 points per direction) against the oracle, which restates the reference's
+points and hex8 with 3, 4 points per direction, the hex8 cell matrices in each of their four forms)
+against the oracle, which restates the reference's
 own Newton iteration (Legendre.cpp:23-55): pattern bit-equal, R and K within 1e-12 of the
 contribution magnitudes, for linear elasticity, local damage and gradient damage. Gradient damage
 with other than 2x2x2 points runs the generic row-owner kernel (the node kernel and the adjoint
 take 8 IPs)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -17,8 +20,17 @@ pytestmark = pytest.mark.gpu
 
 def run(p, two=False, env=None):
     u, e, kap = p.inputs(u_range=4e-4, k_range=1.5e-4)
-    ctx = p.context()
-    R, K = ctx.build_vector_matrix(to_dev(u), to_dev(e), to_dev(kap))
+    old = {k: os.environ.get(k) for k in (env or {})}
+    os.environ.update(env or {})
+    try:
+        ctx = p.context()
+        R, K = ctx.build_vector_matrix(to_dev(u), to_dev(e), to_dev(kap))
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
     ref = p.orc.assemble(u, e, kap)
     rp, ci = ctx.pattern()
     assert np.array_equal(rp, ref["row_ptr"]) and np.array_equal(ci, ref["col"])
@@ -34,13 +46,14 @@ def test_quad4_linear_elastic(n):
     run(p)
 
 
+@pytest.mark.parametrize("form", ["0", "1", "2", "3"])  # cell matrices: columns, Gram (2 forms), MFMA
 @pytest.mark.parametrize("n", [3, 4])
 @pytest.mark.parametrize("kind", ["le", "ld"])
-def test_hex8_momentum(n, kind):
+def test_hex8_momentum(n, kind, form):
     X, C = make_mesh(N.HEX8, (3, 2, 2), (3.0, 2.0, 2.0), jitter=0.1, seed=5)
     p = Pair(N.HEX8, N.HEX8, X, C, *(le_params(3) if kind == "le" else ld_params()), False, rule=(0, n))
     assert p.nip == n ** 3
-    run(p)
+    run(p, env={"NUTO_AMD_CELL_GRAM": form})
 
 
 @pytest.mark.parametrize("n", [3, 4])
