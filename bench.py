@@ -95,7 +95,10 @@ def solve_bench(ctx, X, C, ranks, nn, K, R, iters, length):
     """ConstrainedSystemSolver::SolveTrialState on the assembled system: bottom face clamped, the top
     face's vertical displacement stepped by 1e-3 mm, eeq free; BiCGSTAB with the diagonal
     preconditioner for a fixed number of iterations (the tolerance is out of reach), timed after one
-    untimed solve that builds the constraint matrix."""
+    untimed solve that builds the constraint matrix. A solve also allocates its Krylov vectors and
+    forms the preconditioner; the per-iteration time is the difference of a solve with `iters`
+    iterations and one with 2, divided by iters - 2 (each the fastest of 3 solves: the allocations of
+    a solve take a variable time)."""
     import numpy as np
     import torch
 
@@ -113,18 +116,26 @@ def solve_bench(ctx, X, C, ranks, nn, K, R, iters, length):
     x = torch.empty(ctx.n_rows, dtype=torch.float64, device="cuda")
     ctx.solve_trial_state(K, R, None, rhs_d, method=N.SOLVER_BICGSTAB, tol=1e-300, max_iter=2, out=x)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    _, info = ctx.solve_trial_state(K, R, None, rhs_d, method=N.SOLVER_BICGSTAB, tol=1e-300, max_iter=iters, out=x)
-    torch.cuda.synchronize()
-    ms = 1000.0 * (time.perf_counter() - t0)
+
+    def timed(n_it):
+        t0 = time.perf_counter()
+        _, inf = ctx.solve_trial_state(K, R, None, rhs_d, method=N.SOLVER_BICGSTAB, tol=1e-300, max_iter=n_it,
+                                       out=x)
+        torch.cuda.synchronize()
+        return 1000.0 * (time.perf_counter() - t0), inf
+
+    iters = max(iters, 3)
+    ms2, info2 = min((timed(2) for _ in range(3)), key=lambda v: v[0])
+    ms, info = min((timed(iters) for _ in range(3)), key=lambda v: v[0])
     # the node product reads K's values once, each node's column list once for its 4 rows (3 u + 1
     # eeq), gathers x and writes y once
     spmv_bytes = 8 * ctx.nnz + 4 * ctx.nnz // 4 + 16 * ctx.n_rows
-    per_it = ms / max(info["iterations"], 1)
+    per_it = (ms - ms2) / max(info["iterations"] - info2["iterations"], 1)
     return {"workload": f"hex8 gradient damage {len(C)} cells: bottom face clamped, top face stepped, "
                         "BiCGSTAB + diagonal preconditioner (SolveTrialState), fixed iterations",
             "equations": len(eqs), "independent_dofs": info["independent_dofs"],
             "iterations": info["iterations"], "ms": ms, "ms_per_iteration": per_it,
+            "setup_ms": ms2 - info2["iterations"] * per_it,
             "k_products_per_iteration": 2,
             "k_product_GBps_lower_bound": 2 * spmv_bytes / (per_it * 1e-3) / 1e9,
             "relative_residual": info["relative_residual"]}

@@ -190,6 +190,99 @@ __global__ void __launch_bounds__(256) kBicgP(long long n, const double* __restr
         p[i] = r[i] + beta * (p[i] - w * v[i]);
 }
 
+// Two dot products in one pass: partial[b] = (a . b2) over block b's elements, partial[gridDim + b] =
+// (c . d) (same element order and tree as kDotPartial)
+__device__ inline void BlockTree2(double s0, double s1, double* __restrict__ partial)
+{
+    __shared__ double sh[2][256];
+    sh[0][threadIdx.x] = s0;
+    sh[1][threadIdx.x] = s1;
+    __syncthreads();
+    for (int m = 128; m > 0; m >>= 1)
+    {
+        if ((int)threadIdx.x < m)
+        {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + m];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + m];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        partial[blockIdx.x] = sh[0][0];
+        partial[gridDim.x + blockIdx.x] = sh[1][0];
+    }
+}
+
+__global__ void __launch_bounds__(256) kDot2Partial(long long n, const double* __restrict__ a,
+                                                    const double* __restrict__ b, const double* __restrict__ c,
+                                                    const double* __restrict__ d, double* __restrict__ partial)
+{
+    double s0 = 0.0, s1 = 0.0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    {
+        s0 += a[i] * b[i];
+        s1 += c[i] * d[i];
+    }
+    BlockTree2(s0, s1, partial);
+}
+
+// out[0], out[1] = the sums of the two sets of np partials (one block of 512 threads)
+__global__ void __launch_bounds__(512) kDot2Final(int np, const double* __restrict__ partial, double* __restrict__ out)
+{
+    __shared__ double sh[2][256];
+    const int h = (int)threadIdx.x >> 8, t = (int)threadIdx.x & 255;
+    double s = 0.0;
+    for (int i = t; i < np; i += 256)
+        s += partial[h * np + i];
+    sh[h][t] = s;
+    __syncthreads();
+    for (int m = 128; m > 0; m >>= 1)
+    {
+        if (t < m)
+            sh[h][t] += sh[h][t + m];
+        __syncthreads();
+    }
+    if (t == 0)
+        out[h] = sh[h][0];
+}
+
+// BiCGSTAB's s = r - alpha v and z = invd s (the preconditioned s)
+__global__ void __launch_bounds__(256) kBicgS(long long n, const double* __restrict__ r, double alpha,
+                                              const double* __restrict__ v, const double* __restrict__ invd,
+                                              double* __restrict__ s, double* __restrict__ z)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n)
+    {
+        const double si = -alpha * v[i] + r[i];
+        s[i] = si;
+        z[i] = invd[i] * si;
+    }
+}
+
+// BiCGSTAB's update x += alpha y, x += w z, r = s - w t, with the partial sums of (r . r) and
+// (r0 . r) of the new residual (kDot2Partial's element order and layout; grid-stride over n)
+__global__ void __launch_bounds__(256) kBicgUpdate(long long n, double alpha, const double* __restrict__ y, double w,
+                                                   const double* __restrict__ z, double* __restrict__ x,
+                                                   const double* __restrict__ s, const double* __restrict__ t,
+                                                   double* __restrict__ r, const double* __restrict__ r0,
+                                                   double* __restrict__ partial)
+{
+    double s0 = 0.0, s1 = 0.0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    {
+        double xi = alpha * y[i] + x[i];
+        xi = w * z[i] + xi;
+        x[i] = xi;
+        const double ri = -w * t[i] + s[i];
+        r[i] = ri;
+        s0 += ri * ri;
+        s1 += r0[i] * ri;
+    }
+    BlockTree2(s0, s1, partial);
+}
+
 // delta[dep[e]] = rhsNew[e] - rhsOld[e] (dependent dofs are distinct)
 __global__ void __launch_bounds__(256) kScatterDelta(int nEq, const int* __restrict__ dep,
                                                      const double* __restrict__ rhsOld,
@@ -238,7 +331,7 @@ public:
     {
         hipError_t e;
         if ((e = t1.Alloc(mN * sizeof(double))) != hipSuccess || (e = t2.Alloc(mN * sizeof(double))) != hipSuccess ||
-            (e = part.Alloc(kDotBlocks * sizeof(double))) != hipSuccess ||
+            (e = part.Alloc(2 * kDotBlocks * sizeof(double))) != hipSuccess ||
             (e = scal.Alloc(8 * sizeof(double))) != hipSuccess)
             return e;
         return hipHostMalloc(&mHost, 8 * sizeof(double), hipHostMallocDefault);
@@ -295,6 +388,32 @@ public:
             (err = hipStreamSynchronize(mS)) != hipSuccess)
             return 0.0;
         return mHost[0];
+    }
+    // (a . b) and (c . d) with one host read: out[0], out[1]
+    void Dot2(long long n, const double* a, const double* b, const double* c, const double* d, double out[2],
+              hipError_t& err)
+    {
+        const int g = DotGrid(n);
+        hipLaunchKernelGGL(kDot2Partial, dim3(g), dim3(256), 0, mS, n, a, b, c, d, part.As<double>());
+        Dot2Read(g, out, err);
+    }
+    // the two sums of the 2 x g partials in `part`
+    void Dot2Read(int g, double out[2], hipError_t& err)
+    {
+        hipLaunchKernelGGL(kDot2Final, dim3(1), dim3(512), 0, mS, g, part.As<const double>(), scal.As<double>());
+        if ((err = hipGetLastError()) != hipSuccess ||
+            (err = hipMemcpyAsync(mHost, scal.p, 2 * sizeof(double), hipMemcpyDeviceToHost, mS)) != hipSuccess ||
+            (err = hipStreamSynchronize(mS)) != hipSuccess)
+        {
+            out[0] = out[1] = 0.0;
+            return;
+        }
+        out[0] = mHost[0];
+        out[1] = mHost[1];
+    }
+    static int DotGrid(long long n)
+    {
+        return (int)std::min<long long>(kDotBlocks, (n + 255) / 256 > 0 ? (n + 255) / 256 : 1);
     }
     void Axpby(long long n, double a, const double* x, double b, double* y)
     {
@@ -384,12 +503,13 @@ public:
         Axpby(m, 0.0, nullptr, 0.0, V);
         double r0sq = bb, rho = 1.0, alpha = 1.0, w = 1.0;
         const double tol2 = tol * tol * bb, eps2 = 2.220446049250313e-16 * 2.220446049250313e-16;
-        double rr = bb;
+        double rr = bb, rhoNext = bb;  // (r . r) and (r0 . r) of the current residual (r = r0 = b)
+        const unsigned vg = (unsigned)((m + 255) / 256);
         int i = 0, restarts = 0;
         while (rr > tol2 && i < maxIt && err == hipSuccess)
         {
             const double rhoOld = rho;
-            rho = Dot(m, R0, R, err);
+            rho = rhoNext;
             if (std::fabs(rho) < eps2 * r0sq)
             {
                 // restart with the current residual r = b - A x
@@ -405,25 +525,23 @@ public:
                     i = 0;
             }
             const double beta = (rho / rhoOld) * (alpha / w);
-            hipLaunchKernelGGL(kBicgP, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, mS, m, R, beta, w, V, P);
+            hipLaunchKernelGGL(kBicgP, dim3(vg), dim3(256), 0, mS, m, R, beta, w, V, P);
             Precond(m, invd, P, Y);
             if ((err = Apply(Y, V)) != hipSuccess)
                 break;
             alpha = rho / Dot(m, R0, V, err);
-            if ((err = hipMemcpyAsync(S, R, m * 8, hipMemcpyDeviceToDevice, mS)) != hipSuccess)
-                break;
-            Axpby(m, -alpha, V, 1.0, S);
-            Precond(m, invd, S, Z);
+            hipLaunchKernelGGL(kBicgS, dim3(vg), dim3(256), 0, mS, m, R, alpha, V, invd, S, Z);
             if ((err = Apply(Z, T)) != hipSuccess)
                 break;
-            const double tt = Dot(m, T, T, err);
-            w = tt > 0.0 ? Dot(m, T, S, err) / tt : 0.0;
-            Axpby(m, alpha, Y, 1.0, x);
-            Axpby(m, w, Z, 1.0, x);
-            if ((err = hipMemcpyAsync(R, S, m * 8, hipMemcpyDeviceToDevice, mS)) != hipSuccess)
-                break;
-            Axpby(m, -w, T, 1.0, R);
-            rr = Dot(m, R, R, err);
+            double d2[2];
+            Dot2(m, T, T, T, S, d2, err);
+            w = d2[0] > 0.0 ? d2[1] / d2[0] : 0.0;
+            const int g = DotGrid(m);
+            hipLaunchKernelGGL(kBicgUpdate, dim3(g), dim3(256), 0, mS, m, alpha, Y, w, Z, x, S, T, R, R0,
+                               part.As<double>());
+            Dot2Read(g, d2, err);
+            rr = d2[0];
+            rhoNext = d2[1];
             ++i;
         }
         iters = i;

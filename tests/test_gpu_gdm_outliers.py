@@ -65,9 +65,10 @@ def test_outside_node_by_the_generic_kernel(monkeypatch):
         assert Rn[r] == Rg[r], r
 
 
-def test_outside_node_adjoint():
+@pytest.mark.parametrize("app", [N.FULL, N.UNILATERAL])
+def test_outside_node_adjoint(app):
     X, C = fan_mesh()
-    p = Pair(N.HEX8, N.HEX8, X, C, *gdm_params(), True)
+    p = Pair(N.HEX8, N.HEX8, X, C, *gdm_params(app), True)
     th = S.material_fields(p.ncells)
     u, e, kap = p.inputs(u_range=4e-4, k_range=1.5e-4)
     p.orc.set_theta(*S.perturbed_fields(*th))
