@@ -58,6 +58,10 @@ struct nuto_ctx
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
     std::string err;
+    // the constrained solves' vectors and the pinned word their reductions are read through, kept
+    // between solves (a Newton step solves once per iteration); grown when a solve needs more
+    DevBuf solveWork;
+    double* solveHost = nullptr;
     // cells
     bool cellsSet = false;
     int coordType = 0, dim = 0, nCells = 0, nCoordNodes = 0;
@@ -843,6 +847,8 @@ int nuto_destroy(nuto_ctx* ctx)
         return NUTO_ERR_ARGUMENT;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->solveHost)
+        (void)hipHostFree(ctx->solveHost);
     if (ctx->own)
         (void)hipStreamDestroy(ctx->own);
     delete ctx;
@@ -1417,6 +1423,19 @@ NodeRows KNodeRows(const nuto_ctx* ctx)
     return r;
 }
 
+// The context's solve workspace, at least ReducedSolver::WorkDoubles(n_rows, n_independent) doubles
+// (allocated once, or again when the constraints leave more independent dofs), and its pinned word.
+// The size depends on the context only, so a solve inside a Newton step never reallocates it.
+double* SolveWork(nuto_ctx* ctx)
+{
+    const size_t need = ReducedSolver::WorkDoubles(ctx->nRows, ctx->nInd) * sizeof(double);
+    if (ctx->solveWork.bytes < need)
+        Check(ctx->solveWork.Alloc(need), "solver buffers");
+    if (!ctx->solveHost)
+        Check(hipHostMalloc(&ctx->solveHost, 8 * sizeof(double), hipHostMallocDefault), "solver buffers");
+    return ctx->solveWork.As<double>();
+}
+
 void SolveConstrained(nuto_ctx* ctx, const double* K, const double* f, const double* rhsOld, const double* rhsNew,
                       bool trial, double* x, const nuto_solver_options* opt, nuto_solve_info* info)
 {
@@ -1431,42 +1450,40 @@ void SolveConstrained(nuto_ctx* ctx, const double* K, const double* f, const dou
     Csr Kc{ctx->nRows, ctx->rowPtr.As<int>(), ctx->col.As<int>(), K};
     Csr Cc{ctx->nRows, ctx->cRow.As<int>(), ctx->cCol.As<int>(), ctx->cVal.As<double>()};
     Csr Ct{m, ctx->ctRow.As<int>(), ctx->ctCol.As<int>(), ctx->ctVal.As<double>()};
-    ReducedSolver rs(ctx->stream, Kc, Cc, Ct, n, m, KNodeRows(ctx));
-    Check(rs.Alloc(), "solver buffers");
-    DevBuf invd, b, u, g, delta;
-    Check(invd.Alloc((size_t)m * 8), "solver buffers");
-    Check(b.Alloc((size_t)m * 8), "solver buffers");
-    Check(u.Alloc((size_t)m * 8), "solver buffers");
+    ReducedSolver rs(ctx->stream, Kc, Cc, Ct, n, m, KNodeRows(ctx), SolveWork(ctx), ctx->solveHost);
+    double* delta = rs.Extra();
+    double* g = delta + n;
+    double* invd = g + n;
+    double* b = invd + m;
+    double* u = b + m;
     if (m > 0)
         hipLaunchKernelGGL(kDiagCtKC, dim3((m + 255) / 256), dim3(256), 0, ctx->stream, m, Ct.row, Ct.col, Ct.val,
-                           Kc.row, Kc.col, Kc.val, invd.As<double>());
+                           Kc.row, Kc.col, Kc.val, invd);
     const double* rhs = f;
     if (trial)
     {
         // g = f + K delta, delta = rhs(new) - rhs(old) at the dependent dofs (GetSparseGlobalRhs)
-        Check(delta.Alloc((size_t)n * 8), "solver buffers");
-        Check(g.Alloc((size_t)n * 8), "solver buffers");
-        rs.Axpby(n, 0.0, nullptr, 0.0, delta.As<double>());
+        rs.Axpby(n, 0.0, nullptr, 0.0, delta);
         if (ctx->nEq > 0)
             hipLaunchKernelGGL(kScatterDelta, dim3((ctx->nEq + 255) / 256), dim3(256), 0, ctx->stream, ctx->nEq,
-                               ctx->consDep.As<int>(), rhsOld, rhsNew, delta.As<double>());
-        rs.SpmvK(delta.As<double>(), g.As<double>());
-        rs.Axpby(n, 1.0, f, 1.0, g.As<double>());
-        rhs = g.As<double>();
+                               ctx->consDep.As<int>(), rhsOld, rhsNew, delta);
+        rs.SpmvK(delta, g);
+        rs.Axpby(n, 1.0, f, 1.0, g);
+        rhs = g;
     }
-    rs.Spmv(Ct, rhs, b.As<double>());
+    rs.Spmv(Ct, rhs, b);
     Check(hipGetLastError(), "solver setup");
     const double tol = opt && opt->tolerance > 0.0 ? opt->tolerance : 2.220446049250313e-16;
     const int maxIt = opt && opt->max_iterations > 0 ? opt->max_iterations : 2 * m;
     int iters = 0;
     double rel = 0.0;
     if (method == NUTO_SOLVER_CG)
-        Check(rs.Cg(b.As<double>(), invd.As<double>(), u.As<double>(), tol, maxIt, iters, rel), "conjugate gradients");
+        Check(rs.Cg(b, invd, u, tol, maxIt, iters, rel), "conjugate gradients");
     else
-        Check(rs.Bicgstab(b.As<double>(), invd.As<double>(), u.As<double>(), tol, maxIt, iters, rel), "BiCGSTAB");
-    rs.Spmv(Cc, u.As<double>(), x);  // x = C u
+        Check(rs.Bicgstab(b, invd, u, tol, maxIt, iters, rel), "BiCGSTAB");
+    rs.Spmv(Cc, u, x);  // x = C u
     if (trial)
-        rs.Axpby(n, -1.0, delta.As<double>(), 1.0, x);  // the negative increment: C u - delta
+        rs.Axpby(n, -1.0, delta, 1.0, x);  // the negative increment: C u - delta
     Check(hipGetLastError(), "solver result");
     Check(hipStreamSynchronize(ctx->stream), "solve");
     if (info)
@@ -1508,8 +1525,9 @@ void QuasistaticStep(nuto_ctx* ctx, double* x, double* kappa, const double* rhsO
     Csr Kc{ctx->nRows, ctx->rowPtr.As<int>(), ctx->col.As<int>(), K.As<double>()};
     Csr Cc{ctx->nRows, ctx->cRow.As<int>(), ctx->cCol.As<int>(), ctx->cVal.As<double>()};
     Csr Ct{m, ctx->ctRow.As<int>(), ctx->ctCol.As<int>(), ctx->ctVal.As<double>()};
-    ReducedSolver rs(ctx->stream, Kc, Cc, Ct, n, m, KNodeRows(ctx));
-    Check(rs.Alloc(), "Newton buffers");
+    // the norms use the solve workspace's reduction slots between solves (all on one stream, each
+    // read before the next solve starts)
+    ReducedSolver rs(ctx->stream, Kc, Cc, Ct, n, m, KNodeRows(ctx), SolveWork(ctx), ctx->solveHost);
 
     auto E = [&](const double* v) -> const double* { return two ? v + nU : nullptr; };
     // QuasistaticSolver::Norm: |C^T v|

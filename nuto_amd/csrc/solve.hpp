@@ -302,8 +302,9 @@ struct Csr
     const double* val = nullptr;
 };
 
-// Krylov solver on the operator C^T K C. Buffers are allocated per solve; the stream orders every
-// launch, and the host reads the scalars it branches on (one small copy per reduction).
+// Krylov solver on the operator C^T K C. Its vectors live in a workspace the caller keeps between
+// solves; the stream orders every launch, and the host reads the scalars it branches on (one small
+// copy per reduction or pair of reductions).
 // node structure of an assembled K (rows per node of field 0 and 1), for the node product
 struct NodeRows
 {
@@ -313,33 +314,39 @@ struct NodeRows
 class ReducedSolver
 {
 public:
-    ReducedSolver(hipStream_t s, Csr K, Csr C, Csr Ct, long long nFull, int nInd, NodeRows nodes = NodeRows())
-        : mS(s), mK(K), mC(C), mCt(Ct), mN(nFull), mM(nInd), mNodes(nodes)
+    // doubles of the workspace a constrained solve uses (SolveConstrained's vectors included): the
+    // operator's two full-length temporaries, the reductions' partials and results, 8 Krylov vectors
+    // of the reduced size, then (for the caller) 2 full-length and 3 reduced vectors
+    static size_t WorkDoubles(long long nFull, int nInd)
     {
+        return 4 * (size_t)nFull + 11 * (size_t)nInd + 2 * kDotBlocks + 8;
+    }
+    // work: WorkDoubles(nFull, nInd) doubles of device memory; host: 8 doubles of pinned memory (the
+    // reductions' results). Both belong to the caller and are reused between solves.
+    ReducedSolver(hipStream_t s, Csr K, Csr C, Csr Ct, long long nFull, int nInd, NodeRows nodes, double* work,
+                  double* host)
+        : mS(s), mK(K), mC(C), mCt(Ct), mM(nInd), mNodes(nodes), mHost(host)
+    {
+        t1 = work;
+        t2 = t1 + nFull;
+        part = t2 + nFull;
+        scal = part + 2 * kDotBlocks;
+        krylov = scal + 8;
+        extra = krylov + 8 * (size_t)nInd;
+    }
+    // the caller's vectors after the solver's own: 2 of length nFull, then 3 of length nInd
+    double* Extra() const
+    {
+        return extra;
     }
 
     // y (nInd) = C^T K C x (nInd)
     hipError_t Apply(const double* x, double* y)
     {
-        Spmv(mC, x, t1.As<double>());
-        SpmvK(t1.As<double>(), t2.As<double>());
-        Spmv(mCt, t2.As<double>(), y);
+        Spmv(mC, x, t1);
+        SpmvK(t1, t2);
+        Spmv(mCt, t2, y);
         return hipGetLastError();
-    }
-
-    hipError_t Alloc()
-    {
-        hipError_t e;
-        if ((e = t1.Alloc(mN * sizeof(double))) != hipSuccess || (e = t2.Alloc(mN * sizeof(double))) != hipSuccess ||
-            (e = part.Alloc(2 * kDotBlocks * sizeof(double))) != hipSuccess ||
-            (e = scal.Alloc(8 * sizeof(double))) != hipSuccess)
-            return e;
-        return hipHostMalloc(&mHost, 8 * sizeof(double), hipHostMallocDefault);
-    }
-    ~ReducedSolver()
-    {
-        if (mHost)
-            (void)hipHostFree(mHost);
     }
 
     // C and C^T: one or two entries per row, one lane per row
@@ -381,10 +388,10 @@ public:
     double Dot(long long n, const double* a, const double* b, hipError_t& err)
     {
         const int g = (int)std::min<long long>(kDotBlocks, (n + 255) / 256 > 0 ? (n + 255) / 256 : 1);
-        hipLaunchKernelGGL(kDotPartial, dim3(g), dim3(256), 0, mS, n, a, b, part.As<double>());
-        hipLaunchKernelGGL(kDotFinal, dim3(1), dim3(256), 0, mS, g, part.As<const double>(), scal.As<double>());
+        hipLaunchKernelGGL(kDotPartial, dim3(g), dim3(256), 0, mS, n, a, b, part);
+        hipLaunchKernelGGL(kDotFinal, dim3(1), dim3(256), 0, mS, g, part, scal);
         if ((err = hipGetLastError()) != hipSuccess ||
-            (err = hipMemcpyAsync(mHost, scal.p, sizeof(double), hipMemcpyDeviceToHost, mS)) != hipSuccess ||
+            (err = hipMemcpyAsync(mHost, scal, sizeof(double), hipMemcpyDeviceToHost, mS)) != hipSuccess ||
             (err = hipStreamSynchronize(mS)) != hipSuccess)
             return 0.0;
         return mHost[0];
@@ -394,15 +401,15 @@ public:
               hipError_t& err)
     {
         const int g = DotGrid(n);
-        hipLaunchKernelGGL(kDot2Partial, dim3(g), dim3(256), 0, mS, n, a, b, c, d, part.As<double>());
+        hipLaunchKernelGGL(kDot2Partial, dim3(g), dim3(256), 0, mS, n, a, b, c, d, part);
         Dot2Read(g, out, err);
     }
     // the two sums of the 2 x g partials in `part`
     void Dot2Read(int g, double out[2], hipError_t& err)
     {
-        hipLaunchKernelGGL(kDot2Final, dim3(1), dim3(512), 0, mS, g, part.As<const double>(), scal.As<double>());
+        hipLaunchKernelGGL(kDot2Final, dim3(1), dim3(512), 0, mS, g, part, scal);
         if ((err = hipGetLastError()) != hipSuccess ||
-            (err = hipMemcpyAsync(mHost, scal.p, 2 * sizeof(double), hipMemcpyDeviceToHost, mS)) != hipSuccess ||
+            (err = hipMemcpyAsync(mHost, scal, 2 * sizeof(double), hipMemcpyDeviceToHost, mS)) != hipSuccess ||
             (err = hipStreamSynchronize(mS)) != hipSuccess)
         {
             out[0] = out[1] = 0.0;
@@ -431,12 +438,8 @@ public:
     hipError_t Cg(const double* b, const double* invd, double* x, double tol, int maxIt, int& iters, double& relRes)
     {
         const long long m = mM;
-        DevBuf r, z, p, q;
-        hipError_t err;
-        if ((err = r.Alloc(m * 8)) != hipSuccess || (err = z.Alloc(m * 8)) != hipSuccess ||
-            (err = p.Alloc(m * 8)) != hipSuccess || (err = q.Alloc(m * 8)) != hipSuccess)
-            return err;
-        double *R = r.As<double>(), *Z = z.As<double>(), *P = p.As<double>(), *Q = q.As<double>();
+        hipError_t err = hipSuccess;
+        double *R = krylov, *Z = krylov + m, *P = krylov + 2 * m, *Q = krylov + 3 * m;
         iters = 0;
         relRes = 0.0;
         Axpby(m, 0.0, nullptr, 0.0, x);
@@ -480,14 +483,9 @@ public:
                         double& relRes)
     {
         const long long m = mM;
-        DevBuf bufs[8];
-        hipError_t err;
-        for (DevBuf& d : bufs)
-            if ((err = d.Alloc(m * 8)) != hipSuccess)
-                return err;
-        double *R = bufs[0].As<double>(), *R0 = bufs[1].As<double>(), *P = bufs[2].As<double>(),
-               *V = bufs[3].As<double>(), *Y = bufs[4].As<double>(), *Z = bufs[5].As<double>(),
-               *S = bufs[6].As<double>(), *T = bufs[7].As<double>();
+        hipError_t err = hipSuccess;
+        double *R = krylov, *R0 = krylov + m, *P = krylov + 2 * m, *V = krylov + 3 * m, *Y = krylov + 4 * m,
+               *Z = krylov + 5 * m, *S = krylov + 6 * m, *T = krylov + 7 * m;
         iters = 0;
         relRes = 0.0;
         Axpby(m, 0.0, nullptr, 0.0, x);
@@ -538,7 +536,7 @@ public:
             w = d2[0] > 0.0 ? d2[1] / d2[0] : 0.0;
             const int g = DotGrid(m);
             hipLaunchKernelGGL(kBicgUpdate, dim3(g), dim3(256), 0, mS, m, alpha, Y, w, Z, x, S, T, R, R0,
-                               part.As<double>());
+                               part);
             Dot2Read(g, d2, err);
             rr = d2[0];
             rhoNext = d2[1];
@@ -549,15 +547,14 @@ public:
         return err;
     }
 
-    DevBuf t1, t2, part, scal;
 
 private:
     hipStream_t mS;
     Csr mK, mC, mCt;
-    long long mN;
     int mM;
     NodeRows mNodes;
     double* mHost = nullptr;
+    double *t1 = nullptr, *t2 = nullptr, *part = nullptr, *scal = nullptr, *krylov = nullptr, *extra = nullptr;
 };
 
 }  // namespace nuto_amd
