@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <thread>
 #include <unordered_map>
 
 #include "elements.hpp"
@@ -199,7 +200,7 @@ bool DofLayer(int coordType, int nCells, const int* coordConn, int dofType, std:
     return true;
 }
 
-bool BuildNodeGraph(int nCells, int nnD, const int* ranks, int nNodes, NodeGraph& g, std::string& err)
+bool BuildNodeGraph(int nCells, int nnD, const int* ranks, int nNodes, NodeGraph& g, std::string& err, int maxThreads)
 {
     g = NodeGraph();
     g.nNodes = nNodes;
@@ -238,47 +239,83 @@ bool BuildNodeGraph(int nCells, int nnD, const int* ranks, int nNodes, NodeGraph
                 cur[r]++;
             }
     }
+    // neighbourhoods and slots per node, in node ranges on several threads (each node's result
+    // depends on its own incidence only; the ranges' neighbourhoods are concatenated in node order)
     g.adjPtr.assign((size_t)nNodes + 1, 0);
-    g.adj.clear();
-    g.adj.reserve(nInc * 4);
     g.slot.resize(nInc * nnD);
-    std::vector<int> buf;
-    std::vector<char> seen;
-    for (int p = 0; p < nNodes; p++)
+    const unsigned hw = std::thread::hardware_concurrency();
+    const unsigned cap = maxThreads > 0 ? (unsigned)maxThreads : 16u;
+    const int nThreads = (int)std::max(1u, std::min({hw ? hw : 1u, cap, (unsigned)(nNodes / 16384 + 1)}));
+    std::vector<std::vector<int>> part(nThreads);
+    std::vector<char> tooLarge(nThreads, 0);
+    auto work = [&](int t) {
+        const int p0 = (int)((long long)nNodes * t / nThreads), p1 = (int)((long long)nNodes * (t + 1) / nThreads);
+        std::vector<int>& out = part[t];
+        std::vector<int> buf;
+        std::vector<char> seen;
+        for (int p = p0; p < p1; p++)
+        {
+            buf.clear();
+            for (int i = g.incPtr[p]; i < g.incPtr[p + 1]; i++)
+                for (int b = 0; b < nnD; b++)
+                    buf.push_back(ranks[(size_t)g.incCell[i] * nnD + b]);
+            std::sort(buf.begin(), buf.end());
+            buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+            if (buf.size() >= 32768)
+            {
+                tooLarge[t] = 1;
+                return;
+            }
+            g.adjPtr[p + 1] = (int)buf.size();  // the length; prefix sums below
+            out.insert(out.end(), buf.begin(), buf.end());
+            seen.assign(buf.size(), 0);
+            for (int i = g.incPtr[p]; i < g.incPtr[p + 1]; i++)
+                for (int b = 0; b < nnD; b++)
+                {
+                    const int q = ranks[(size_t)g.incCell[i] * nnD + b];
+                    const int s = (int)(std::lower_bound(buf.begin(), buf.end(), q) - buf.begin());
+                    unsigned short v = (unsigned short)s;
+                    if (!seen[s])
+                    {
+                        seen[s] = 1;
+                        v |= 0x8000;
+                    }
+                    g.slot[(size_t)i * nnD + b] = v;
+                }
+        }
+    };
+    if (nThreads == 1)
+        work(0);
+    else
     {
-        buf.clear();
-        for (int i = g.incPtr[p]; i < g.incPtr[p + 1]; i++)
-            for (int b = 0; b < nnD; b++)
-                buf.push_back(ranks[(size_t)g.incCell[i] * nnD + b]);
-        std::sort(buf.begin(), buf.end());
-        buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
-        if (buf.size() >= 32768)
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nThreads; t++)
+            pool.emplace_back(work, t);
+        for (std::thread& th : pool)
+            th.join();
+    }
+    for (int t = 0; t < nThreads; t++)
+        if (tooLarge[t])
         {
             err = "node neighbourhood too large for 15-bit slots";
             return false;
         }
-        const size_t base = g.adj.size();
-        g.adj.insert(g.adj.end(), buf.begin(), buf.end());
-        g.adjPtr[p + 1] = (int)g.adj.size();
-        seen.assign(buf.size(), 0);
-        for (int i = g.incPtr[p]; i < g.incPtr[p + 1]; i++)
-            for (int b = 0; b < nnD; b++)
-            {
-                const int q = ranks[(size_t)g.incCell[i] * nnD + b];
-                const int s = (int)(std::lower_bound(g.adj.begin() + base, g.adj.end(), q) - (g.adj.begin() + base));
-                unsigned short v = (unsigned short)s;
-                if (!seen[s])
-                {
-                    seen[s] = 1;
-                    v |= 0x8000;
-                }
-                g.slot[(size_t)i * nnD + b] = v;
-            }
-        if (g.adj.size() > (size_t)INT_MAX)
-        {
-            err = "node graph too large";
-            return false;
-        }
+    size_t total = 0;
+    for (const std::vector<int>& v : part)
+        total += v.size();
+    if (total > (size_t)INT_MAX)
+    {
+        err = "node graph too large";
+        return false;
+    }
+    for (int p = 0; p < nNodes; p++)
+        g.adjPtr[p + 1] += g.adjPtr[p];
+    g.adj.resize(total);
+    size_t at = 0;
+    for (const std::vector<int>& v : part)
+    {
+        std::copy(v.begin(), v.end(), g.adj.begin() + (long)at);
+        at += v.size();
     }
     return true;
 }

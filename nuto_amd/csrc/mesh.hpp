@@ -35,7 +35,9 @@ struct NodeGraph
     std::vector<unsigned short> slot;
 };
 
-bool BuildNodeGraph(int nCells, int nnD, const int* ranks, int nNodes, NodeGraph& g, std::string& err);
+// Large graphs are built on up to 16 threads (maxThreads > 0 caps that; 1: the calling thread only).
+bool BuildNodeGraph(int nCells, int nnD, const int* ranks, int nNodes, NodeGraph& g, std::string& err,
+                    int maxThreads = 0);
 
 // Monolithic CSR row pointer over fields sharing one node graph: rows are field 0 nodes x ncomp0
 // then field 1 nodes x ncomp1 (DofMatrixSparseConvertEigen.h:40-67 block order), every row of

@@ -1,6 +1,8 @@
 // C ABI of the MI355X assembly path (declared in include/nuto_amd.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cstdlib>
 #include <cstring>
@@ -8,6 +10,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/nuto_amd.h"
@@ -2267,6 +2270,38 @@ int nuto_batch_build(nuto_batch* b)
                 if (bm.ct == g.ct && bm.dt == g.dt)
                     bound += (long long)bm.nCells * nd * nd;
             const bool exact = bound > kMaxNnz;
+            // exact nonzeros of the type's meshes, the meshes on several threads
+            std::vector<long long> mnzOf(b->meshes.size(), 0);
+            if (exact)
+            {
+                std::vector<int> todo;
+                for (size_t m = 0; m < b->meshes.size(); m++)
+                    if (b->meshes[m].ct == g.ct && b->meshes[m].dt == g.dt)
+                        todo.push_back((int)m);
+                std::vector<std::string> errs(todo.size());
+                std::atomic<size_t> next{0};
+                auto count = [&] {
+                    for (size_t k; (k = next.fetch_add(1)) < todo.size();)
+                    {
+                        const BatchMesh& bm = b->meshes[todo[k]];
+                        NodeGraph mg;
+                        if (BuildNodeGraph(bm.nCells, NodesOf(bm.dt), bm.ranks.data(), bm.nDof, mg, errs[k], 1))
+                            mnzOf[todo[k]] = (long long)nc * nc * (long long)mg.adj.size();
+                        else if (errs[k].empty())
+                            errs[k] = "node graph";
+                    }
+                };
+                const unsigned hw = std::thread::hardware_concurrency();
+                const int nt = (int)std::max(1u, std::min({hw ? hw : 1u, 16u, (unsigned)todo.size()}));
+                std::vector<std::thread> pool;
+                for (int t = 1; t < nt; t++)
+                    pool.emplace_back(count);
+                count();
+                for (std::thread& th : pool)
+                    th.join();
+                for (const std::string& e : errs)
+                    Require(e.empty(), NUTO_ERR_ARGUMENT, e);
+            }
             long long nodes = 0, nnz = 0;
             for (size_t m = 0; m < b->meshes.size(); m++)
             {
@@ -2275,17 +2310,9 @@ int nuto_batch_build(nuto_batch* b)
                     continue;
                 Require(!gdm || (g.ct == kHex8 && g.dt == kHex8), NUTO_ERR_UNSUPPORTED,
                         "gradient-damage batches hold hex8 meshes only");
-                long long mnz = 0;
-                if (exact)
-                {
-                    NodeGraph mg;
-                    std::string err;
-                    Require(BuildNodeGraph(bm.nCells, NodesOf(bm.dt), bm.ranks.data(), bm.nDof, mg, err),
-                            NUTO_ERR_ARGUMENT, err);
-                    mnz = (long long)nc * nc * (long long)mg.adj.size();
-                    Require(mnz <= kMaxNnz, NUTO_ERR_UNSUPPORTED,
-                            "a mesh of the batch exceeds 2^31 - 1 nonzeros (int32 CSR offsets)");
-                }
+                const long long mnz = mnzOf[m];
+                Require(mnz <= kMaxNnz, NUTO_ERR_UNSUPPORTED,
+                        "a mesh of the batch exceeds 2^31 - 1 nonzeros (int32 CSR offsets)");
                 if (!g.meshes.empty() && (nodes + bm.nDof > cap || nnz + mnz > kMaxNnz))
                 {
                     plan.push_back(g);

@@ -283,6 +283,27 @@ __global__ void __launch_bounds__(256) kBicgUpdate(long long n, double alpha, co
     BlockTree2(s0, s1, partial);
 }
 
+// Conjugate gradients' update x += alpha p, r -= alpha q, z = invd r, with the partial sums of
+// (r . r) and (r . z) (kDot2Partial's element order and layout; grid-stride over n)
+__global__ void __launch_bounds__(256) kCgUpdate(long long n, double alpha, const double* __restrict__ p,
+                                                 const double* __restrict__ q, double* __restrict__ x,
+                                                 double* __restrict__ r, const double* __restrict__ invd,
+                                                 double* __restrict__ z, double* __restrict__ partial)
+{
+    double s0 = 0.0, s1 = 0.0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    {
+        x[i] = alpha * p[i] + x[i];
+        const double ri = -alpha * q[i] + r[i];
+        r[i] = ri;
+        const double zi = invd[i] * ri;
+        z[i] = zi;
+        s0 += ri * ri;
+        s1 += ri * zi;
+    }
+    BlockTree2(s0, s1, partial);
+}
+
 // delta[dep[e]] = rhsNew[e] - rhsOld[e] (dependent dofs are distinct)
 __global__ void __launch_bounds__(256) kScatterDelta(int nEq, const int* __restrict__ dep,
                                                      const double* __restrict__ rhsOld,
@@ -462,14 +483,16 @@ public:
             if ((err = Apply(P, Q)) != hipSuccess)
                 break;
             const double alpha = absNew / Dot(m, P, Q, err);
-            Axpby(m, alpha, P, 1.0, x);
-            Axpby(m, -alpha, Q, 1.0, R);
-            rr = Dot(m, R, R, err);
+            // x, r, z = invd r and the pair (r . r), (r . z) in one pass, one host read
+            const int g = DotGrid(m);
+            hipLaunchKernelGGL(kCgUpdate, dim3(g), dim3(256), 0, mS, m, alpha, P, Q, x, R, invd, Z, part);
+            double d2[2];
+            Dot2Read(g, d2, err);
+            rr = d2[0];
             if (rr < thr)
                 break;
-            Precond(m, invd, R, Z);
             const double absOld = absNew;
-            absNew = Dot(m, R, Z, err);
+            absNew = d2[1];
             Axpby(m, 1.0, Z, absNew / absOld, P);
             i++;
         }
