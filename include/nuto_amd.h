@@ -237,7 +237,9 @@ typedef struct nuto_solve_info
  * diagonal), x = C u with (C^T K C) u = C^T f. K: values in the context's pattern (as
  * nuto_build_matrix writes them), f and x: length n_rows; device arrays. A = C^T K C is applied
  * as three sparse products, never formed; K stays on the device. options NULL: BiCGSTAB,
- * machine-epsilon tolerance, 2 n iterations. info may be NULL. */
+ * machine-epsilon tolerance, 2 n iterations. info may be NULL. The solver's vectors, about
+ * 4 n_rows + 11 n_independent doubles of device memory, are allocated by the context's first solve
+ * and kept for the next ones (grown when needed) until nuto_destroy. */
 int nuto_solve(nuto_ctx* ctx, const double* K, const double* f, double* x, const nuto_solver_options* options,
                nuto_solve_info* info);
 
