@@ -190,8 +190,8 @@ __global__ void __launch_bounds__(256) kBicgP(long long n, const double* __restr
         p[i] = r[i] + beta * (p[i] - w * v[i]);
 }
 
-// Two dot products in one pass: partial[b] = (a . b2) over block b's elements, partial[gridDim + b] =
-// (c . d) (same element order and tree as kDotPartial)
+// Block sums of two per-lane values with kDotPartial's tree: partial[blockIdx] = the block's sum of
+// s0, partial[gridDim + blockIdx] = its sum of s1
 __device__ inline void BlockTree2(double s0, double s1, double* __restrict__ partial)
 {
     __shared__ double sh[2][256];
@@ -214,6 +214,7 @@ __device__ inline void BlockTree2(double s0, double s1, double* __restrict__ par
     }
 }
 
+// Two dot products in one pass, (a . b) and (c . d), in kDotPartial's element order
 __global__ void __launch_bounds__(256) kDot2Partial(long long n, const double* __restrict__ a,
                                                     const double* __restrict__ b, const double* __restrict__ c,
                                                     const double* __restrict__ d, double* __restrict__ partial)
@@ -323,15 +324,15 @@ struct Csr
     const double* val = nullptr;
 };
 
-// Krylov solver on the operator C^T K C. Its vectors live in a workspace the caller keeps between
-// solves; the stream orders every launch, and the host reads the scalars it branches on (one small
-// copy per reduction or pair of reductions).
 // node structure of an assembled K (rows per node of field 0 and 1), for the node product
 struct NodeRows
 {
     int nNodes = 0, nU = 0, c0 = 0, c1 = 0;
 };
 
+// Krylov solver on the operator C^T K C. Its vectors live in a workspace the caller keeps between
+// solves; the stream orders every launch, and the host reads the scalars it branches on (one small
+// copy per reduction or pair of reductions).
 class ReducedSolver
 {
 public:
