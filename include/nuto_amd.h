@@ -131,7 +131,8 @@ int nuto_set_material_fields(nuto_ctx* ctx, const double* E, const double* ft, c
 int nuto_set_material_fields_device(nuto_ctx* ctx, const double* E, const double* ft, const double* gf);
 
 /* Sparsity of BuildMatrix's result (setFromTriplets pattern: every dof pair of every cell, explicit
- * zeros kept; SimpleAssembler.cpp:100-156), built once and kept on the device. */
+ * zeros kept; SimpleAssembler.cpp:100-156), built once and kept on the device. The host part runs on
+ * up to 16 threads for meshes of 16,384 dof nodes or more (the calling thread waits for them). */
 int nuto_build_pattern(nuto_ctx* ctx, long long* nnz, int* n_rows);
 int nuto_copy_pattern(nuto_ctx* ctx, int* row_ptr, int* col);
 int nuto_pattern_device(nuto_ctx* ctx, const int** row_ptr, const int** col);
@@ -335,7 +336,8 @@ int nuto_batch_set_group_limit(nuto_batch* batch, long long max_dof_nodes);
  * group at once. Default: the whole group when it can be allocated, else 1 GiB; nuto_batch_autotune
  * chooses per group among the whole group, 1 GiB, 256 MiB and 64 MiB. After nuto_batch_build. */
 int nuto_batch_set_stage_bytes(nuto_batch* batch, long long bytes);
-int nuto_batch_build(nuto_batch* batch); /* patterns of all meshes, once */
+/* patterns of all meshes, once (host work on up to 16 threads, as nuto_build_pattern) */
+int nuto_batch_build(nuto_batch* batch);
 /* sizes[6]: lengths of u, eeq, R, K, history and d_theta */
 int nuto_batch_sizes(nuto_batch* batch, long long* sizes);
 int nuto_batch_mesh_info(nuto_batch* batch, int mesh, long long* info /* 17 */);
