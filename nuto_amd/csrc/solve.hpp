@@ -181,13 +181,38 @@ __global__ void __launch_bounds__(256) kJacobi(long long n, const double* __rest
         z[i] = invd[i] * x[i];
 }
 
-// BiCGSTAB's p = r + beta (p - w v)
-__global__ void __launch_bounds__(256) kBicgP(long long n, const double* __restrict__ r, double beta, double w,
-                                              const double* __restrict__ v, double* __restrict__ p)
+// BiCGSTAB's p = r + beta (p - w v) and y = invd p (the preconditioned p)
+__global__ void __launch_bounds__(256) kBicgPY(long long n, const double* __restrict__ r, double beta, double w,
+                                               const double* __restrict__ v, const double* __restrict__ invd,
+                                               double* __restrict__ p, double* __restrict__ y)
 {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i < n)
-        p[i] = r[i] + beta * (p[i] - w * v[i]);
+    {
+        const double pi = r[i] + beta * (p[i] - w * v[i]);
+        p[i] = pi;
+        y[i] = invd[i] * pi;
+    }
+}
+
+// scal[2] = alpha = rho / (r0 . v) from kDotPartial's np partials (kDotFinal's tree)
+__global__ void __launch_bounds__(256) kBicgAlpha(int np, const double* __restrict__ partial, double rho,
+                                                  double* __restrict__ scal)
+{
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += 256)
+        s += partial[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int m = 128; m > 0; m >>= 1)
+    {
+        if ((int)threadIdx.x < m)
+            sh[threadIdx.x] += sh[threadIdx.x + m];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        scal[2] = rho / sh[0];
 }
 
 // Block sums of two per-lane values with kDotPartial's tree: partial[blockIdx] = the block's sum of
@@ -228,7 +253,9 @@ __global__ void __launch_bounds__(256) kDot2Partial(long long n, const double* _
     BlockTree2(s0, s1, partial);
 }
 
-// out[0], out[1] = the sums of the two sets of np partials (one block of 512 threads)
+// out[0], out[1] = the sums of the two sets of np partials (one block of 512 threads); OMEGA: also
+// out[3] = out[1] / out[0] when out[0] > 0, else 0 (BiCGSTAB's w = (t . s) / (t . t))
+template <bool OMEGA>
 __global__ void __launch_bounds__(512) kDot2Final(int np, const double* __restrict__ partial, double* __restrict__ out)
 {
     __shared__ double sh[2][256];
@@ -246,14 +273,21 @@ __global__ void __launch_bounds__(512) kDot2Final(int np, const double* __restri
     }
     if (t == 0)
         out[h] = sh[h][0];
+    if constexpr (OMEGA)
+    {
+        __syncthreads();
+        if (threadIdx.x == 0)
+            out[3] = sh[0][0] > 0.0 ? sh[1][0] / sh[0][0] : 0.0;
+    }
 }
 
-// BiCGSTAB's s = r - alpha v and z = invd s (the preconditioned s)
-__global__ void __launch_bounds__(256) kBicgS(long long n, const double* __restrict__ r, double alpha,
+// BiCGSTAB's s = r - alpha v and z = invd s (the preconditioned s); alpha = scal[2]
+__global__ void __launch_bounds__(256) kBicgS(long long n, const double* __restrict__ r, const double* __restrict__ scal,
                                               const double* __restrict__ v, const double* __restrict__ invd,
                                               double* __restrict__ s, double* __restrict__ z)
 {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    const double alpha = scal[2];
     if (i < n)
     {
         const double si = -alpha * v[i] + r[i];
@@ -263,13 +297,15 @@ __global__ void __launch_bounds__(256) kBicgS(long long n, const double* __restr
 }
 
 // BiCGSTAB's update x += alpha y, x += w z, r = s - w t, with the partial sums of (r . r) and
-// (r0 . r) of the new residual (kDot2Partial's element order and layout; grid-stride over n)
-__global__ void __launch_bounds__(256) kBicgUpdate(long long n, double alpha, const double* __restrict__ y, double w,
-                                                   const double* __restrict__ z, double* __restrict__ x,
-                                                   const double* __restrict__ s, const double* __restrict__ t,
-                                                   double* __restrict__ r, const double* __restrict__ r0,
-                                                   double* __restrict__ partial)
+// (r0 . r) of the new residual (kDot2Partial's element order and layout; grid-stride over n);
+// alpha = scal[2], w = scal[3]
+__global__ void __launch_bounds__(256) kBicgUpdate(long long n, const double* __restrict__ scal,
+                                                   const double* __restrict__ y, const double* __restrict__ z,
+                                                   double* __restrict__ x, const double* __restrict__ s,
+                                                   const double* __restrict__ t, double* __restrict__ r,
+                                                   const double* __restrict__ r0, double* __restrict__ partial)
 {
+    const double alpha = scal[2], w = scal[3];
     double s0 = 0.0, s1 = 0.0;
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
     {
@@ -429,7 +465,7 @@ public:
     // the two sums of the 2 x g partials in `part`
     void Dot2Read(int g, double out[2], hipError_t& err)
     {
-        hipLaunchKernelGGL(kDot2Final, dim3(1), dim3(512), 0, mS, g, part, scal);
+        hipLaunchKernelGGL(kDot2Final<false>, dim3(1), dim3(512), 0, mS, g, part, scal);
         if ((err = hipGetLastError()) != hipSuccess ||
             (err = hipMemcpyAsync(mHost, scal, 2 * sizeof(double), hipMemcpyDeviceToHost, mS)) != hipSuccess ||
             (err = hipStreamSynchronize(mS)) != hipSuccess)
@@ -547,23 +583,29 @@ public:
                     i = 0;
             }
             const double beta = (rho / rhoOld) * (alpha / w);
-            hipLaunchKernelGGL(kBicgP, dim3(vg), dim3(256), 0, mS, m, R, beta, w, V, P);
-            Precond(m, invd, P, Y);
+            hipLaunchKernelGGL(kBicgPY, dim3(vg), dim3(256), 0, mS, m, R, beta, w, V, invd, P, Y);
             if ((err = Apply(Y, V)) != hipSuccess)
                 break;
-            alpha = rho / Dot(m, R0, V, err);
-            hipLaunchKernelGGL(kBicgS, dim3(vg), dim3(256), 0, mS, m, R, alpha, V, invd, S, Z);
+            // alpha = rho / (r0 . v), w = (t . s) / (t . t) on the device; the host reads them with
+            // (r . r) and the next rho at the end of the iteration
+            const int g = DotGrid(m);
+            hipLaunchKernelGGL(kDotPartial, dim3(g), dim3(256), 0, mS, m, R0, V, part);
+            hipLaunchKernelGGL(kBicgAlpha, dim3(1), dim3(256), 0, mS, g, part, rho, scal);
+            hipLaunchKernelGGL(kBicgS, dim3(vg), dim3(256), 0, mS, m, R, scal, V, invd, S, Z);
             if ((err = Apply(Z, T)) != hipSuccess)
                 break;
-            double d2[2];
-            Dot2(m, T, T, T, S, d2, err);
-            w = d2[0] > 0.0 ? d2[1] / d2[0] : 0.0;
-            const int g = DotGrid(m);
-            hipLaunchKernelGGL(kBicgUpdate, dim3(g), dim3(256), 0, mS, m, alpha, Y, w, Z, x, S, T, R, R0,
-                               part);
-            Dot2Read(g, d2, err);
-            rr = d2[0];
-            rhoNext = d2[1];
+            hipLaunchKernelGGL(kDot2Partial, dim3(g), dim3(256), 0, mS, m, T, T, T, S, part);
+            hipLaunchKernelGGL(kDot2Final<true>, dim3(1), dim3(512), 0, mS, g, part, scal);
+            hipLaunchKernelGGL(kBicgUpdate, dim3(g), dim3(256), 0, mS, m, scal, Y, Z, x, S, T, R, R0, part);
+            hipLaunchKernelGGL(kDot2Final<false>, dim3(1), dim3(512), 0, mS, g, part, scal);
+            if ((err = hipGetLastError()) != hipSuccess ||
+                (err = hipMemcpyAsync(mHost, scal, 4 * sizeof(double), hipMemcpyDeviceToHost, mS)) != hipSuccess ||
+                (err = hipStreamSynchronize(mS)) != hipSuccess)
+                break;
+            rr = mHost[0];
+            rhoNext = mHost[1];
+            alpha = mHost[2];
+            w = mHost[3];
             ++i;
         }
         iters = i;
