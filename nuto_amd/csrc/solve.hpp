@@ -195,8 +195,9 @@ __global__ void __launch_bounds__(256) kBicgPY(long long n, const double* __rest
     }
 }
 
-// scal[2] = alpha = rho / (r0 . v) from kDotPartial's np partials (kDotFinal's tree)
-__global__ void __launch_bounds__(256) kBicgAlpha(int np, const double* __restrict__ partial, double rho,
+// scal[2] = num / (the dot product of kDotPartial's np partials, kDotFinal's tree): BiCGSTAB's
+// alpha = rho / (r0 . v), conjugate gradients' alpha = (r . z) / (p . q)
+__global__ void __launch_bounds__(256) kBicgAlpha(int np, const double* __restrict__ partial, double num,
                                                   double* __restrict__ scal)
 {
     __shared__ double sh[256];
@@ -212,7 +213,7 @@ __global__ void __launch_bounds__(256) kBicgAlpha(int np, const double* __restri
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        scal[2] = rho / sh[0];
+        scal[2] = num / sh[0];
 }
 
 // Block sums of two per-lane values with kDotPartial's tree: partial[blockIdx] = the block's sum of
@@ -321,12 +322,14 @@ __global__ void __launch_bounds__(256) kBicgUpdate(long long n, const double* __
 }
 
 // Conjugate gradients' update x += alpha p, r -= alpha q, z = invd r, with the partial sums of
-// (r . r) and (r . z) (kDot2Partial's element order and layout; grid-stride over n)
-__global__ void __launch_bounds__(256) kCgUpdate(long long n, double alpha, const double* __restrict__ p,
-                                                 const double* __restrict__ q, double* __restrict__ x,
-                                                 double* __restrict__ r, const double* __restrict__ invd,
-                                                 double* __restrict__ z, double* __restrict__ partial)
+// (r . r) and (r . z) (kDot2Partial's element order and layout; grid-stride over n); alpha = scal[2]
+__global__ void __launch_bounds__(256) kCgUpdate(long long n, const double* __restrict__ scal,
+                                                 const double* __restrict__ p, const double* __restrict__ q,
+                                                 double* __restrict__ x, double* __restrict__ r,
+                                                 const double* __restrict__ invd, double* __restrict__ z,
+                                                 double* __restrict__ partial)
 {
+    const double alpha = scal[2];
     double s0 = 0.0, s1 = 0.0;
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
     {
@@ -519,10 +522,12 @@ public:
         {
             if ((err = Apply(P, Q)) != hipSuccess)
                 break;
-            const double alpha = absNew / Dot(m, P, Q, err);
-            // x, r, z = invd r and the pair (r . r), (r . z) in one pass, one host read
+            // alpha = |r|_M^2 / (p . q) on the device (scal[2]); then x, r, z = invd r and the pair
+            // (r . r), (r . z) in one pass, one host read
             const int g = DotGrid(m);
-            hipLaunchKernelGGL(kCgUpdate, dim3(g), dim3(256), 0, mS, m, alpha, P, Q, x, R, invd, Z, part);
+            hipLaunchKernelGGL(kDotPartial, dim3(g), dim3(256), 0, mS, m, P, Q, part);
+            hipLaunchKernelGGL(kBicgAlpha, dim3(1), dim3(256), 0, mS, g, part, absNew, scal);
+            hipLaunchKernelGGL(kCgUpdate, dim3(g), dim3(256), 0, mS, m, scal, P, Q, x, R, invd, Z, part);
             double d2[2];
             Dot2Read(g, d2, err);
             rr = d2[0];
